@@ -1,0 +1,328 @@
+"""bhfluid: MI355X-native GRMHD fluid step, a drop-in for the fluid path of
+Fjoerde/black-hole-simulator (src/fluid).
+
+The reference exposes this path as C++ classes; this package mirrors them:
+
+=========================  ==========================================================
+bhfluid                    reference
+=========================  ==========================================================
+``AmrTree``                ``grid::amrtree`` (src/fluid/grid.hpp:109-144)
+``init.fm_init``           ``torus::init::fm_init`` (solver/initconds.cpp:228-591)
+``init.B_pot_init``        ``torus::init::B_pot_init`` over every patch (main.cpp:142-145)
+``RK2Integrator``          ``integ::rk2integrator`` (src/fluid/rk2.hpp:19-30)
+``RK2Integrator.step``     ``rk2integrator::step`` (solver/integrator.cpp:262-275)
+``DomainError``            ``std::domain_error`` thrown by the EOS (physics/thermo.cpp:11)
+=========================  ==========================================================
+
+The step runs on the GPU through the C ABI in include/bhfluid.h
+(``libbhfluid.so``, built in-tree by ``make -C black-hole-simulator_amd``).
+There is no CPU fallback: without the library, or without a GPU, the step
+raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+from typing import Optional, Sequence, Tuple
+
+import numpy as np
+
+__all__ = ["AmrTree", "RK2Integrator", "DomainError", "BhError", "init", "torus_setup", "library_path",
+           "NCELL", "NFACE", "BLOCK", "GHOST"]
+
+BLOCK = 16
+GHOST = 2
+CPN = BLOCK + 2 * GHOST
+NCELL = CPN ** 3
+NFACE = (CPN + 1) * CPN * CPN
+
+_HERE = Path(__file__).resolve().parent
+_LIB_PATH = _HERE / "libbhfluid.so"
+_lib = None
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_d = ctypes.c_double
+_i = ctypes.c_int
+_v = ctypes.c_void_p
+
+
+class BhError(RuntimeError):
+    """A call into libbhfluid failed (bad arguments, device or state error)."""
+
+
+class DomainError(ValueError):
+    """The equation of state saw rho <= 0.
+
+    The reference throws ``std::domain_error`` at that point
+    (src/fluid/physics/thermo.cpp:11) and the process terminates.  Here the
+    step completes, the fault is counted, and this exception is raised
+    afterwards; ``faults`` holds the count.
+    """
+
+    def __init__(self, msg, faults):
+        super().__init__(msg)
+        self.faults = faults
+
+
+def library_path() -> Path:
+    return _LIB_PATH
+
+
+def _sig(L, name, res, args):
+    f = getattr(L, name)
+    f.restype = res
+    f.argtypes = args
+
+
+def lib():
+    """Load libbhfluid.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _LIB_PATH.exists():
+            raise BhError(f"{_LIB_PATH} is missing: build it with `make -C black-hole-simulator_amd` "
+                          "(or __graft_entry__.build())")
+        L = ctypes.CDLL(str(_LIB_PATH))
+        _sig(L, "bh_tree_create", _i, [_dp, _dp, _i, _d, _d, _d, _d, ctypes.POINTER(_v)])
+        _sig(L, "bh_tree_destroy", None, [_v])
+        _sig(L, "bh_last_error", ctypes.c_char_p, [_v])
+        _sig(L, "bh_tree_fm_init", _i, [_v])
+        _sig(L, "bh_tree_rho_max", _i, [_v, _dp])
+        _sig(L, "bh_tree_B_pot_init", _i, [_v, _d])
+        _sig(L, "bh_tree_ptoc_interior", _i, [_v])
+        _sig(L, "bh_tree_regrid", _i, [_v])
+        _sig(L, "bh_tree_nblocks", _i, [_v, _ip])
+        _sig(L, "bh_tree_get_blocks", _i, [_v, _ip, _dp])
+        _sig(L, "bh_tree_get_state", _i, [_v, _dp, _dp, _dp])
+        _sig(L, "bh_tree_set_state", _i, [_v, _dp, _dp, _dp])
+        _sig(L, "bh_rk2_step", _i, [_v, _d, _dp])
+        _sig(L, "bh_rk2_dtcomp", _i, [_v, _d, _dp])
+        _sig(L, "bh_rk2_rkstg", _i, [_v, _d, _i])
+        _sig(L, "bh_tree_floors", _i, [_v])
+        _sig(L, "bh_rk2_advance", _i, [_v, _d, _i, _d, _dp, _ip])
+        _sig(L, "bh_tree_take_faults", _i, [_v, ctypes.POINTER(ctypes.c_longlong)])
+        _sig(L, "bh_tree_maxdivB", _i, [_v, _i, _dp])
+        _sig(L, "bh_ptoc_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _dp, _dp])
+        _sig(L, "bh_ctop_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _dp, _dp, _ip])
+        _sig(L, "bh_hlld_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp])
+        _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
+        _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    if a is None:
+        return None
+    if a.dtype != np.float64 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous float64 array")
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    if a.dtype != np.int32 or not a.flags["C_CONTIGUOUS"]:
+        raise TypeError("expected a C-contiguous int32 array")
+    return a.ctypes.data_as(_ip)
+
+
+class Metric:
+    """Black-hole parameters (``metric``, src/fluid/metric.hpp:23-31)."""
+
+    def __init__(self, M=1.0, a=0.0, Q=0.0):
+        self.M, self.a, self.Q = float(M), float(a), float(Q)
+
+    def r_horizon(self):
+        return self.M + np.sqrt(self.M * self.M - self.a * self.a - self.Q * self.Q)
+
+
+class State:
+    """Ideal-gas EOS parameter (``state``, src/fluid/state.hpp:9-20)."""
+
+    def __init__(self, gm=5.0 / 3.0):
+        self.gamma = float(gm)
+
+
+class AmrTree:
+    """Block-structured tree of 16^3 patches with two ghost layers.
+
+    Mirrors ``grid::amrtree(dom_l, dom_h, nqlt, M, a, Q, gm)``
+    (src/fluid/grid/grid.cpp:177-206): ``nqlt**3`` root patches tiling the box,
+    every cell at the floor state.  The state lives on the GPU while it is
+    being stepped and is copied back on demand (``state()``).
+    """
+
+    def __init__(self, dom_l: Sequence[float], dom_h: Sequence[float], nqlt: int, M: float = 1.0, a: float = 0.0,
+                 Q: float = 0.0, gm: float = 5.0 / 3.0):
+        L = lib()
+        lo = np.ascontiguousarray(dom_l, dtype=np.float64)
+        hi = np.ascontiguousarray(dom_h, dtype=np.float64)
+        if lo.shape != (3,) or hi.shape != (3,) or int(nqlt) < 1:
+            raise ValueError("dom_l and dom_h need three coordinates and nqlt >= 1")
+        h = _v()
+        rc = L.bh_tree_create(_p(lo), _p(hi), int(nqlt), float(M), float(a), float(Q), float(gm), ctypes.byref(h))
+        if rc != 0:
+            raise BhError(L.bh_last_error(None).decode())
+        self._h = h
+        self._L = L
+        self.mtr = Metric(M, a, Q)
+        self.stt = State(gm)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._L.bh_tree_destroy(h)
+            self._h = None
+
+    # -- error handling -------------------------------------------------
+    def _check(self, rc, what):
+        if rc == 0:
+            return
+        if rc == 1:
+            n = ctypes.c_longlong(0)
+            self._L.bh_tree_take_faults(self._h, ctypes.byref(n))
+            raise DomainError(f"{what}: Thermodynamic calculations threw back an error: the density is not "
+                              f"positive! ({n.value} equation-of-state calls)", n.value)
+        raise BhError(f"{what}: {self._L.bh_last_error(self._h).decode()} (code {rc})")
+
+    # -- structure --------------------------------------------------------
+    @property
+    def nblocks(self) -> int:
+        n = ctypes.c_int(0)
+        self._L.bh_tree_nblocks(self._h, ctypes.byref(n))
+        return n.value
+
+    def blocks(self) -> Tuple[np.ndarray, np.ndarray]:
+        """Block table: meta[nb, 14] (lvl, i, j, k, leaf, parent, children[8])
+        and edges[nb, 3, 17] (cell-edge coordinates), in quilt order."""
+        nb = self.nblocks
+        meta = np.zeros((nb, 14), np.int32)
+        edges = np.zeros((nb, 3, 17), np.float64)
+        self._L.bh_tree_get_blocks(self._h, _pi(meta), _p(edges))
+        return meta, edges
+
+    @property
+    def leaves(self) -> np.ndarray:
+        meta, _ = self.blocks()
+        return np.flatnonzero(meta[:, 4])
+
+    def state(self):
+        """Copy of (U, W, Bf): U, W [nb, 8, 8000], Bf [nb, 3, 8400]."""
+        nb = self.nblocks
+        U = np.zeros((nb, 8, NCELL))
+        W = np.zeros((nb, 8, NCELL))
+        Bf = np.zeros((nb, 3, NFACE))
+        self._check(self._L.bh_tree_get_state(self._h, _p(U), _p(W), _p(Bf)), "get_state")
+        return U, W, Bf
+
+    def set_state(self, U=None, W=None, Bf=None):
+        nb = self.nblocks
+        args = []
+        for a, shp in ((U, (nb, 8, NCELL)), (W, (nb, 8, NCELL)), (Bf, (nb, 3, NFACE))):
+            if a is None:
+                args.append(None)
+                continue
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            if a.shape != shp:
+                raise ValueError(f"expected shape {shp}, got {a.shape}")
+            args.append(a)
+        self._check(self._L.bh_tree_set_state(self._h, *[_p(a) for a in args]), "set_state")
+
+    # -- host-side tree operations ---------------------------------------
+    def rho_max(self) -> float:
+        """Maximum rho over active zones (src/fluid/main.cpp:101-111)."""
+        out = ctypes.c_double(0.0)
+        self._check(self._L.bh_tree_rho_max(self._h, ctypes.byref(out)), "rho_max")
+        return out.value
+
+    def ptoc_interior(self):
+        """U = ptoc(W) on active zones (src/fluid/main.cpp:148-159)."""
+        self._check(self._L.bh_tree_ptoc_interior(self._h), "ptoc_interior")
+
+    def regrid(self):
+        """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""
+        self._check(self._L.bh_tree_regrid(self._h), "regrid")
+
+    def floors(self):
+        """``patch::floors`` on every patch (grid.cpp:331-358)."""
+        self._check(self._L.bh_tree_floors(self._h), "floors")
+
+    def maxdivB(self, b: int) -> float:
+        """``constrans::maxdivB`` of block b (grid/transport.cpp:77-89)."""
+        out = ctypes.c_double(0.0)
+        self._check(self._L.bh_tree_maxdivB(self._h, int(b), ctypes.byref(out)), "maxdivB")
+        return out.value
+
+
+class init:
+    """``torus::init`` (src/fluid/initial.hpp:17-22)."""
+
+    @staticmethod
+    def fm_init(tree: AmrTree) -> bool:
+        """Fishbone-Moncrief torus on active zones.  Returns False where the
+        reference prints an error and returns without a torus."""
+        rc = tree._L.bh_tree_fm_init(tree._h)
+        if rc == 2:
+            return False
+        tree._check(rc, "fm_init")
+        return True
+
+    @staticmethod
+    def B_pot_init(tree: AmrTree, rho_max: float):
+        """Face fields from the curl of A_phi on every patch, in quilt order."""
+        tree._check(tree._L.bh_tree_B_pot_init(tree._h, float(rho_max)), "B_pot_init")
+
+
+def torus_setup(tree: AmrTree, regrid: bool = False) -> float:
+    """The initialisation sequence of src/fluid/main.cpp:98-168: torus, global
+    rho_max, magnetic field, conserved variables, optional regrid.  Returns
+    rho_max."""
+    init.fm_init(tree)  # main.cpp:98 ignores a torus that could not be placed
+    rho_max = tree.rho_max()
+    if rho_max < 1e-14:
+        raise BhError("rho_max is zero after the torus initialisation")
+    init.B_pot_init(tree, rho_max)
+    tree.ptoc_interior()
+    if regrid:
+        tree.regrid()
+    return rho_max
+
+
+class RK2Integrator:
+    """``integ::rk2integrator`` (src/fluid/rk2.hpp:19-30)."""
+
+    def __init__(self, cfl: float = 0.4):
+        self.cfl = float(cfl)
+
+    def step(self, tree: AmrTree) -> float:
+        """One RK2 step on the GPU: dtcomp, two stages, floors.  Returns dt."""
+        dt = ctypes.c_double(0.0)
+        tree._check(tree._L.bh_rk2_step(tree._h, self.cfl, ctypes.byref(dt)), "step")
+        return dt.value
+
+    @staticmethod
+    def dtcomp(tree: AmrTree, cfl: float) -> float:
+        dt = ctypes.c_double(0.0)
+        tree._check(tree._L.bh_rk2_dtcomp(tree._h, float(cfl), ctypes.byref(dt)), "dtcomp")
+        return dt.value
+
+    def rkstg(self, tree: AmrTree, dt: float, stg: int):
+        tree._check(tree._L.bh_rk2_rkstg(tree._h, float(dt), int(stg)), "rkstg")
+
+    def advance(self, tree: AmrTree, nsteps: Optional[int] = None, t_end: Optional[float] = None,
+                t: float = 0.0) -> Tuple[float, int]:
+        """The time loop of src/fluid/main.cpp:169-173: step while t < t_end
+        and fewer than nsteps steps were taken.  Returns (t, steps)."""
+        if nsteps is None and t_end is None:
+            raise ValueError("give nsteps or t_end")
+        tt = ctypes.c_double(float(t))
+        n = ctypes.c_int(0)
+        rc = tree._L.bh_rk2_advance(tree._h, self.cfl, int(nsteps if nsteps is not None else 2 ** 31 - 1),
+                                    float(t_end if t_end is not None else np.inf), ctypes.byref(tt), ctypes.byref(n))
+        tree._check(rc, "advance")
+        return tt.value, n.value
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    lib().bh_device_info(buf, 256)
+    return buf.value.decode()

@@ -1,0 +1,1113 @@
+// gfx950 kernels for one RK2 step of the GRMHD fluid solver, the device-side
+// tree, and the C ABI declared in include/bhfluid.h.
+//
+// One launch per phase of the reference's rk2integrator::rkstg
+// (src/fluid/solver/integrator.cpp:191-260), each over every block of the
+// tree at once: ghost fill (in ordered waves), con2prim, PLM+HLLD fluxes,
+// EMF average, face-field update, face-to-cell field average, conserved
+// update; plus the CFL reduction and the floors pass.  State is SoA per
+// block in HBM; all static geometry is precomputed.  dt stays on the device
+// between launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "bh_common.hpp"
+#include "bh_crmath.hpp"
+#include "bh_physics.hpp"
+#include "bh_tree.hpp"
+#include "../../include/bhfluid.h"
+
+namespace bh {
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) throw DevError(std::string(#x) + ": " + hipGetErrorString(e_));      \
+    } while (0)
+
+struct DevError {
+    std::string msg;
+    explicit DevError(std::string m) : msg(std::move(m)) {}
+};
+
+constexpr int TPB = 256;
+constexpr int WG_GHOST = (NGHOST + TPB - 1) / TPB;   // 16
+constexpr int WG_CELL = (NCELL + TPB - 1) / TPB;     // 32
+constexpr int WG_FACE = (3 * NFC + TPB - 1) / TPB;   // 51
+constexpr int WG_EDGE = (19 * 19 * 19 + TPB - 1) / TPB;  // 27
+constexpr int WG_BF = (18 * 18 * 18 + TPB - 1) / TPB;    // 23
+constexpr int WG_INT = NINT / TPB;                   // 16
+
+// Kernel view of the device tree.
+struct KP {
+    double gamma;
+    int limiter;
+    double* U;           // [nb][8][8000]
+    double* W;           // [nb][8][8000]
+    double* Bf;          // [nb][3][8400]
+    double* Us;          // [nb][5][4096] RK store of D, S, tau
+    double* F;           // [nb][3][8][8400] fluxes, zero outside computed faces
+    double* E;           // [nb][3][8820] EMFs
+    const double* gct;   // [nb][8][8000]
+    const double* gcc;   // [nb][11][8000]
+    const double* gfl;   // [nb][2][8000]
+    const double* gfc;   // [nb][3][8][4352]
+    const double* gchr;  // [nb][40][4096]
+    const double* dxyz;  // [nb][3]
+    const int* ghtab;    // [nb][12]
+    const short* ghost_cells;  // [3904]
+    unsigned long long* dtbits;  // dt as an ordered bit pattern
+    unsigned long long* faults;
+};
+
+__device__ __forceinline__ double& Ua(const KP& P, int b, int v, int c) { return P.U[((size_t)b * 8 + v) * NCELL + c]; }
+__device__ __forceinline__ double& Wa(const KP& P, int b, int v, int c) { return P.W[((size_t)b * 8 + v) * NCELL + c]; }
+
+__device__ __forceinline__ Cons loadU(const KP& P, int b, int c) {
+    Cons q;
+    q.D = Ua(P, b, UD, c);
+    q.S[0] = Ua(P, b, US0, c); q.S[1] = Ua(P, b, US1, c); q.S[2] = Ua(P, b, US2, c);
+    q.tau = Ua(P, b, UTAU, c);
+    q.B[0] = Ua(P, b, UB0, c); q.B[1] = Ua(P, b, UB1, c); q.B[2] = Ua(P, b, UB2, c);
+    return q;
+}
+__device__ __forceinline__ void storeU(const KP& P, int b, int c, const Cons& q) {
+    Ua(P, b, UD, c) = q.D;
+    Ua(P, b, US0, c) = q.S[0]; Ua(P, b, US1, c) = q.S[1]; Ua(P, b, US2, c) = q.S[2];
+    Ua(P, b, UTAU, c) = q.tau;
+    Ua(P, b, UB0, c) = q.B[0]; Ua(P, b, UB1, c) = q.B[1]; Ua(P, b, UB2, c) = q.B[2];
+}
+__device__ __forceinline__ Prim loadW(const KP& P, int b, int c) {
+    Prim w;
+    w.rho = Wa(P, b, WRHO, c); w.eps = Wa(P, b, WEPS, c);
+    for (int d = 0; d < 3; d++) { w.v[d] = Wa(P, b, WV0 + d, c); w.B[d] = Wa(P, b, WB0 + d, c); }
+    return w;
+}
+__device__ __forceinline__ void storeW(const KP& P, int b, int c, const Prim& w) {
+    Wa(P, b, WRHO, c) = w.rho; Wa(P, b, WEPS, c) = w.eps;
+    for (int d = 0; d < 3; d++) { Wa(P, b, WV0 + d, c) = w.v[d]; Wa(P, b, WB0 + d, c) = w.B[d]; }
+}
+// tree metric at a cell centre
+__device__ __forceinline__ GPt cell_tree(const KP& P, int b, int c) {
+    const double* g = P.gct + (size_t)b * NGCT * NCELL + c;
+    GPt m;
+    m.sqrtdetg = 0.0; m.beta0 = 0.0;
+    m.sqrtdetgam = g[0]; m.alpha = g[NCELL]; m.g11 = g[2 * NCELL]; m.g13 = g[3 * NCELL];
+    m.g22 = g[4 * NCELL]; m.g33 = g[5 * NCELL]; m.gi13 = g[6 * NCELL]; m.gi33 = g[7 * NCELL];
+    return m;
+}
+__device__ __forceinline__ double cell_sqrtdetg(const KP& P, int b, int c) {
+    return P.gcc[(size_t)b * NGCC * NCELL + c];
+}
+__device__ __forceinline__ double gcc_at(const KP& P, int b, int v, int c) {
+    return P.gcc[((size_t)b * NGCC + v) * NCELL + c];
+}
+__device__ __forceinline__ void add_faults(const KP& P, int nf) {
+    if (nf) atomicAdd(P.faults, (unsigned long long)nf);
+}
+
+// con2prim of one cell with the floor fallback of the c2p phase
+// (integrator.cpp:205-220): floor state keeps the cell's B, U recomputed.
+__device__ __forceinline__ void c2p_cell(const KP& P, int b, int c, int& nf) {
+    GPt m = cell_tree(P, b, c);
+    Prim pv;
+    if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
+        storeW(P, b, c, pv);
+    } else {
+        const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+        Prim f = floor_prim(fl[c], fl[NCELL + c]);
+        f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
+        storeW(P, b, c, f);
+        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+    }
+}
+
+__device__ __forceinline__ int cell_of(int dim, int a, int t1, int t2) {
+    return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
+}
+
+// ---------------------------------------------------------------- ghosts
+// amrtree::ghosts for the leaves of one wave, grid.cpp:390-571: each thread
+// fills one ghost cell from its face neighbour (copy, coarse prolongation or
+// domain-boundary mirror; edge and corner ghosts are not filled, quirk Q27),
+// then runs the ghost con2prim (failure keeps v and B, quirk Q29).
+__global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blocks) {
+    const int b = blocks[blockIdx.x / WG_GHOST];
+    const int t = (blockIdx.x % WG_GHOST) * TPB + threadIdx.x;
+    if (t >= NGHOST) return;
+    const int c = P.ghost_cells[t];
+    const int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
+    int nf = 0;
+    const int oi = (i < 0 || i >= BLK), oj = (j < 0 || j >= BLK), ok = (k < 0 || k >= BLK);
+    if (oi + oj + ok == 1) {
+        const int dim = oi ? 0 : (oj ? 1 : 2);
+        const int a = dim == 0 ? i : (dim == 1 ? j : k);
+        const int s = a < 0 ? 0 : 1;
+        const int g = s ? a - BLK : a + GH;
+        const int t1 = dim == 0 ? j : i;
+        const int t2 = dim == 2 ? j : k;
+        const int kind = P.ghtab[b * 12 + (2 * dim + s) * 2];
+        const int nb = P.ghtab[b * 12 + (2 * dim + s) * 2 + 1];
+        if (kind == GH_COPY || kind == GH_BNDY) {
+            int src_b, src_c;
+            if (kind == GH_COPY) { src_b = nb; src_c = cell_of(dim, (s ? 0 : BLK - GH) + g, t1, t2); }
+            else { src_b = b; src_c = cell_of(dim, s ? BLK - 1 - g : g, t1, t2); }
+            for (int v = 0; v < 8; v++) {
+                Ua(P, b, v, c) = Ua(P, src_b, v, src_c);
+                Wa(P, b, v, c) = Wa(P, src_b, v, src_c);
+            }
+        } else {
+            // gh_prolong, grid.cpp:427-492 (quirk Q25: coarse index ignores octant and side)
+            const int ci = dim == 0 ? g / 2 : t1 / 2;
+            const int cj = dim == 1 ? g / 2 : (dim == 0 ? t1 / 2 : t2 / 2);
+            const int ck = dim == 2 ? g / 2 : t2 / 2;
+            const double offg = (g % 2 == 0) ? -0.25 : 0.25;
+            const double offt1 = (t1 % 2 == 0) ? -0.25 : 0.25;
+            const double offt2 = (t2 % 2 == 0) ? -0.25 : 0.25;
+            double on, ot1, ot2;
+            if (dim == 0) { on = offg; ot1 = offt1; ot2 = offt2; }
+            else if (dim == 1) { on = offt1; ot1 = offg; ot2 = offt2; }
+            else { on = offt1; ot1 = offt2; ot2 = offg; }
+            const int cc = lindx(ci, cj, ck);
+            for (int v = 0; v < 8; v++) {
+                const double vc = Ua(P, nb, v, cc);
+                double sn = minmod(vc - Ua(P, nb, v, lindx(ci - 1, cj, ck)), Ua(P, nb, v, lindx(ci + 1, cj, ck)) - vc);
+                double st1 = minmod(vc - Ua(P, nb, v, lindx(ci, cj - 1, ck)), Ua(P, nb, v, lindx(ci, cj + 1, ck)) - vc);
+                double st2 = minmod(vc - Ua(P, nb, v, lindx(ci, cj, ck - 1)), Ua(P, nb, v, lindx(ci, cj, ck + 1)) - vc);
+                Ua(P, b, v, c) = vc + sn * on + st1 * ot1 + st2 * ot2;
+            }
+            c2p_cell(P, b, c, nf);
+        }
+    }
+    // ghost con2prim, grid.cpp:538-570
+    GPt m = cell_tree(P, b, c);
+    Prim pv;
+    if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
+        storeW(P, b, c, pv);
+        eos_press(P.gamma, pv.rho, pv.eps, nf);
+    } else {
+        const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+        Prim w = loadW(P, b, c);
+        w.rho = fl[c];
+        w.eps = fl[NCELL + c];
+        Wa(P, b, WRHO, c) = w.rho;
+        Wa(P, b, WEPS, c) = w.eps;
+        storeU(P, b, c, ptoc(m, P.gamma, w, nf));
+    }
+    add_faults(P, nf);
+}
+
+// con2prim on every stored cell of every leaf, integrator.cpp:201-222
+__global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leaves) {
+    const int b = leaves[blockIdx.x / WG_CELL];
+    const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
+    if (c >= NCELL) return;
+    int nf = 0;
+    c2p_cell(P, b, c, nf);
+    add_faults(P, nf);
+}
+
+// ---------------------------------------------------------------- fluxes
+// patch::fluxcomp, grid.cpp:255-329: PLM reconstruction (log rho, log eps,
+// linear v and transverse B; reconstruction.cpp:40-97), the normal field
+// taken from the face, ptoc of both states at the face and the HLLD flux.
+// quirk Q1: the Riemann solver is called with dim 0 for every direction.
+__global__ __launch_bounds__(TPB) void k_flux(KP P) {
+    const int b = blockIdx.x / WG_FACE;
+    const int f = (blockIdx.x % WG_FACE) * TPB + threadIdx.x;
+    if (f >= 3 * NFC) return;
+    const int dir = f / NFC;
+    const int lf = f % NFC;
+    int i, j, k;
+    if (dir == 0) { i = lf / 256 - 1; j = (lf / 16) % 16; k = lf % 16; }
+    else if (dir == 1) { i = lf / 272; j = (lf / 16) % 17 - 1; k = lf % 16; }
+    else { i = lf / 272; j = (lf / 17) % 16; k = lf % 17 - 1; }
+    const int stride = dir == 0 ? CPN * CPN : (dir == 1 ? CPN : 1);
+    const int c0 = lindx(i, j, k);
+    const int cm = c0 - stride, cp = c0 + stride, cpp = c0 + 2 * stride;
+    int nf = 0;
+    Prim L, R;
+    {
+        double fl, fr;
+        recon_scal(cr_log(Wa(P, b, WRHO, cm)), cr_log(Wa(P, b, WRHO, c0)), cr_log(Wa(P, b, WRHO, cp)),
+                   cr_log(Wa(P, b, WRHO, cpp)), P.limiter, fl, fr);
+        L.rho = cr_exp(fl); R.rho = cr_exp(fr);
+        recon_scal(cr_log(Wa(P, b, WEPS, cm)), cr_log(Wa(P, b, WEPS, c0)), cr_log(Wa(P, b, WEPS, cp)),
+                   cr_log(Wa(P, b, WEPS, cpp)), P.limiter, fl, fr);
+        L.eps = cr_exp(fl); R.eps = cr_exp(fr);
+        for (int d = 0; d < 3; d++) {
+            recon_scal(Wa(P, b, WV0 + d, cm), Wa(P, b, WV0 + d, c0), Wa(P, b, WV0 + d, cp), Wa(P, b, WV0 + d, cpp),
+                       P.limiter, fl, fr);
+            L.v[d] = fl; R.v[d] = fr;
+        }
+        causal_v(L.v, gcc_at(P, b, 3, c0), gcc_at(P, b, 4, c0), gcc_at(P, b, 5, c0));
+        causal_v(R.v, gcc_at(P, b, 3, cp), gcc_at(P, b, 4, cp), gcc_at(P, b, 5, cp));
+        for (int d = 0; d < 3; d++) {
+            if (d == dir) continue;
+            recon_scal(Wa(P, b, WB0 + d, cm), Wa(P, b, WB0 + d, c0), Wa(P, b, WB0 + d, cp), Wa(P, b, WB0 + d, cpp),
+                       P.limiter, fl, fr);
+            L.B[d] = fl; R.B[d] = fr;
+        }
+    }
+    // normal field from the face (grid.cpp:263-264, 287-288, 311-312)
+    const double* Bf = P.Bf + ((size_t)b * 3 + dir) * NFACE;
+    const int bidx = dir == 0 ? ixf(i + 1, j, k) : (dir == 1 ? iyf(i, j + 1, k) : izf(i, j, k + 1));
+    L.B[dir] = Bf[bidx];
+    R.B[dir] = Bf[bidx];
+    // tree metric at the face
+    const double* gf = P.gfc + ((size_t)b * 3 + dir) * NGFC * NFC + lf;
+    GPt m;
+    m.sqrtdetg = gf[0]; m.sqrtdetgam = gf[NFC]; m.alpha = gf[2 * NFC]; m.beta0 = gf[3 * NFC];
+    m.g11 = gf[4 * NFC]; m.g13 = gf[5 * NFC]; m.g22 = gf[6 * NFC]; m.g33 = gf[7 * NFC];
+    m.gi13 = 0.0; m.gi33 = 0.0;
+    Cons UL = ptoc(m, P.gamma, L, nf);
+    Cons UR = ptoc(m, P.gamma, R, nf);
+    double ct;
+    Cons Fl = hlld(L, R, UL, UR, m, P.gamma, 0, ct, nf);
+    const int fidx = dir == 0 ? ixf(i, j, k) : (dir == 1 ? iyf(i, j, k) : izf(i, j, k));
+    double* Fo = P.F + ((size_t)b * 3 + dir) * 8 * NFACE + fidx;
+    Fo[0] = Fl.D; Fo[NFACE] = Fl.S[0]; Fo[2 * NFACE] = Fl.S[1]; Fo[3 * NFACE] = Fl.S[2];
+    Fo[4 * NFACE] = Fl.tau; Fo[5 * NFACE] = Fl.B[0]; Fo[6 * NFACE] = Fl.B[1]; Fo[7 * NFACE] = Fl.B[2];
+    add_faults(P, nf);
+}
+
+// ---------------------------------------------------- constrained transport
+// constrans::emfcomp, transport.cpp:11-36: edge EMFs as 4-point averages of
+// the transverse face fluxes of B (zero outside computed faces, quirk Q19).
+__global__ __launch_bounds__(TPB) void k_emf(KP P) {
+    const int b = blockIdx.x / WG_EDGE;
+    const int t = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
+    if (t >= 19 * 19 * 19) return;
+    const int i = t / 361 - GH, j = (t / 19) % 19 - GH, k = t % 19 - GH;
+    const double* Fx = P.F + ((size_t)b * 3 + 0) * 8 * NFACE;
+    const double* Fy = P.F + ((size_t)b * 3 + 1) * 8 * NFACE;
+    const double* Fz = P.F + ((size_t)b * 3 + 2) * 8 * NFACE;
+    double* E = P.E + (size_t)b * 3 * NEDGE;
+    E[iex(i, j, k)] = (Fy[7 * NFACE + iyf(i, j, k)] + Fy[7 * NFACE + iyf(i, j, k + 1)] - Fz[6 * NFACE + izf(i, j, k)] -
+                       Fz[6 * NFACE + izf(i, j + 1, k)]) / 4;
+    E[NEDGE + izf(i, j, k)] = (Fz[5 * NFACE + izf(i, j, k)] + Fz[5 * NFACE + izf(i + 1, j, k)] -
+                               Fx[7 * NFACE + ixf(i, j, k)] - Fx[7 * NFACE + ixf(i, j, k + 1)]) / 4;
+    E[2 * NEDGE + iyf(i, j, k)] = (Fx[6 * NFACE + ixf(i, j, k)] + Fx[6 * NFACE + ixf(i, j + 1, k)] -
+                                   Fy[5 * NFACE + iyf(i, j, k)] - Fy[5 * NFACE + iyf(i + 1, j, k)]) / 4;
+}
+
+// constrans::Bfupdate, transport.cpp:38-58 (not RK-averaged, quirk Q21)
+__global__ __launch_bounds__(TPB) void k_bfupd(KP P) {
+    const int b = blockIdx.x / WG_BF;
+    const int t = (blockIdx.x % WG_BF) * TPB + threadIdx.x;
+    if (t >= 18 * 18 * 18) return;
+    const int i = t / 324 - 1, j = (t / 18) % 18 - 1, k = t % 18 - 1;
+    const double dt = __longlong_as_double((long long)*P.dtbits);
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const double* Ex = P.E + (size_t)b * 3 * NEDGE;
+    const double* Ey = Ex + NEDGE;
+    const double* Ez = Ex + 2 * NEDGE;
+    double* Bx = P.Bf + (size_t)b * 3 * NFACE;
+    double* By = Bx + NFACE;
+    double* Bz = Bx + 2 * NFACE;
+    double dEz_dy = (Ez[iyf(i, j, k)] - Ez[iyf(i, j - 1, k)]) / dy;
+    double dEy_dz = (Ey[izf(i, j, k)] - Ey[izf(i, j, k - 1)]) / dz;
+    Bx[ixf(i, j, k)] += dt * (dEz_dy - dEy_dz);
+    double dEz_dx = (Ez[iyf(i, j, k)] - Ez[iyf(i - 1, j, k)]) / dx;
+    double dEx_dz = (Ex[iex(i, j, k)] - Ex[iex(i, j, k - 1)]) / dz;
+    By[iyf(i, j, k)] += dt * (dEx_dz - dEz_dx);
+    double dEy_dx = (Ey[izf(i, j, k)] - Ey[izf(i - 1, j, k)]) / dx;
+    double dEx_dy = (Ex[iex(i, j, k)] - Ex[iex(i, j - 1, k)]) / dy;
+    Bz[izf(i, j, k)] += dt * (dEy_dx - dEx_dy);
+}
+
+// constrans::f2cB, transport.cpp:60-75: cell B = face average; U.B uses the
+// cell metric's sqrt(-g) (quirk Q4)
+__global__ __launch_bounds__(TPB) void k_f2cb(KP P) {
+    const int b = blockIdx.x / WG_EDGE;
+    const int t = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
+    if (t >= 19 * 19 * 19) return;
+    const int i = t / 361 - 1, j = (t / 19) % 19 - 1, k = t % 19 - 1;
+    const double* Bx = P.Bf + (size_t)b * 3 * NFACE;
+    const double* By = Bx + NFACE;
+    const double* Bz = Bx + 2 * NFACE;
+    const int c = lindx(i, j, k);
+    double b0 = (Bx[ixf(i, j, k)] + Bx[ixf(i - 1, j, k)]) / 2;
+    double b1 = (By[iyf(i, j, k)] + By[iyf(i, j - 1, k)]) / 2;
+    double b2 = (Bz[izf(i, j, k)] + Bz[izf(i, j, k - 1)]) / 2;
+    Wa(P, b, WB0, c) = b0; Wa(P, b, WB1, c) = b1; Wa(P, b, WB2, c) = b2;
+    double sg = cell_sqrtdetg(P, b, c);
+    Ua(P, b, UB0, c) = sg * b0; Ua(P, b, UB1, c) = sg * b1; Ua(P, b, UB2, c) = sg * b2;
+}
+
+// ---------------------------------------------------------------- update
+// Geometric source term, rk2integrator::source, integrator.cpp:41-153, with
+// the cell metric (a=0); spherical components land in Cartesian slots.
+__device__ __forceinline__ void source_terms(const KP& P, int b, int c, int a, const Prim& W, double src[4],
+                                             int& nf) {
+    const double sgc = gcc_at(P, b, 0, c), alpha = gcc_at(P, b, 1, c), beta0 = gcc_at(P, b, 2, c);
+    const double g11 = gcc_at(P, b, 3, c), g22 = gcc_at(P, b, 4, c), g33 = gcc_at(P, b, 5, c);
+    const double gam[3][3] = {{g11, 0.0, CELL_G13}, {0.0, g22, 0.0}, {CELL_G13, 0.0, g33}};
+    const double gi00 = gcc_at(P, b, 6, c), gi01 = gcc_at(P, b, 7, c), gi11 = gcc_at(P, b, 8, c);
+    const double gi22 = gcc_at(P, b, 9, c), gi33 = gcc_at(P, b, 10, c);
+    const double gi[4][4] = {{gi00, gi01, 0.0, 0.0}, {gi01, gi11, 0.0, 0.0}, {0.0, 0.0, gi22, 0.0}, {0.0, 0.0, 0.0, gi33}};
+    const double beta[3] = {beta0, 0.0, 0.0};
+    double v2 = 0.0;
+    double vlow[3] = {0.0, 0.0, 0.0}, Blow[3] = {0.0, 0.0, 0.0};
+    for (int m = 0; m < 3; m++) {
+        for (int n = 0; n < 3; n++) {
+            vlow[m] += gam[m][n] * W.v[n];
+            Blow[m] += gam[m][n] * W.B[n];
+        }
+        v2 += vlow[m] * W.v[m];
+    }
+    v2 = smin(v2, 1 - 1e-10);
+    double ltz2 = 1.0 / (1.0 - v2);
+    double ltz = sqrt(ltz2);
+    double Bv = 0.0, Bsq = 0.0;
+    for (int m = 0; m < 3; m++) { Bv += Blow[m] * W.v[m]; Bsq += Blow[m] * W.B[m]; }
+    double b2 = Bv * Bv + Bsq / ltz2;
+    double px = eos_press(P.gamma, W.rho, W.eps, nf);
+    double h = eos_enth(P.gamma, W.rho, W.eps, nf);
+    double u4[4], b4[4];
+    u4[0] = ltz / alpha;
+    for (int m = 0; m < 3; m++) u4[m + 1] = ltz * (W.v[m] - beta[m] / alpha);
+    b4[0] = ltz * Bv / alpha;
+    for (int m = 0; m < 3; m++) b4[m + 1] = (W.B[m] + alpha * b4[0] * u4[m + 1]) / ltz;
+    double rhohb2 = W.rho * h + b2;
+    double T[4][4];
+    for (int m = 0; m < 4; m++)
+        for (int n = 0; n < 4; n++) T[m][n] = rhohb2 * u4[m] * u4[n] + px * gi[m][n] - b4[m] * b4[n];
+    const double* G = P.gchr + (size_t)b * NCHR * NINT + a;
+    for (int m = 0; m < 4; m++) {
+        double s = 0.0;
+        for (int n = 0; n < 4; n++)
+            for (int l = 0; l < 4; l++) s += T[n][l] * G[(size_t)(m * 10 + sym4(n, l)) * NINT];
+        src[m] = s * sgc;
+    }
+}
+
+// Conserved update on active zones of every block (integrator.cpp:238-259;
+// quirks Q20, Q22, Q31): dU = div F + S; stage 0 stores U and takes an Euler
+// step, stage 1 averages; U.B is reset from the cell-centred field.
+__global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int c = lindx(i, j, k);
+    const double dt = __longlong_as_double((long long)*P.dtbits);
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const double sg = cell_sqrtdetg(P, b, c);
+    const double* Fx = P.F + ((size_t)b * 3 + 0) * 8 * NFACE;
+    const double* Fy = P.F + ((size_t)b * 3 + 1) * 8 * NFACE;
+    const double* Fz = P.F + ((size_t)b * 3 + 2) * 8 * NFACE;
+    const int xR = ixf(i, j, k), xL = ixf(i - 1, j, k);
+    const int yR = iyf(i, j, k), yL = iyf(i, j - 1, k);
+    const int zR = izf(i, j, k), zL = izf(i, j, k - 1);
+    int nf = 0;
+    Prim W = loadW(P, b, c);
+    double src[4];
+    source_terms(P, b, c, a, W, src, nf);
+    // U component v in {D, S0, S1, S2, tau}; source slot: tau <- src0, S_d <- src_{d+1}
+    for (int v = 0; v < 5; v++) {
+        const size_t o = (size_t)v * NFACE;
+        double div = -((Fx[o + xR] - Fx[o + xL]) / dx + (Fy[o + yR] - Fy[o + yL]) / dy +
+                       (Fz[o + zR] - Fz[o + zL]) / dz) / sg;
+        double s = v == UD ? 0.0 : (v == UTAU ? src[0] : src[v]);
+        double dU = div + s;
+        double u = Ua(P, b, v, c);
+        double* us = P.Us + ((size_t)b * 5 + v) * NINT + a;
+        double nu;
+        if (stage == 0) {
+            *us = u;
+            nu = u + dU * dt;
+        } else {
+            nu = ((*us + u) + dU * dt) * 0.5;
+        }
+        Ua(P, b, v, c) = nu;
+    }
+    for (int d = 0; d < 3; d++) Ua(P, b, UB0 + d, c) = sg * W.B[d];
+    add_faults(P, nf);
+}
+
+// ------------------------------------------------------------------- dt
+// rk2integrator::dtcomp, integrator.cpp:156-189: min over leaf active zones
+// and directions of cfl*dx/|alpha(v +- c_s) - beta| (cell metric, quirk Q23).
+// Candidates are positive, so their bit patterns order like their values.
+__global__ void k_dtinit(KP P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *P.dtbits = (unsigned long long)__double_as_longlong(DBL_MAX);
+}
+__global__ __launch_bounds__(TPB) void k_dtcomp(KP P, const int* __restrict__ leaves, double cfl) {
+    const int b = leaves[blockIdx.x / WG_INT];
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int c = lindx(a / 256, (a / 16) % 16, a % 16);
+    int nf = 0;
+    const double rho = Wa(P, b, WRHO, c), eps = Wa(P, b, WEPS, c);
+    const double cs = sqrt(eos_cs2(P.gamma, rho, eps, nf));
+    const double alpha = gcc_at(P, b, 1, c);
+    const double beta[3] = {gcc_at(P, b, 2, c), 0.0, 0.0};
+    unsigned long long best = (unsigned long long)__double_as_longlong(DBL_MAX);
+    for (int d = 0; d < 3; d++) {
+        double v = Wa(P, b, WV0 + d, c);
+        double ld_p = fabs(alpha * (v + cs) - beta[d]);
+        double ld_m = fabs(alpha * (v - cs) - beta[d]);
+        double vmax = smax(ld_p, ld_m);
+        if (vmax > 1e-14) {
+            double cand = cfl * P.dxyz[3 * b + d] / vmax;
+            unsigned long long cb = (unsigned long long)__double_as_longlong(cand);
+            best = cb < best ? cb : best;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(best, off);
+        best = o < best ? o : best;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(P.dtbits, best);
+    add_faults(P, nf);
+}
+
+// --------------------------------------------------------------- floors
+// patch::floors on active zones of every block, grid.cpp:331-358
+__global__ __launch_bounds__(TPB) void k_floors(KP P) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int c = lindx(a / 256, (a / 16) % 16, a % 16);
+    int nf = 0;
+    GPt m = cell_tree(P, b, c);
+    const double D = Ua(P, b, UD, c), tau = Ua(P, b, UTAU, c);
+    bool bhl = !isfinite(D) || !isfinite(tau) || D <= 0.0 || tau <= 0.0;
+    if (!bhl) {
+        Prim pv;
+        bhl = !ctop(m, P.gamma, loadU(P, b, c), pv, nf);
+        if (!bhl) {
+            bool hl = !(pv.rho <= 0.0 || pv.eps <= 0.0);
+            if (hl) storeW(P, b, c, pv);
+        }
+    }
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    if (bhl || Wa(P, b, WRHO, c) <= fl[c]) {
+        Prim f = floor_prim(fl[c], fl[NCELL + c]);
+        f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
+        storeW(P, b, c, f);
+        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+    }
+    add_faults(P, nf);
+}
+
+// ---------------------------------------------------- pointwise batches
+__global__ void k_ptoc_batch(int n, double gm, const double* g, const double* W, double* U,
+                             unsigned long long* faults) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    GPt m;
+    m.sqrtdetg = g[10 * t]; m.sqrtdetgam = g[10 * t + 1]; m.alpha = g[10 * t + 2]; m.beta0 = g[10 * t + 3];
+    m.g11 = g[10 * t + 4]; m.g13 = g[10 * t + 5]; m.g22 = g[10 * t + 6]; m.g33 = g[10 * t + 7];
+    m.gi13 = g[10 * t + 8]; m.gi33 = g[10 * t + 9];
+    Prim w;
+    w.rho = W[8 * t]; w.eps = W[8 * t + 1];
+    for (int d = 0; d < 3; d++) { w.v[d] = W[8 * t + 2 + d]; w.B[d] = W[8 * t + 5 + d]; }
+    int nf = 0;
+    Cons q = ptoc(m, gm, w, nf);
+    U[8 * t] = q.D; U[8 * t + 1] = q.S[0]; U[8 * t + 2] = q.S[1]; U[8 * t + 3] = q.S[2];
+    U[8 * t + 4] = q.tau; U[8 * t + 5] = q.B[0]; U[8 * t + 6] = q.B[1]; U[8 * t + 7] = q.B[2];
+    if (nf) atomicAdd(faults, (unsigned long long)nf);
+}
+__global__ void k_ctop_batch(int n, double gm, const double* g, const double* U, double* W, int* ok,
+                             unsigned long long* faults) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    GPt m;
+    m.sqrtdetg = g[10 * t]; m.sqrtdetgam = g[10 * t + 1]; m.alpha = g[10 * t + 2]; m.beta0 = g[10 * t + 3];
+    m.g11 = g[10 * t + 4]; m.g13 = g[10 * t + 5]; m.g22 = g[10 * t + 6]; m.g33 = g[10 * t + 7];
+    m.gi13 = g[10 * t + 8]; m.gi33 = g[10 * t + 9];
+    Cons q;
+    q.D = U[8 * t]; q.S[0] = U[8 * t + 1]; q.S[1] = U[8 * t + 2]; q.S[2] = U[8 * t + 3];
+    q.tau = U[8 * t + 4]; q.B[0] = U[8 * t + 5]; q.B[1] = U[8 * t + 6]; q.B[2] = U[8 * t + 7];
+    int nf = 0;
+    Prim w;
+    for (int v = 0; v < 8; v++) W[8 * t + v] = 0.0;
+    bool r = ctop(m, gm, q, w, nf);
+    if (r) {
+        W[8 * t] = w.rho; W[8 * t + 1] = w.eps;
+        for (int d = 0; d < 3; d++) { W[8 * t + 2 + d] = w.v[d]; W[8 * t + 5 + d] = w.B[d]; }
+    }
+    ok[t] = r ? 1 : 0;
+    if (nf) atomicAdd(faults, (unsigned long long)nf);
+}
+__global__ void k_hlld_batch(int n, double gm, int dim, const double* g, const double* WL, const double* WR,
+                             const double* UL, const double* UR, double* F, double* ct,
+                             unsigned long long* faults) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    GPt m;
+    m.sqrtdetg = g[10 * t]; m.sqrtdetgam = g[10 * t + 1]; m.alpha = g[10 * t + 2]; m.beta0 = g[10 * t + 3];
+    m.g11 = g[10 * t + 4]; m.g13 = g[10 * t + 5]; m.g22 = g[10 * t + 6]; m.g33 = g[10 * t + 7];
+    m.gi13 = g[10 * t + 8]; m.gi33 = g[10 * t + 9];
+    auto ldW = [&](const double* A) {
+        Prim w;
+        w.rho = A[8 * t]; w.eps = A[8 * t + 1];
+        for (int d = 0; d < 3; d++) { w.v[d] = A[8 * t + 2 + d]; w.B[d] = A[8 * t + 5 + d]; }
+        return w;
+    };
+    auto ldU = [&](const double* A) {
+        Cons q;
+        q.D = A[8 * t]; q.S[0] = A[8 * t + 1]; q.S[1] = A[8 * t + 2]; q.S[2] = A[8 * t + 3];
+        q.tau = A[8 * t + 4]; q.B[0] = A[8 * t + 5]; q.B[1] = A[8 * t + 6]; q.B[2] = A[8 * t + 7];
+        return q;
+    };
+    int nf = 0;
+    double c;
+    Cons q = hlld(ldW(WL), ldW(WR), ldU(UL), ldU(UR), m, gm, dim, c, nf);
+    F[8 * t] = q.D; F[8 * t + 1] = q.S[0]; F[8 * t + 2] = q.S[1]; F[8 * t + 3] = q.S[2];
+    F[8 * t + 4] = q.tau; F[8 * t + 5] = q.B[0]; F[8 * t + 6] = q.B[1]; F[8 * t + 7] = q.B[2];
+    ct[t] = c;
+    if (nf) atomicAdd(faults, (unsigned long long)nf);
+}
+__global__ void k_math_batch(int op, int n, const double* x, double* y) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    y[t] = op == 0 ? cr_log(x[t]) : cr_exp(x[t]);
+}
+
+// ------------------------------------------------------------ device tree
+template <class T>
+struct DBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        release();
+        if (count) HIPCHK(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        HIPCHK(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void download(T* h, size_t count, hipStream_t s) const {
+        HIPCHK(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    ~DBuf() { release(); }
+};
+
+class DeviceTree {
+public:
+    int nb = 0;
+    double gamma = 5.0 / 3.0;
+    int limiter = LIM_VANLEER;
+    hipStream_t stream = nullptr;
+    GhostPlan plan;
+    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz;
+    DBuf<int> ghtab, leaves, waveblk;
+    std::vector<int> wave_off;  // offsets into waveblk
+    DBuf<short> ghost_cells;
+    DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
+
+    DeviceTree() { HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+    ~DeviceTree() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+
+    KP kp() {
+        KP P;
+        P.gamma = gamma; P.limiter = limiter;
+        P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p; P.E = E.p;
+        P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p;
+        P.ghtab = ghtab.p; P.ghost_cells = ghost_cells.p;
+        P.dtbits = scal.p; P.faults = scal.p + 1;
+        return P;
+    }
+
+    // (Re)build for the host tree's current block structure.
+    void build(const HostTree& t) {
+        gamma = t.prm.gamma;
+        nb = t.nblocks();
+        GeomHost g;
+        build_geometry(t, g);
+        build_ghost_plan(t, plan);
+        U.alloc((size_t)nb * 8 * NCELL);
+        W.alloc((size_t)nb * 8 * NCELL);
+        Bf.alloc((size_t)nb * 3 * NFACE);
+        Us.alloc((size_t)nb * 5 * NINT);
+        F.alloc((size_t)nb * 3 * 8 * NFACE);
+        E.alloc((size_t)nb * 3 * NEDGE);
+        HIPCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
+        HIPCHK(hipMemsetAsync(E.p, 0, E.n * sizeof(double), stream));
+        HIPCHK(hipMemsetAsync(Us.p, 0, Us.n * sizeof(double), stream));
+        gct.alloc(g.cell_tree.size()); gct.upload(g.cell_tree.data(), g.cell_tree.size(), stream);
+        gcc.alloc(g.cell_cell.size()); gcc.upload(g.cell_cell.data(), g.cell_cell.size(), stream);
+        gfl.alloc(g.floors.size()); gfl.upload(g.floors.data(), g.floors.size(), stream);
+        gfc.alloc(g.face.size()); gfc.upload(g.face.data(), g.face.size(), stream);
+        gchr.alloc(g.chr.size()); gchr.upload(g.chr.data(), g.chr.size(), stream);
+        dxyz.alloc(g.dxyz.size()); dxyz.upload(g.dxyz.data(), g.dxyz.size(), stream);
+        ghtab.alloc(plan.tab.size()); ghtab.upload(plan.tab.data(), plan.tab.size(), stream);
+        leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
+        if (!plan.leaves.empty()) leaves.upload(plan.leaves.data(), plan.leaves.size(), stream);
+        std::vector<int> wb;
+        wave_off.assign(1, 0);
+        for (auto& w : plan.waves) {
+            wb.insert(wb.end(), w.begin(), w.end());
+            wave_off.push_back((int)wb.size());
+        }
+        waveblk.alloc(std::max<size_t>(1, wb.size()));
+        if (!wb.empty()) waveblk.upload(wb.data(), wb.size(), stream);
+        std::vector<short> gc;
+        for (int c = 0; c < NCELL; c++) {
+            int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
+            if (i < 0 || i >= BLK || j < 0 || j >= BLK || k < 0 || k >= BLK) gc.push_back((short)c);
+        }
+        ghost_cells.alloc(gc.size()); ghost_cells.upload(gc.data(), gc.size(), stream);
+        scal.alloc(2);
+        HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    void upload_state(const HostTree& t) {
+        std::vector<double> u((size_t)nb * 8 * NCELL), w((size_t)nb * 8 * NCELL), bf((size_t)nb * 3 * NFACE);
+        t.export_state(u.data(), w.data(), bf.data());
+        U.upload(u.data(), u.size(), stream);
+        W.upload(w.data(), w.size(), stream);
+        Bf.upload(bf.data(), bf.size(), stream);
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void download_state(HostTree& t) {
+        std::vector<double> u((size_t)nb * 8 * NCELL), w((size_t)nb * 8 * NCELL), bf((size_t)nb * 3 * NFACE);
+        U.download(u.data(), u.size(), stream);
+        W.download(w.data(), w.size(), stream);
+        Bf.download(bf.data(), bf.size(), stream);
+        HIPCHK(hipStreamSynchronize(stream));
+        t.import_state(u.data(), w.data(), bf.data());
+    }
+
+    void launch_dtcomp(double cfl) {
+        KP P = kp();
+        hipLaunchKernelGGL(k_dtinit, dim3(1), dim3(64), 0, stream, P);
+        int nl = (int)plan.leaves.size();
+        if (nl) hipLaunchKernelGGL(k_dtcomp, dim3(nl * WG_INT), dim3(TPB), 0, stream, P, leaves.p, cfl);
+    }
+    void set_dt(double dt) {
+        unsigned long long bits;
+        std::memcpy(&bits, &dt, sizeof bits);
+        HIPCHK(hipMemcpyAsync(scal.p, &bits, sizeof bits, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+    void launch_rkstg(int stage) {
+        KP P = kp();
+        for (size_t w = 0; w + 1 < wave_off.size(); w++) {
+            int n = wave_off[w + 1] - wave_off[w];
+            if (n) hipLaunchKernelGGL(k_ghost, dim3(n * WG_GHOST), dim3(TPB), 0, stream, P, waveblk.p + wave_off[w]);
+        }
+        int nl = (int)plan.leaves.size();
+        if (nl) hipLaunchKernelGGL(k_c2p, dim3(nl * WG_CELL), dim3(TPB), 0, stream, P, leaves.p);
+        hipLaunchKernelGGL(k_flux, dim3(nb * WG_FACE), dim3(TPB), 0, stream, P);
+        hipLaunchKernelGGL(k_emf, dim3(nb * WG_EDGE), dim3(TPB), 0, stream, P);
+        hipLaunchKernelGGL(k_bfupd, dim3(nb * WG_BF), dim3(TPB), 0, stream, P);
+        hipLaunchKernelGGL(k_f2cb, dim3(nb * WG_EDGE), dim3(TPB), 0, stream, P);
+        hipLaunchKernelGGL(k_update, dim3(nb * WG_INT), dim3(TPB), 0, stream, P, stage);
+    }
+    void launch_floors() {
+        KP P = kp();
+        hipLaunchKernelGGL(k_floors, dim3(nb * WG_INT), dim3(TPB), 0, stream, P);
+    }
+    void launch_step(double cfl) {
+        launch_dtcomp(cfl);
+        launch_rkstg(0);
+        launch_rkstg(1);
+        launch_floors();
+    }
+    double read_dt() {
+        unsigned long long bits = 0;
+        HIPCHK(hipMemcpyAsync(&bits, scal.p, sizeof bits, hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        double dt;
+        std::memcpy(&dt, &bits, sizeof dt);
+        return dt;
+    }
+    long long take_faults() {
+        unsigned long long f = 0;
+        HIPCHK(hipMemcpyAsync(&f, scal.p + 1, sizeof f, hipMemcpyDeviceToHost, stream));
+        unsigned long long zero = 0;
+        HIPCHK(hipMemcpyAsync(scal.p + 1, &zero, sizeof zero, hipMemcpyHostToDevice, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        return (long long)f;
+    }
+    void sync() {
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(stream));
+    }
+};
+
+}  // namespace bh
+
+// =================================================================== C ABI
+using namespace bh;
+
+struct bh_tree {
+    std::unique_ptr<HostTree> host;
+    std::unique_ptr<DeviceTree> dev;
+    bool host_valid = true;   // host copy holds the current state
+    bool dev_valid = false;   // device copy holds the current state
+    bool layout_dirty = true; // block structure changed since the device build
+    long long faults = 0;
+    std::string err;
+};
+
+namespace {
+thread_local std::string g_create_err;
+
+int fail(bh_tree* t, int code, const std::string& msg) {
+    if (t) t->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+void need_host(bh_tree* t) {
+    if (!t->host_valid) {
+        t->dev->download_state(*t->host);
+        t->host_valid = true;
+    }
+}
+void host_changed(bh_tree* t) {
+    t->host_valid = true;
+    t->dev_valid = false;
+}
+void need_device(bh_tree* t) {
+    if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+    if (t->layout_dirty) {
+        need_host(t);
+        t->dev->build(*t->host);
+        t->layout_dirty = false;
+        t->dev_valid = false;
+    }
+    if (!t->dev_valid) {
+        t->dev->upload_state(*t->host);
+        t->dev_valid = true;
+    }
+}
+void device_changed(bh_tree* t) {
+    t->dev_valid = true;
+    t->host_valid = false;
+}
+int finish_faults(bh_tree* t, long long add) {
+    t->faults += add;
+    return add > 0 ? BH_EDOMAIN : BH_OK;
+}
+
+template <class F>
+int guarded(bh_tree* t, F&& f) {
+    if (!t) return fail(nullptr, BH_EINVAL, "null tree");
+    try {
+        return f();
+    } catch (const DevError& e) {
+        return fail(t, BH_EDEVICE, e.msg);
+    } catch (const std::exception& e) {
+        return fail(t, BH_ESTATE, e.what());
+    }
+}
+
+std::vector<double> point_geometry(double M, double a, double Q, int n, const double* r, const double* th) {
+    std::vector<double> g((size_t)10 * n);
+    for (int i = 0; i < n; i++) {
+        GPt m = gpt_from(metric_comp(M, a, Q, r[i], th[i]));
+        double v[10] = {m.sqrtdetg, m.sqrtdetgam, m.alpha, m.beta0, m.g11, m.g13, m.g22, m.g33, m.gi13, m.gi33};
+        for (int k = 0; k < 10; k++) g[(size_t)10 * i + k] = v[k];
+    }
+    return g;
+}
+
+struct Scratch {
+    std::vector<void*> ptrs;
+    template <class T>
+    T* put(const T* h, size_t n) {
+        T* p = nullptr;
+        HIPCHK(hipMalloc(&p, std::max<size_t>(1, n) * sizeof(T)));
+        ptrs.push_back(p);
+        if (h && n) HIPCHK(hipMemcpy(p, h, n * sizeof(T), hipMemcpyHostToDevice));
+        return p;
+    }
+    ~Scratch() {
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+}  // namespace
+
+extern "C" {
+
+int bh_tree_create(const double dom_l[3], const double dom_h[3], int nqlt, double M, double a, double Q, double gm,
+                   bh_tree** out) {
+    if (!out || !dom_l || !dom_h || nqlt < 1) return fail(nullptr, BH_EINVAL, "invalid arguments");
+    try {
+        auto* t = new bh_tree;
+        t->host = std::make_unique<HostTree>(dom_l, dom_h, nqlt, M, a, Q, gm);
+        t->faults = t->host->faults;
+        *out = t;
+        return BH_OK;
+    } catch (const std::exception& e) {
+        return fail(nullptr, BH_ESTATE, e.what());
+    }
+}
+void bh_tree_destroy(bh_tree* t) { delete t; }
+const char* bh_last_error(const bh_tree* t) { return t ? t->err.c_str() : g_create_err.c_str(); }
+
+int bh_tree_fm_init(bh_tree* t) {
+    return guarded(t, [&] {
+        need_host(t);
+        int rc = t->host->fm_init();
+        host_changed(t);
+        if (rc == 1) return fail(t, BH_ESTATE, t->host->last_msg);
+        if (rc == 2) { t->err = t->host->last_msg; return 2; }
+        return BH_OK;
+    });
+}
+int bh_tree_rho_max(bh_tree* t, double* rho_max) {
+    return guarded(t, [&] {
+        need_host(t);
+        *rho_max = t->host->rho_max();
+        return BH_OK;
+    });
+}
+int bh_tree_B_pot_init(bh_tree* t, double rho_max) {
+    return guarded(t, [&] {
+        need_host(t);
+        for (auto& p : t->host->quilt) t->host->B_pot_init(*p, rho_max);
+        host_changed(t);
+        return BH_OK;
+    });
+}
+int bh_tree_ptoc_interior(bh_tree* t) {
+    return guarded(t, [&] {
+        need_host(t);
+        long long f0 = t->host->faults;
+        t->host->ptoc_interior();
+        host_changed(t);
+        return finish_faults(t, t->host->faults - f0);
+    });
+}
+int bh_tree_regrid(bh_tree* t) {
+    return guarded(t, [&] {
+        need_host(t);
+        long long f0 = t->host->faults;
+        t->host->regrid();
+        host_changed(t);
+        t->layout_dirty = true;
+        return finish_faults(t, t->host->faults - f0);
+    });
+}
+int bh_tree_nblocks(const bh_tree* t, int* nb) {
+    if (!t || !nb) return BH_EINVAL;
+    *nb = t->host->nblocks();
+    return BH_OK;
+}
+int bh_tree_get_blocks(const bh_tree* t, int* meta, double* edges) {
+    if (!t || !meta || !edges) return BH_EINVAL;
+    t->host->export_blocks(meta, edges);
+    return BH_OK;
+}
+int bh_tree_get_state(bh_tree* t, double* U, double* W, double* Bf) {
+    return guarded(t, [&] {
+        need_host(t);
+        t->host->export_state(U, W, Bf);
+        return BH_OK;
+    });
+}
+int bh_tree_set_state(bh_tree* t, const double* U, const double* W, const double* Bf) {
+    return guarded(t, [&] {
+        need_host(t);
+        t->host->import_state(U, W, Bf);
+        host_changed(t);
+        return BH_OK;
+    });
+}
+
+int bh_rk2_dtcomp(bh_tree* t, double cfl, double* dt) {
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->launch_dtcomp(cfl);
+        t->dev->sync();
+        *dt = t->dev->read_dt();
+        return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_rk2_rkstg(bh_tree* t, double dt, int stage) {
+    if (stage != 0 && stage != 1) return fail(t, BH_EINVAL, "stage must be 0 or 1");
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->set_dt(dt);
+        t->dev->launch_rkstg(stage);
+        t->dev->sync();
+        device_changed(t);
+        return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_tree_floors(bh_tree* t) {
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->launch_floors();
+        t->dev->sync();
+        device_changed(t);
+        return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_rk2_step(bh_tree* t, double cfl, double* dt) {
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->launch_step(cfl);
+        t->dev->sync();
+        device_changed(t);
+        if (dt) *dt = t->dev->read_dt();
+        return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_io, int* steps_out) {
+    return guarded(t, [&] {
+        need_device(t);
+        double tt = t_io ? *t_io : 0.0;
+        int n = 0;
+        long long nf = 0;
+        while (tt < t_end && n < nsteps) {
+            t->dev->launch_step(cfl);
+            t->dev->sync();
+            tt += t->dev->read_dt();
+            n++;
+        }
+        device_changed(t);
+        nf = t->dev->take_faults();
+        if (t_io) *t_io = tt;
+        if (steps_out) *steps_out = n;
+        return finish_faults(t, nf);
+    });
+}
+int bh_tree_take_faults(bh_tree* t, long long* n) {
+    if (!t || !n) return BH_EINVAL;
+    *n = t->faults;
+    t->faults = 0;
+    return BH_OK;
+}
+int bh_tree_maxdivB(bh_tree* t, int b, double* out) {
+    return guarded(t, [&] {
+        need_host(t);
+        if (b < 0 || b >= t->host->nblocks()) return fail(t, BH_EINVAL, "block index out of range");
+        Patch& p = *t->host->quilt[b];
+        double dx = p.dx(), dy = p.dy(), dz = p.dz(), mx = 0.0;
+        for (int i = -GH + 1; i < BLK + GH - 1; i++)
+            for (int j = -GH + 1; j < BLK + GH - 1; j++)
+                for (int k = -GH + 1; k < BLK + GH - 1; k++) {
+                    double dv = (p.bfx()[ixf(i, j, k)] - p.bfx()[ixf(i - 1, j, k)]) / dx +
+                                (p.bfy()[iyf(i, j, k)] - p.bfy()[iyf(i, j - 1, k)]) / dy +
+                                (p.bfz()[izf(i, j, k)] - p.bfz()[izf(i, j, k - 1)]) / dz;
+                    mx = smax(mx, fabs(dv));
+                }
+        *out = mx;
+        return BH_OK;
+    });
+}
+
+int bh_ptoc_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th,
+                  const double* W, double* U) {
+    if (n < 0) return BH_EINVAL;
+    try {
+        std::vector<double> g = point_geometry(M, a, Q, n, r, th);
+        Scratch s;
+        double* dg = s.put(g.data(), g.size());
+        double* dW = s.put(W, (size_t)8 * n);
+        double* dU = s.put<double>(nullptr, (size_t)8 * n);
+        unsigned long long zero = 0;
+        unsigned long long* df = s.put(&zero, 1);
+        if (n) hipLaunchKernelGGL(k_ptoc_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gm, dg, dW, dU, df);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(U, dU, (size_t)8 * n * sizeof(double), hipMemcpyDeviceToHost));
+        unsigned long long f = 0;
+        HIPCHK(hipMemcpy(&f, df, sizeof f, hipMemcpyDeviceToHost));
+        return f ? BH_EDOMAIN : BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_ctop_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th,
+                  const double* U, double* W, int* ok) {
+    if (n < 0) return BH_EINVAL;
+    try {
+        std::vector<double> g = point_geometry(M, a, Q, n, r, th);
+        Scratch s;
+        double* dg = s.put(g.data(), g.size());
+        double* dU = s.put(U, (size_t)8 * n);
+        double* dW = s.put<double>(nullptr, (size_t)8 * n);
+        int* dok = s.put<int>(nullptr, n);
+        unsigned long long zero = 0;
+        unsigned long long* df = s.put(&zero, 1);
+        if (n) hipLaunchKernelGGL(k_ctop_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gm, dg, dU, dW, dok, df);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(W, dW, (size_t)8 * n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ok, dok, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        unsigned long long f = 0;
+        HIPCHK(hipMemcpy(&f, df, sizeof f, hipMemcpyDeviceToHost));
+        return f ? BH_EDOMAIN : BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th, int dim,
+                  const double* WL, const double* WR, const double* UL, const double* UR, double* F,
+                  double* ctspeed) {
+    if (n < 0 || dim < 0 || dim > 2) return BH_EINVAL;
+    try {
+        std::vector<double> g = point_geometry(M, a, Q, n, r, th);
+        Scratch s;
+        double* dg = s.put(g.data(), g.size());
+        double* dWL = s.put(WL, (size_t)8 * n);
+        double* dWR = s.put(WR, (size_t)8 * n);
+        double* dUL = s.put(UL, (size_t)8 * n);
+        double* dUR = s.put(UR, (size_t)8 * n);
+        double* dF = s.put<double>(nullptr, (size_t)8 * n);
+        double* dc = s.put<double>(nullptr, n);
+        unsigned long long zero = 0;
+        unsigned long long* df = s.put(&zero, 1);
+        if (n)
+            hipLaunchKernelGGL(k_hlld_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gm, dim, dg, dWL, dWR, dUL,
+                               dUR, dF, dc, df);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(F, dF, (size_t)8 * n * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(ctspeed, dc, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        unsigned long long f = 0;
+        HIPCHK(hipMemcpy(&f, df, sizeof f, hipMemcpyDeviceToHost));
+        return f ? BH_EDOMAIN : BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_math_batch(int op, int n, const double* x, double* y, int on_device) {
+    if (n < 0 || (op != 0 && op != 1)) return BH_EINVAL;
+    if (!on_device) {
+        for (int i = 0; i < n; i++) y[i] = op == 0 ? cr_log(x[i]) : cr_exp(x[i]);
+        return BH_OK;
+    }
+    try {
+        Scratch s;
+        double* dx = s.put(x, n);
+        double* dy = s.put<double>(nullptr, n);
+        if (n) hipLaunchKernelGGL(k_math_batch, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        return BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_device_info(char* buf, int len) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) {
+        if (buf && len > 0) std::snprintf(buf, len, "no HIP device");
+        return BH_EDEVICE;
+    }
+    if (buf && len > 0)
+        std::snprintf(buf, len, "%s %s CUs=%d HBM=%.1fGiB", prop.name, prop.gcnArchName, prop.multiProcessorCount,
+                      prop.totalGlobalMem / 1073741824.0);
+    return BH_OK;
+}
+
+}  // extern "C"

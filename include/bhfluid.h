@@ -1,0 +1,101 @@
+/* bhfluid: C ABI of the MI355X (gfx950) GRMHD fluid step.
+ *
+ * Drop-in for the reference's fluid hot path (Fjoerde/black-hole-simulator,
+ * src/fluid).  The reference exposes this path only as C++ classes; each
+ * entry point below names the reference member it replaces.  Plain pointers
+ * and sizes only; every call returns 0 on success, BH_EDOMAIN (1) when the
+ * step completed but the equation of state saw rho <= 0 (where the reference
+ * throws std::domain_error, src/fluid/physics/thermo.cpp:11), or a negative
+ * code with the reason in bh_last_error().
+ *
+ * State layout (structure of arrays, one block = one 16^3 patch with two
+ * ghost layers, cell index (i+2)*400 + (j+2)*20 + (k+2), k fastest):
+ *   U [nb][8][8000]  D, S0, S1, S2, tau, B0, B1, B2      (cons, src/fluid/cons.hpp:14-59)
+ *   W [nb][8][8000]  rho, eps, v0, v1, v2, B0, B1, B2    (prim, src/fluid/prim.hpp:14-27)
+ *   Bf[nb][3][8400]  face fields Bfx, Bfy, Bfz in the patch's face indexing
+ *                    (src/fluid/grid/grid.cpp:40-42, 72-80)
+ * Block table: meta[nb][14] = lvl, i, j, k, leaf, parent, children[8]
+ * (-1 = none); edges[nb][3][17] = the patch's x, y, z cell-edge coordinates.
+ * Block order is the reference's quilt order.
+ */
+#ifndef BHFLUID_H
+#define BHFLUID_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BH_OK 0
+#define BH_EDOMAIN 1
+#define BH_EINVAL (-1)
+#define BH_EDEVICE (-2)
+#define BH_ESTATE (-3)
+
+typedef struct bh_tree bh_tree;
+
+/* grid::amrtree::amrtree(dom_l, dom_h, nqlt, M, a, Q, gm), src/fluid/grid/grid.cpp:177-206:
+ * nqlt^3 root patches, floor state in every cell. */
+int bh_tree_create(const double dom_l[3], const double dom_h[3], int nqlt, double M, double a, double Q, double gm,
+                   bh_tree** out);
+void bh_tree_destroy(bh_tree* t);
+/* Message of the last failing call on t (or of the last failing bh_tree_create if t is NULL). */
+const char* bh_last_error(const bh_tree* t);
+
+/* torus::init::fm_init(tree), src/fluid/solver/initconds.cpp:228-591.
+ * Returns 0, or 2 where the reference prints an error and returns. */
+int bh_tree_fm_init(bh_tree* t);
+/* Maximum rho over active zones, src/fluid/main.cpp:101-111. */
+int bh_tree_rho_max(bh_tree* t, double* rho_max);
+/* torus::init::B_pot_init(patch, mtr, rho_max) on every patch in quilt order,
+ * src/fluid/solver/initconds.cpp:594-668 as called at src/fluid/main.cpp:142-145. */
+int bh_tree_B_pot_init(bh_tree* t, double rho_max);
+/* U = conserved::ptoc(W) on active zones, src/fluid/main.cpp:148-159. */
+int bh_tree_ptoc_interior(bh_tree* t);
+/* grid::amrtree::regrid(), src/fluid/grid/grid.cpp:739-835. */
+int bh_tree_regrid(bh_tree* t);
+
+int bh_tree_nblocks(const bh_tree* t, int* nb);
+int bh_tree_get_blocks(const bh_tree* t, int* meta, double* edges);
+/* Copy the state out of / into the tree (any pointer may be NULL). */
+int bh_tree_get_state(bh_tree* t, double* U, double* W, double* Bf);
+int bh_tree_set_state(bh_tree* t, const double* U, const double* W, const double* Bf);
+
+/* integ::rk2integrator::step(tree), src/fluid/solver/integrator.cpp:262-275:
+ * dtcomp, two RK stages, floors; dt returned. */
+int bh_rk2_step(bh_tree* t, double cfl, double* dt);
+/* rk2integrator::dtcomp, integrator.cpp:156-189 */
+int bh_rk2_dtcomp(bh_tree* t, double cfl, double* dt);
+/* rk2integrator::rkstg(tree, dt, stage), integrator.cpp:191-260 */
+int bh_rk2_rkstg(bh_tree* t, double dt, int stage);
+/* patch::floors on every patch, as at integrator.cpp:269-273 */
+int bh_tree_floors(bh_tree* t);
+/* The time loop of src/fluid/main.cpp:169-173: step while t < t_end and
+ * steps < nsteps; *t_io is advanced, *steps_out counts steps taken. */
+int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_io, int* steps_out);
+/* Equation-of-state domain faults (rho <= 0) since the last call. */
+int bh_tree_take_faults(bh_tree* t, long long* n);
+/* constrans::maxdivB of block b, src/fluid/grid/transport.cpp:77-89 */
+int bh_tree_maxdivB(bh_tree* t, int b, double* out);
+
+/* Pointwise device entry points, for parity tests.  Inputs are host arrays
+ * of n points; the metric (M, a, Q) is evaluated at each (r, th).
+ *   conserved::ptoc  src/fluid/physics/conserved.cpp:17-73
+ *   conserved::ctop  src/fluid/physics/conserved.cpp:135-230 (ok[i] = 1 on success)
+ *   hlld             src/fluid/solver/hlld.cpp:230-312 */
+int bh_ptoc_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th,
+                  const double* W, double* U);
+int bh_ctop_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th,
+                  const double* U, double* W, int* ok);
+int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th, int dim,
+                  const double* WL, const double* WR, const double* UL, const double* UR, double* F,
+                  double* ctspeed);
+/* op 0: log, 1: exp, correctly rounded; on_device 0 evaluates the same code on the host. */
+int bh_math_batch(int op, int n, const double* x, double* y, int on_device);
+
+/* Device and build information. */
+int bh_device_info(char* buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

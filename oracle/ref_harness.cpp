@@ -1,0 +1,357 @@
+// Test-infrastructure only: a C-ABI harness around the compiled reference
+// fluid solver (Fjoerde/black-hole-simulator, src/fluid).  It is linked with
+// the reference's own translation units by oracle/Makefile into
+// oracle/_ref/libbhref.so and is used solely as a checker (golden vectors,
+// pinning the oracle restatement, and the timed CPU baseline).  Nothing in
+// the product library links or loads it.
+//
+// Every entry point only *calls* the reference's public API:
+//   grid::amrtree ctor            src/fluid/grid/grid.cpp:177-206
+//   torus::init::fm_init          src/fluid/solver/initconds.cpp:228-591
+//   torus::init::B_pot_init       src/fluid/solver/initconds.cpp:594-668
+//   conserved::ptoc / ctop        src/fluid/physics/conserved.cpp:17-230
+//   hlld                          src/fluid/solver/hlld.cpp:230-312
+//   reconfp                       src/fluid/grid/reconstruction.cpp:40-97
+//   rk2integrator::{step,dtcomp,rkstg,div_flux,source}  src/fluid/solver/integrator.cpp
+//   amrtree::{ghosts,regrid}, patch::{fluxcomp,floors}  src/fluid/grid/grid.cpp
+//   constrans::*                  src/fluid/grid/transport.cpp
+// The state exchange layout (SoA per block) is the one the product uses:
+//   U[b][8][20^3]   (D, S0, S1, S2, tau, B0, B1, B2)
+//   W[b][8][20^3]   (rho, eps, v0, v1, v2, B0, B1, B2)
+//   Bf[b][3][8400]  (Bfx, Bfy, Bfz in the reference's face indexing)
+// with the cell index (i+2)*400+(j+2)*20+(k+2), identical to patch::lindx.
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+#include <array>
+#include <memory>
+#include <omp.h>
+
+#include "grid.hpp"
+#include "rk2.hpp"
+#include "ct.hpp"
+#include "initial.hpp"
+#include "recon.hpp"
+#include "hlld.hpp"
+
+namespace {
+constexpr int NCELL = cpn * cpn * cpn;
+constexpr int NFACE = (cpn + 1) * cpn * cpn;
+
+struct RefTree {
+    std::unique_ptr<grid::amrtree> tree;
+    char err[256] = {0};
+};
+
+void put_prim(const prim& w, double* o) {
+    o[0] = w.rho; o[1] = w.eps;
+    for (int d = 0; d < 3; d++) { o[2 + d] = w.v[d]; o[5 + d] = w.B[d]; }
+}
+void get_prim(const double* o, prim& w) {
+    w.rho = o[0]; w.eps = o[1];
+    for (int d = 0; d < 3; d++) { w.v[d] = o[2 + d]; w.B[d] = o[5 + d]; }
+}
+void put_cons(const cons& u, double* o) {
+    o[0] = u.D;
+    for (int d = 0; d < 3; d++) o[1 + d] = u.S[d];
+    o[4] = u.tau;
+    for (int d = 0; d < 3; d++) o[5 + d] = u.B[d];
+}
+void get_cons(const double* o, cons& u) {
+    u.D = o[0];
+    for (int d = 0; d < 3; d++) u.S[d] = o[1 + d];
+    u.tau = o[4];
+    for (int d = 0; d < 3; d++) u.B[d] = o[5 + d];
+}
+}  // namespace
+
+extern "C" {
+
+// ---------------------------------------------------------------- tree
+void* ref_tree_new(const double* lo, const double* hi, int nqlt, double M, double a, double Q, double gm) {
+    auto* t = new RefTree;
+    t->tree = std::make_unique<grid::amrtree>(std::array<double, 3>{lo[0], lo[1], lo[2]},
+                                              std::array<double, 3>{hi[0], hi[1], hi[2]}, nqlt, M, a, Q, gm);
+    return t;
+}
+void ref_tree_free(void* h) { delete static_cast<RefTree*>(h); }
+const char* ref_tree_error(void* h) { return static_cast<RefTree*>(h)->err; }
+
+int ref_nblocks(void* h) { return (int)static_cast<RefTree*>(h)->tree->quilt.size(); }
+
+// per block: lvl, ijk[3], leaf, parent index (-1), children[8] (-1)
+void ref_get_blocks(void* h, int* meta /*[nb][14]*/, double* edges /*[nb][3][17]*/) {
+    auto& q = static_cast<RefTree*>(h)->tree->quilt;
+    int nb = (int)q.size();
+    auto index_of = [&](const grid::patch* p) -> int {
+        if (!p) return -1;
+        for (int i = 0; i < nb; i++)
+            if (q[i].get() == p) return i;
+        return -2;
+    };
+    for (int b = 0; b < nb; b++) {
+        const grid::patch* p = q[b].get();
+        int* m = meta + 14 * b;
+        m[0] = p->lvl;
+        m[1] = p->ijk[0]; m[2] = p->ijk[1]; m[3] = p->ijk[2];
+        m[4] = p->leaf ? 1 : 0;
+        m[5] = index_of(p->parent);
+        for (int c = 0; c < 8; c++) m[6 + c] = index_of(p->children[c]);
+        double* e = edges + 51 * b;
+        for (int i = 0; i <= block; i++) {
+            e[i] = p->xedge[i]; e[17 + i] = p->yedge[i]; e[34 + i] = p->zedge[i];
+        }
+    }
+}
+
+// export state; any pointer may be null
+void ref_get_state(void* h, double* U, double* W, double* Bf, double* dU) {
+    auto& q = static_cast<RefTree*>(h)->tree->quilt;
+    int nb = (int)q.size();
+    for (int b = 0; b < nb; b++) {
+        const grid::patch* p = q[b].get();
+        for (int c = 0; c < NCELL; c++) {
+            const cell& cl = p->cells[c];
+            double u[8], w[8], du[8];
+            put_cons(cl.U, u); put_prim(cl.W, w); put_cons(cl.dU, du);
+            for (int v = 0; v < 8; v++) {
+                if (U) U[((size_t)b * 8 + v) * NCELL + c] = u[v];
+                if (W) W[((size_t)b * 8 + v) * NCELL + c] = w[v];
+                if (dU) dU[((size_t)b * 8 + v) * NCELL + c] = du[v];
+            }
+        }
+        if (Bf) {
+            double* o = Bf + (size_t)b * 3 * NFACE;
+            for (int f = 0; f < NFACE; f++) {
+                o[f] = p->Bfx[f]; o[NFACE + f] = p->Bfy[f]; o[2 * NFACE + f] = p->Bfz[f];
+            }
+        }
+    }
+}
+
+void ref_set_state(void* h, const double* U, const double* W, const double* Bf, const double* dU) {
+    auto& q = static_cast<RefTree*>(h)->tree->quilt;
+    int nb = (int)q.size();
+    for (int b = 0; b < nb; b++) {
+        grid::patch* p = q[b].get();
+        for (int c = 0; c < NCELL; c++) {
+            cell& cl = p->cells[c];
+            double u[8], w[8], du[8];
+            for (int v = 0; v < 8; v++) {
+                if (U) u[v] = U[((size_t)b * 8 + v) * NCELL + c];
+                if (W) w[v] = W[((size_t)b * 8 + v) * NCELL + c];
+                if (dU) du[v] = dU[((size_t)b * 8 + v) * NCELL + c];
+            }
+            if (U) get_cons(u, cl.U);
+            if (W) get_prim(w, cl.W);
+            if (dU) get_cons(du, cl.dU);
+        }
+        if (Bf) {
+            const double* o = Bf + (size_t)b * 3 * NFACE;
+            for (int f = 0; f < NFACE; f++) {
+                p->Bfx[f] = o[f]; p->Bfy[f] = o[NFACE + f]; p->Bfz[f] = o[2 * NFACE + f];
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------ initial conditions
+// Returns 0 on success, 1 when the reference threw.
+int ref_fm_init(void* h) {
+    auto* t = static_cast<RefTree*>(h);
+    try { torus::init::fm_init(*t->tree); }
+    catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+// global rho_max over interior cells of every patch, main.cpp:101-111
+double ref_rho_max(void* h) {
+    double rho_max = 0.0;
+    for (const auto& p : static_cast<RefTree*>(h)->tree->quilt)
+        for (int i = 0; i < block; i++)
+            for (int j = 0; j < block; j++)
+                for (int k = 0; k < block; k++) rho_max = std::max(rho_max, p->cell_(i, j, k).W.rho);
+    return rho_max;
+}
+// B_pot_init over every patch in quilt order, main.cpp:142-145
+void ref_B_pot_init_all(void* h, double rho_max) {
+    auto* t = static_cast<RefTree*>(h);
+    for (const auto& p : t->tree->quilt) torus::init::B_pot_init(*p, t->tree->mtr, rho_max);
+}
+// U = ptoc(W) on interior cells, main.cpp:148-159
+int ref_ptoc_interior(void* h) {
+    auto* t = static_cast<RefTree*>(h);
+    try {
+        for (const auto& p : t->tree->quilt)
+            for (int i = 0; i < block; i++)
+                for (int j = 0; j < block; j++)
+                    for (int k = 0; k < block; k++) {
+                        cell& c = p->cell_(i, j, k);
+                        c.U = t->tree->cnsv.ptoc(c.W, c.r, c.th);
+                    }
+    } catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+int ref_regrid(void* h) {
+    auto* t = static_cast<RefTree*>(h);
+    try { t->tree->regrid(); }
+    catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+
+// ------------------------------------------------------------ the step
+// Each returns 0 on success, 1 if the reference threw (EOS domain error);
+// an exception escaping an OpenMP region would terminate the process, so
+// callers that expect a throw set threads to 1 first.
+void ref_set_threads(int n) { omp_set_num_threads(n); }
+
+int ref_step(void* h, double cfl, double* dt) {
+    auto* t = static_cast<RefTree*>(h);
+    try { integ::rk2integrator rk(cfl); *dt = rk.step(*t->tree); }
+    catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+int ref_dtcomp(void* h, double cfl, double* dt) {
+    auto* t = static_cast<RefTree*>(h);
+    try { *dt = integ::rk2integrator::dtcomp(*t->tree, cfl); }
+    catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+int ref_rkstg(void* h, double dt, int stage) {
+    auto* t = static_cast<RefTree*>(h);
+    try { integ::rk2integrator rk(0.4); rk.rkstg(*t->tree, dt, stage); }
+    catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+int ref_floors(void* h) {
+    auto* t = static_cast<RefTree*>(h);
+    try {
+        for (auto& p : t->tree->quilt) p->floors(*t->tree, *p, t->tree->stt, t->tree->mtr);
+    } catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+// The serial ghost phase alone (first loop of rkstg, integrator.cpp:195-199)
+int ref_ghosts_all(void* h) {
+    auto* t = static_cast<RefTree*>(h);
+    try {
+        for (auto& p : t->tree->quilt)
+            if (p->leaf) t->tree->ghosts(p.get());
+    } catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+// fluxcomp + CT on one patch; exports Fx/Fy/Fz ([8400][8] each) and EMFs
+int ref_flux_ct_patch(void* h, int b, double dt, double* F /*[3][8400][8]*/, double* E /*[3][8820]*/) {
+    auto* t = static_cast<RefTree*>(h);
+    try {
+        grid::patch* p = t->tree->quilt[b].get();
+        p->fluxcomp(t->tree->mtr, t->tree->stt);
+        constrans::emfcomp(*p);
+        if (F) {
+            for (int f = 0; f < NFACE; f++) {
+                put_cons(p->Fx[f], F + 8 * f);
+                put_cons(p->Fy[f], F + 8 * (NFACE + f));
+                put_cons(p->Fz[f], F + 8 * (2 * NFACE + f));
+            }
+        }
+        if (E) {
+            const int NE = cpn * (cpn + 1) * (cpn + 1);
+            for (int f = 0; f < NE; f++) { E[f] = p->EMFx[f]; E[NE + f] = p->EMFy[f]; E[2 * NE + f] = p->EMFz[f]; }
+        }
+        constrans::Bfupdate(*p, dt);
+        constrans::f2cB(*p);
+    } catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+// div_flux + source for every interior cell of patch b (after ref_flux_ct_patch)
+int ref_update_terms(void* h, int b, double* dF /*[8][4096]*/, double* S /*[8][4096]*/) {
+    auto* t = static_cast<RefTree*>(h);
+    try {
+        grid::patch* p = t->tree->quilt[b].get();
+        for (int i = 0; i < block; i++)
+            for (int j = 0; j < block; j++)
+                for (int k = 0; k < block; k++) {
+                    int c = (i * block + j) * block + k;
+                    double a[8], s[8];
+                    put_cons(integ::rk2integrator::div_flux(*p, i, j, k), a);
+                    put_cons(integ::rk2integrator::source(*p, i, j, k, t->tree->mtr, t->tree->stt), s);
+                    for (int v = 0; v < 8; v++) { dF[v * 4096 + c] = a[v]; S[v * 4096 + c] = s[v]; }
+                }
+    } catch (const std::exception& e) { snprintf(t->err, sizeof t->err, "%s", e.what()); return 1; }
+    return 0;
+}
+double ref_maxdivB(void* h, int b) {
+    return constrans::maxdivB(*static_cast<RefTree*>(h)->tree->quilt[b]);
+}
+// cell geometry (r, th, phi, xc, yc, zc, vol, pvol) of patch b
+void ref_cell_geom(void* h, int b, double* out /*[8000][8]*/) {
+    const grid::patch* p = static_cast<RefTree*>(h)->tree->quilt[b].get();
+    for (int c = 0; c < NCELL; c++) {
+        const cell& cl = p->cells[c];
+        double* o = out + 8 * c;
+        o[0] = cl.r; o[1] = cl.th; o[2] = cl.phi; o[3] = cl.xc; o[4] = cl.yc; o[5] = cl.zc; o[6] = cl.vol; o[7] = cl.pvol;
+    }
+}
+
+// ------------------------------------------------------ pointwise kernels
+// metriccomp flattened: g[16], g_inv[16], Gamma[64], sqrtdetg, gam[9],
+// gam_inv[9], sqrtdetgam, alpha, beta[3]  = 120 doubles
+void ref_metric(double M, double a, double Q, double r, double th, double* o) {
+    metric m(M, a, Q);
+    metriccomp mc = m.comp(r, th);
+    int n = 0;
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) o[n++] = mc.g[i][j];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) o[n++] = mc.g_inv[i][j];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) for (int k = 0; k < 4; k++) o[n++] = mc.Gamma[i][j][k];
+    o[n++] = mc.sqrtdetg;
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o[n++] = mc.gam[i][j];
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) o[n++] = mc.gam_inv[i][j];
+    o[n++] = mc.sqrtdetgam;
+    o[n++] = mc.alpha;
+    for (int i = 0; i < 3; i++) o[n++] = mc.beta[i];
+}
+
+// ptoc: prim[8] -> cons[8]; derived[4] = lor, p, h, b2.  Returns 1 on throw.
+int ref_ptoc(double M, double a, double Q, double gm, const double* w, double r, double th, double* u, double* der) {
+    metric m(M, a, Q); state s(gm); conserved cv(m, s);
+    prim pv; get_prim(w, pv);
+    try {
+        cons c = cv.ptoc(pv, r, th);
+        put_cons(c, u);
+        if (der) { der[0] = pv.lor; der[1] = pv.p; der[2] = pv.h; der[3] = pv.b2; }
+    } catch (const std::exception&) { return 1; }
+    return 0;
+}
+// ctop: cons[8] -> prim[8]; returns 0 ok, 1 false, 2 threw.
+// derived[5] = p, T, h, lor, b2
+int ref_ctop(double M, double a, double Q, double gm, const double* u, double r, double th, double* w, double* der) {
+    metric m(M, a, Q); state s(gm); conserved cv(m, s);
+    cons c; get_cons(u, c);
+    prim pv; std::memset(&pv, 0, sizeof pv);
+    try {
+        bool ok = cv.ctop(c, r, th, pv);
+        put_prim(pv, w);
+        if (der) { der[0] = pv.p; der[1] = pv.T; der[2] = pv.h; der[3] = pv.lor; der[4] = pv.b2; }
+        return ok ? 0 : 1;
+    } catch (const std::exception&) { return 2; }
+}
+// hlld at a face: returns 0 ok, 2 threw.  F[8] in cons order, ct = ctspeed
+int ref_hlld(double M, double a, double Q, double gm, const double* WL, const double* WR, const double* UL,
+             const double* UR, double r, double th, int dim, double* F, double* ct) {
+    metric m(M, a, Q); state s(gm);
+    prim wl, wr; get_prim(WL, wl); get_prim(WR, wr);
+    cons ul, ur; get_cons(UL, ul); get_cons(UR, ur);
+    try {
+        hlldflux fx = hlld(wl, wr, ul, ur, m, r, th, s, dim);
+        put_cons(fx.F, F); *ct = fx.ctspeed;
+    } catch (const std::exception&) { return 2; }
+    return 0;
+}
+// reconfp on patch b of a tree, at (i,j,k,dim) with limiter lim (0 minmod, 1 vanleer, 2 mc)
+void ref_reconfp(void* h, int b, int i, int j, int k, int dim, int lim, double* L, double* R) {
+    const grid::patch* p = static_cast<RefTree*>(h)->tree->quilt[b].get();
+    facestate fs = reconfp(*p, i, j, k, dim, static_cast<limiter>(lim));
+    put_prim(fs.L, L); put_prim(fs.R, R);
+}
+// patch::B_init on patch b
+void ref_B_init(void* h, int b) { static_cast<RefTree*>(h)->tree->quilt[b]->B_init(); }
+}

@@ -1,0 +1,143 @@
+"""GPU parity: the HIP path against the oracle, through the C ABI.
+
+The oracle runs in its correctly-rounded libm mode here (the device's log/exp
+are correctly rounded), so the comparison is bit for bit, NaN masks
+included.  Against the oracle's C-library mode (what the reference calls)
+the bar is relative L-infinity 1e-10 on finite primitives while the
+reference's own dynamics stay finite.
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+import bhfluid
+import configs
+from oracle_libs import OracleTree, State, bits_equal, first_diff, oracle_lib
+
+pytestmark = pytest.mark.gpu
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def P(a):
+    return a.ctypes.data_as(_dp)
+
+
+def build(cfg):
+    lo, hi, nqlt, regrid, a, Q = cfg
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    bhfluid.torus_setup(t, regrid=regrid)
+    return t, (1.0, a, Q, configs.GAMMA)
+
+
+def snapshot(t):
+    meta, edges = t.blocks()
+    U, W, Bf = t.state()
+    return State(meta, edges, U, W, Bf)
+
+
+def test_device_math_correctly_rounded():
+    L = bhfluid.lib()
+    O = oracle_lib()
+    rng = np.random.default_rng(7)
+    xs = {0: np.concatenate([np.exp(rng.uniform(-700, 700, 40000)), np.exp(rng.uniform(-40, 5, 40000)),
+                             1 + rng.uniform(-1e-6, 1e-6, 4000), np.array([1.0, 2.0, 0.5, 1e-300, 1e300])]),
+          1: np.concatenate([rng.uniform(-745, 709, 40000), rng.uniform(-40, 5, 40000), rng.uniform(-1e-8, 1e-8, 4000),
+                             np.array([0.0, -0.0, 1.0, 709.78, -745.0])])}
+    for op, x in xs.items():
+        y = np.zeros_like(x)
+        assert L.bh_math_batch(op, len(x), P(x), P(y), 1) == 0
+        q = np.array([O.or_log(v, 1) if op == 0 else O.or_exp(v, 1) for v in x])
+        n, ex = first_diff(y, q)
+        assert n == 0, f"op {op}: {n} results differ from the correctly rounded value, e.g. {ex}"
+
+
+def _random_prims(rng, n, r):
+    w = np.empty((n, 8))
+    w[:, 0] = np.exp(rng.uniform(np.log(1e-8), 0, n))
+    w[:, 1] = np.exp(rng.uniform(np.log(1e-7), 0, n))
+    w[:, 2] = rng.uniform(-0.3, 0.3, n)
+    w[:, 3] = rng.uniform(-0.3, 0.3, n) / r
+    w[:, 4] = rng.uniform(-0.3, 0.3, n) / r
+    w[:, 5:8] = rng.uniform(-0.3, 0.3, (n, 3))
+    return w
+
+
+def test_pointwise_ptoc_ctop_hlld_bitwise():
+    L = bhfluid.lib()
+    O = oracle_lib()
+    rng = np.random.default_rng(11)
+    n = 4000
+    for (a, Q) in ((0.9, 0.1), (0.0, 0.0), (0.9375, 0.0)):
+        r = np.exp(rng.uniform(np.log(1.5), np.log(60), n))
+        th = rng.uniform(0.02, np.pi - 0.02, n)
+        W = _random_prims(rng, n, r)
+        U = np.zeros_like(W)
+        L.bh_ptoc_batch(1.0, a, Q, configs.GAMMA, n, P(r), P(th), P(W), P(U))
+        Uo = np.zeros(8)
+        for i in range(0, n, 7):
+            O.or_ptoc(1.0, a, Q, configs.GAMMA, P(W[i].copy()), r[i], th[i], P(Uo), None)
+            assert bits_equal(U[i], Uo), (i, U[i], Uo)
+        Wd = np.zeros_like(U)
+        ok = np.zeros(n, np.int32)
+        L.bh_ctop_batch(1.0, a, Q, configs.GAMMA, n, P(r), P(th), P(U), P(Wd), ok.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        Wo = np.zeros(8)
+        for i in range(0, n, 5):
+            rc = O.or_ctop(1.0, a, Q, configs.GAMMA, P(U[i].copy()), r[i], th[i], P(Wo), None, 1)
+            assert (rc == 0) == bool(ok[i]), (i, rc, ok[i])
+            if ok[i]:
+                assert bits_equal(Wd[i], Wo), (i, Wd[i], Wo)
+        WR = _random_prims(rng, n, r)
+        UR = np.zeros_like(WR)
+        L.bh_ptoc_batch(1.0, a, Q, configs.GAMMA, n, P(r), P(th), P(WR), P(UR))
+        for dim in range(3):
+            F = np.zeros_like(W)
+            ct = np.zeros(n)
+            L.bh_hlld_batch(1.0, a, Q, configs.GAMMA, n, P(r), P(th), dim, P(W), P(WR), P(U), P(UR), P(F), P(ct))
+            Fo = np.zeros(8)
+            cto = ctypes.c_double(0)
+            for i in range(0, n, 9):
+                O.or_hlld(1.0, a, Q, configs.GAMMA, P(W[i].copy()), P(WR[i].copy()), P(U[i].copy()), P(UR[i].copy()),
+                          r[i], th[i], dim, P(Fo), ctypes.byref(cto))
+                assert bits_equal(F[i], Fo) and ct[i] == cto.value, (dim, i, F[i], Fo)
+
+
+def _run_parity(cfg, nsteps, name):
+    t, params = build(cfg)
+    s0 = snapshot(t)
+    ot = OracleTree.from_state(s0, params, libm=1)
+    rk = bhfluid.RK2Integrator(configs.CFL)
+    for step in range(nsteps):
+        print(f"[{name}] step {step}", flush=True)
+        try:
+            dt_g = rk.step(t)
+            fault_g = 0
+        except bhfluid.DomainError as e:
+            dt_g = None
+            fault_g = e.faults
+        dt_o, fault_o = ot.step(configs.CFL)
+        assert (fault_g > 0) == (fault_o > 0), f"step {step}: faults gpu={fault_g} oracle={fault_o}"
+        g = snapshot(t)
+        o = ot.state()
+        for nm, x, y in (("U", g.U, o.U), ("W", g.W, o.W), ("Bf", g.Bf, o.Bf)):
+            n, ex = first_diff(x, y)
+            assert n == 0, f"{name} step {step}: {nm} differs in {n} values, first {ex}"
+        if dt_g is not None:
+            assert dt_g == dt_o, (step, dt_g, dt_o)
+
+
+def test_step_R1_bitwise_10_steps():
+    _run_parity(configs.R1, 10, "R1")
+
+
+def test_step_R4_torus_bitwise():
+    _run_parity(configs.R4, 3, "R4")
+
+
+def test_step_R2_regridded_bitwise():
+    _run_parity(configs.R2, 2, "R2")
+
+
+def test_step_R3_two_levels_bitwise():
+    _run_parity(configs.R3, 2, "R3")
