@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark: zone-updates/s of the FP64 GRMHD step on one MI355X.
+
+Workload (the reference's own production set-up, src/fluid/main.cpp:61-168):
+Fishbone-Moncrief torus (a=0.9, Q=0.1, Gamma=5/3, r_in=4.5, r_max=12) with the
+vector-potential field, on [-50M, 50M]^3 tiled by 8^3 root patches of 16^3
+zones, after one regrid: 4544 patches, 4040 leaves on two levels
+(16.5 M leaf zones).  The arithmetic is the reference's (every quirk kept),
+in FP64.  A step is one rk2integrator::step (dtcomp, two RK stages, floors)
+over the whole tree.  W untimed warm-up steps, then the initial data is
+restored and K timed steps advance it.  Inputs are resident in HBM
+when the timed region starts.
+
+Prints one JSON line.  Per-kernel durations come from HIP events recorded
+on the tree's stream over the timed region; the dominant kernel's roofline
+uses its algorithmic bytes per launch (see ALGO_BYTES_PER_BLOCK).  The CPU
+baseline is the oracle (the C++/OpenMP restatement of the reference) on a
+bounded sample of the same grid, on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "black-hole-simulator_amd"))
+sys.path.insert(0, str(ROOT / "tests"))
+
+METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
+    "zone-updates/s"
+HBM_PEAK_GBS = 8000.0
+
+DOM = 50.0
+NQLT = 8
+A_SPIN, Q_CHARGE, GAMMA, CFL = 0.9, 0.1, 5.0 / 3.0, 0.4
+
+# Algorithmic HBM bytes per block and launch: the state a kernel must read
+# and write at least once (FP64), excluding static geometry tables and
+# scratch that a fused kernel would keep on chip.
+#   k_flux   : W on the 7168 stencil cells (8 words) + 13056 normal face B
+#              in, 13056 faces x 8 flux words out
+#   k_c2p    : U in, W out on 8000 cells
+#   k_ghost  : U, W of 3072 face-ghost sources in, U, W of 3904 ghost cells out
+#   k_update : U, W, RK store in (4096 x 21 words), U and store out (4096 x 13),
+#              fluxes 5 words on 3 x 4352 faces in
+#   k_emf    : 6 flux-B words on 3 x 4352 faces in, 3 x 6859 EMFs out
+#   k_bfupd  : 3 x 6859 EMFs in, 3 x 5832 face B in/out
+#   k_f2cb   : 3 x 8400 face B in, 6859 cells x 6 words out
+#   k_floors : U, W in and out on 4096 cells
+#   k_dtcomp : 5 W words on 4096 cells in
+W8 = 8
+ALGO_BYTES_PER_BLOCK = {
+    "k_flux": (7168 * 8 + 13056 + 13056 * 8) * W8,
+    "k_c2p": (8000 * 16) * W8,
+    "k_ghost": (3072 * 16 + 3904 * 16) * W8,
+    "k_update": (4096 * (21 + 13) + 3 * 4352 * 5) * W8,
+    "k_emf": (3 * 4352 * 6 + 3 * 6859) * W8,
+    "k_bfupd": (3 * 6859 + 2 * 3 * 5832) * W8,
+    "k_f2cb": (3 * 8400 + 6859 * 6) * W8,
+    "k_floors": (4096 * 32) * W8,
+    "k_dtcomp": (4096 * 5) * W8,
+}
+# launches per step and the blocks each launch covers ("leaf" or "all")
+KERNEL_SCOPE = {"k_flux": "all", "k_emf": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
+                "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf"}
+
+
+def build_tree(dom=DOM, nqlt=NQLT, regrid=True):
+    import bhfluid
+
+    t = bhfluid.AmrTree((-dom,) * 3, (dom,) * 3, nqlt, 1.0, A_SPIN, Q_CHARGE, GAMMA)
+    bhfluid.torus_setup(t, regrid=regrid)
+    return t
+
+
+def cpu_baseline():
+    """Oracle step on the central [-25M, 25M]^3 of the same grid (4^3 roots of
+    the same size, one regrid), one step, all host threads."""
+    import bhfluid
+    from oracle_libs import OracleTree, State, oracle_lib
+
+    t0 = time.perf_counter()
+    t = build_tree(dom=25.0, nqlt=4, regrid=True)
+    meta, edges = t.blocks()
+    U, W, Bf = t.state()
+    params = (1.0, A_SPIN, Q_CHARGE, GAMMA)
+    ot = OracleTree.from_state(State(meta, edges, U, W, Bf), params, libm=0)
+    threads = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS")
+    if omp:
+        threads = min(threads, int(omp))
+    oracle_lib().or_set_threads(threads)
+    leaf_zones = int(meta[:, 4].sum()) * 4096
+    t1 = time.perf_counter()
+    ot.step(CFL)
+    el = time.perf_counter() - t1
+    return {
+        "value": leaf_zones / el,
+        "unit": "zone-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"1 step of the oracle on the central [-25M,25M]^3 of the same grid (4^3 roots + 1 regrid, "
+                  f"{meta.shape[0]} patches, {leaf_zones} leaf zones), {el:.2f} s; setup {t1 - t0:.1f} s untimed",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nqlt", type=int, default=NQLT)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import bhfluid
+
+    t_setup = time.time()
+    tree = build_tree(nqlt=args.nqlt)
+    meta, _ = tree.blocks()
+    nb = meta.shape[0]
+    nleaf = int(meta[:, 4].sum())
+    levels = np.bincount(meta[meta[:, 4] == 1, 0]).tolist()
+    leaf_zones = nleaf * 4096
+    tree.prepare()
+    setup_s = time.time() - t_setup
+
+    U0, W0, Bf0 = tree.state()
+    faults = 0
+    if args.warmup:
+        try:
+            bhfluid.run_steps(tree, args.warmup, CFL)
+        except bhfluid.DomainError:
+            pass
+        # the timed steps start again from the initial data
+        tree.set_state(U0, W0, Bf0)
+        tree.prepare()
+    del U0, W0, Bf0
+    tree.profile(True)
+    bhfluid.device_sync()
+    t0 = time.perf_counter()
+    try:
+        dts = bhfluid.run_steps(tree, args.steps, CFL)
+    except bhfluid.DomainError as e:
+        faults += e.faults
+        dts = None
+    bhfluid.device_sync()
+    el = time.perf_counter() - t0
+    prof = tree.profile_report()
+    tree.profile(False)
+    U, _, _ = tree.state()
+    leaf = meta[:, 4] == 1
+    Ui = U.reshape(nb, 8, 20, 20, 20)[leaf][:, :, 2:18, 2:18, 2:18]
+    nonfinite = int((~np.isfinite(Ui).all(axis=1)).sum())
+    del U, Ui
+
+    value = leaf_zones * args.steps / el
+    ms_per_step = 1e3 * el / args.steps
+    # dominant kernel by total time
+    dom_name = max(prof, key=lambda k: prof[k][1])
+    n_launch, tot_ms = prof[dom_name]
+    avg_ms = tot_ms / n_launch
+    blocks = nb if KERNEL_SCOPE[dom_name] == "all" else nleaf
+    if dom_name == "k_ghost":
+        blocks = nleaf / max(1, n_launch / (2 * args.steps))  # one launch per wave
+    algo = ALGO_BYTES_PER_BLOCK[dom_name] * blocks
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "avg_ms": round(v[1] / v[0], 4)}
+               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline()
+        except Exception as e:  # reported, not fatal
+            cpu = {"error": repr(e)}
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"main.cpp torus: FM torus a={A_SPIN} Q={Q_CHARGE} Gamma=5/3 on [-{DOM:g}M,{DOM:g}M]^3, "
+                        f"{args.nqlt}^3 roots + 1 regrid, reference arithmetic in FP64",
+            "patches": nb,
+            "leaves": nleaf,
+            "leaf_zones_per_level": [n * 4096 for n in levels],
+            "leaf_zones": leaf_zones,
+            "cfl": CFL,
+            "eos_faults_in_timed_steps": faults,
+            "nonfinite_leaf_zones_after": nonfinite,
+            "setup_s": round(setup_s, 1),
+            "kernels": kernels,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": dom_name,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "algo_bytes_per_launch": algo,
+            "avg_launch_ms": avg_ms,
+        },
+        "cpu_baseline": cpu,
+    }
+    if dts is not None:
+        line["config"]["dt_first_last"] = [float(dts[0]), float(dts[-1])]
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -101,6 +101,11 @@ def lib():
         _sig(L, "bh_tree_floors", _i, [_v])
         _sig(L, "bh_rk2_advance", _i, [_v, _d, _i, _d, _dp, _ip])
         _sig(L, "bh_tree_take_faults", _i, [_v, ctypes.POINTER(ctypes.c_longlong)])
+        _sig(L, "bh_rk2_steps", _i, [_v, _d, _i, _dp])
+        _sig(L, "bh_tree_prepare", _i, [_v])
+        _sig(L, "bh_profile", _i, [_v, _i])
+        _sig(L, "bh_profile_report", _i, [_v, ctypes.c_char_p, _i])
+        _sig(L, "bh_device_sync", _i, [])
         _sig(L, "bh_tree_maxdivB", _i, [_v, _i, _dp])
         _sig(L, "bh_ptoc_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _dp, _dp])
         _sig(L, "bh_ctop_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _dp, _dp, _ip])
@@ -246,6 +251,24 @@ class AmrTree:
         """``patch::floors`` on every patch (grid.cpp:331-358)."""
         self._check(self._L.bh_tree_floors(self._h), "floors")
 
+    def prepare(self):
+        """Build the device copy (geometry tables, ghost plan, state upload)."""
+        self._check(self._L.bh_tree_prepare(self._h), "prepare")
+
+    def profile(self, enable: bool = True):
+        """Per-kernel HIP-event timing of subsequent launches."""
+        self._check(self._L.bh_profile(self._h, 1 if enable else 0), "profile")
+
+    def profile_report(self):
+        """{kernel: (launches, total_ms)} accumulated since profiling was enabled."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(self._L.bh_profile_report(self._h, buf, 1 << 16), "profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            out[name] = (int(n), float(ms))
+        return out
+
     def maxdivB(self, b: int) -> float:
         """``constrans::maxdivB`` of block b (grid/transport.cpp:77-89)."""
         out = ctypes.c_double(0.0)
@@ -320,6 +343,20 @@ class RK2Integrator:
                                     float(t_end if t_end is not None else np.inf), ctypes.byref(tt), ctypes.byref(n))
         tree._check(rc, "advance")
         return tt.value, n.value
+
+
+def run_steps(tree: AmrTree, nsteps: int, cfl: float = 0.4) -> np.ndarray:
+    """nsteps RK2 steps enqueued back to back, one synchronisation at the end;
+    returns each step's dt.  Raises DomainError after the run if any step saw
+    rho <= 0."""
+    dts = np.zeros(max(int(nsteps), 1))
+    tree._check(tree._L.bh_rk2_steps(tree._h, float(cfl), int(nsteps), _p(dts)), "steps")
+    return dts[:nsteps]
+
+
+def device_sync():
+    if lib().bh_device_sync() != 0:
+        raise BhError("hipDeviceSynchronize failed")
 
 
 def device_info() -> str:

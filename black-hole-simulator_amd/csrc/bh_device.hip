@@ -65,6 +65,10 @@ struct KP {
     const short* ghost_cells;  // [3904]
     unsigned long long* dtbits;  // dt as an ordered bit pattern
     unsigned long long* faults;
+    // cst[b][c] = 1 when W holds the successful con2prim of the cell's current
+    // U; a repeated con2prim of the same U is then skipped (it would return
+    // the same W).  Every kernel that writes U keeps it current.
+    unsigned char* cst;
 };
 
 __device__ __forceinline__ double& Ua(const KP& P, int b, int v, int c) { return P.U[((size_t)b * 8 + v) * NCELL + c]; }
@@ -115,19 +119,22 @@ __device__ __forceinline__ void add_faults(const KP& P, int nf) {
 
 // con2prim of one cell with the floor fallback of the c2p phase
 // (integrator.cpp:205-220): floor state keeps the cell's B, U recomputed.
-__device__ __forceinline__ void c2p_cell(const KP& P, int b, int c, int& nf) {
+__device__ __forceinline__ bool c2p_cell(const KP& P, int b, int c, int& nf) {
     GPt m = cell_tree(P, b, c);
     Prim pv;
     if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
         storeW(P, b, c, pv);
+        return true;
     } else {
         const double* fl = P.gfl + (size_t)b * 2 * NCELL;
         Prim f = floor_prim(fl[c], fl[NCELL + c]);
         f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
         storeW(P, b, c, f);
         storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+        return false;
     }
 }
+__device__ __forceinline__ unsigned char& cst_at(const KP& P, int b, int c) { return P.cst[(size_t)b * NCELL + c]; }
 
 __device__ __forceinline__ int cell_of(int dim, int a, int t1, int t2) {
     return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
@@ -183,7 +190,12 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                 double st2 = minmod(vc - Ua(P, nb, v, lindx(ci, cj, ck - 1)), Ua(P, nb, v, lindx(ci, cj, ck + 1)) - vc);
                 Ua(P, b, v, c) = vc + sn * on + st1 * ot1 + st2 * ot2;
             }
-            c2p_cell(P, b, c, nf);
+            if (c2p_cell(P, b, c, nf)) {
+                // the ghost con2prim below would repeat this one on the same U
+                cst_at(P, b, c) = 1;
+                add_faults(P, nf);
+                return;
+            }
         }
     }
     // ghost con2prim, grid.cpp:538-570
@@ -191,8 +203,9 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
     Prim pv;
     if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
         storeW(P, b, c, pv);
-        eos_press(P.gamma, pv.rho, pv.eps, nf);
+        cst_at(P, b, c) = 1;
     } else {
+        cst_at(P, b, c) = 0;
         const double* fl = P.gfl + (size_t)b * 2 * NCELL;
         Prim w = loadW(P, b, c);
         w.rho = fl[c];
@@ -209,8 +222,9 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     const int b = leaves[blockIdx.x / WG_CELL];
     const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
     if (c >= NCELL) return;
+    if (cst_at(P, b, c)) return;
     int nf = 0;
-    c2p_cell(P, b, c, nf);
+    cst_at(P, b, c) = c2p_cell(P, b, c, nf) ? 1 : 0;
     add_faults(P, nf);
 }
 
@@ -340,6 +354,7 @@ __global__ __launch_bounds__(TPB) void k_f2cb(KP P) {
     Wa(P, b, WB0, c) = b0; Wa(P, b, WB1, c) = b1; Wa(P, b, WB2, c) = b2;
     double sg = cell_sqrtdetg(P, b, c);
     Ua(P, b, UB0, c) = sg * b0; Ua(P, b, UB1, c) = sg * b1; Ua(P, b, UB2, c) = sg * b2;
+    cst_at(P, b, c) = 0;
 }
 
 // ---------------------------------------------------------------- update
@@ -429,6 +444,7 @@ __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
         Ua(P, b, v, c) = nu;
     }
     for (int d = 0; d < 3; d++) Ua(P, b, UB0 + d, c) = sg * W.B[d];
+    cst_at(P, b, c) = 0;
     add_faults(P, nf);
 }
 
@@ -478,12 +494,13 @@ __global__ __launch_bounds__(TPB) void k_floors(KP P) {
     GPt m = cell_tree(P, b, c);
     const double D = Ua(P, b, UD, c), tau = Ua(P, b, UTAU, c);
     bool bhl = !isfinite(D) || !isfinite(tau) || D <= 0.0 || tau <= 0.0;
+    unsigned char st = 0;
     if (!bhl) {
         Prim pv;
         bhl = !ctop(m, P.gamma, loadU(P, b, c), pv, nf);
         if (!bhl) {
             bool hl = !(pv.rho <= 0.0 || pv.eps <= 0.0);
-            if (hl) storeW(P, b, c, pv);
+            if (hl) { storeW(P, b, c, pv); st = 1; }
         }
     }
     const double* fl = P.gfl + (size_t)b * 2 * NCELL;
@@ -492,7 +509,9 @@ __global__ __launch_bounds__(TPB) void k_floors(KP P) {
         f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
         storeW(P, b, c, f);
         storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+        st = 0;
     }
+    cst_at(P, b, c) = st;
     add_faults(P, nf);
 }
 
@@ -607,9 +626,55 @@ public:
     std::vector<int> wave_off;  // offsets into waveblk
     DBuf<short> ghost_cells;
     DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
+    DBuf<double> dts;               // per-step dt of a multi-step run
+    DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
+
+    // Per-kernel timing with HIP events on this tree's stream.
+    bool profile = false;
+    struct Rec { const char* name; hipEvent_t a, b; };
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> pool;
+    struct Acc { std::string name; double ms = 0; long n = 0; };
+    std::vector<Acc> acc;
+
+    hipEvent_t ev() {
+        if (!pool.empty()) { hipEvent_t e = pool.back(); pool.pop_back(); return e; }
+        hipEvent_t e;
+        HIPCHK(hipEventCreate(&e));
+        return e;
+    }
+    template <class Kern, class... Args>
+    void launch(const char* name, Kern k, dim3 g, Args... args) {
+        if (profile) {
+            Rec r{name, ev(), ev()};
+            HIPCHK(hipEventRecord(r.a, stream));
+            hipLaunchKernelGGL(k, g, dim3(TPB), 0, stream, args...);
+            HIPCHK(hipEventRecord(r.b, stream));
+            recs.push_back(r);
+        } else {
+            hipLaunchKernelGGL(k, g, dim3(TPB), 0, stream, args...);
+        }
+    }
+    void collect() {
+        if (recs.empty()) return;
+        HIPCHK(hipStreamSynchronize(stream));
+        for (auto& r : recs) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+            auto it = std::find_if(acc.begin(), acc.end(), [&](const Acc& x) { return x.name == r.name; });
+            if (it == acc.end()) { acc.push_back({r.name}); it = acc.end() - 1; }
+            it->ms += ms;
+            it->n += 1;
+            pool.push_back(r.a);
+            pool.push_back(r.b);
+        }
+        recs.clear();
+    }
 
     DeviceTree() { HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
     ~DeviceTree() {
+        for (auto& r : recs) { pool.push_back(r.a); pool.push_back(r.b); }
+        for (auto e : pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 
@@ -620,6 +685,7 @@ public:
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p;
         P.ghtab = ghtab.p; P.ghost_cells = ghost_cells.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
+        P.cst = cst.p;
         return P;
     }
 
@@ -664,6 +730,7 @@ public:
         ghost_cells.alloc(gc.size()); ghost_cells.upload(gc.data(), gc.size(), stream);
         scal.alloc(2);
         HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
+        cst.alloc((size_t)nb * NCELL);
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -673,6 +740,7 @@ public:
         U.upload(u.data(), u.size(), stream);
         W.upload(w.data(), w.size(), stream);
         Bf.upload(bf.data(), bf.size(), stream);
+        HIPCHK(hipMemsetAsync(cst.p, 0, cst.n, stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
     void download_state(HostTree& t) {
@@ -686,9 +754,9 @@ public:
 
     void launch_dtcomp(double cfl) {
         KP P = kp();
-        hipLaunchKernelGGL(k_dtinit, dim3(1), dim3(64), 0, stream, P);
+        launch("k_dtinit", k_dtinit, dim3(1), P);
         int nl = (int)plan.leaves.size();
-        if (nl) hipLaunchKernelGGL(k_dtcomp, dim3(nl * WG_INT), dim3(TPB), 0, stream, P, leaves.p, cfl);
+        if (nl) launch("k_dtcomp", k_dtcomp, dim3(nl * WG_INT), P, (const int*)leaves.p, cfl);
     }
     void set_dt(double dt) {
         unsigned long long bits;
@@ -700,25 +768,32 @@ public:
         KP P = kp();
         for (size_t w = 0; w + 1 < wave_off.size(); w++) {
             int n = wave_off[w + 1] - wave_off[w];
-            if (n) hipLaunchKernelGGL(k_ghost, dim3(n * WG_GHOST), dim3(TPB), 0, stream, P, waveblk.p + wave_off[w]);
+            if (n) launch("k_ghost", k_ghost, dim3(n * WG_GHOST), P, (const int*)(waveblk.p + wave_off[w]));
         }
         int nl = (int)plan.leaves.size();
-        if (nl) hipLaunchKernelGGL(k_c2p, dim3(nl * WG_CELL), dim3(TPB), 0, stream, P, leaves.p);
-        hipLaunchKernelGGL(k_flux, dim3(nb * WG_FACE), dim3(TPB), 0, stream, P);
-        hipLaunchKernelGGL(k_emf, dim3(nb * WG_EDGE), dim3(TPB), 0, stream, P);
-        hipLaunchKernelGGL(k_bfupd, dim3(nb * WG_BF), dim3(TPB), 0, stream, P);
-        hipLaunchKernelGGL(k_f2cb, dim3(nb * WG_EDGE), dim3(TPB), 0, stream, P);
-        hipLaunchKernelGGL(k_update, dim3(nb * WG_INT), dim3(TPB), 0, stream, P, stage);
+        if (nl) launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
+        launch("k_flux", k_flux, dim3(nb * WG_FACE), P);
+        launch("k_emf", k_emf, dim3(nb * WG_EDGE), P);
+        launch("k_bfupd", k_bfupd, dim3(nb * WG_BF), P);
+        launch("k_f2cb", k_f2cb, dim3(nb * WG_EDGE), P);
+        launch("k_update", k_update, dim3(nb * WG_INT), P, stage);
     }
     void launch_floors() {
         KP P = kp();
-        hipLaunchKernelGGL(k_floors, dim3(nb * WG_INT), dim3(TPB), 0, stream, P);
+        launch("k_floors", k_floors, dim3(nb * WG_INT), P);
     }
     void launch_step(double cfl) {
         launch_dtcomp(cfl);
         launch_rkstg(0);
         launch_rkstg(1);
         launch_floors();
+    }
+    void run_steps(double cfl, int nsteps) {
+        if ((int)dts.n < nsteps) dts.alloc(nsteps);
+        for (int s = 0; s < nsteps; s++) {
+            launch_step(cfl);
+            HIPCHK(hipMemcpyAsync(dts.p + s, scal.p, sizeof(double), hipMemcpyDeviceToDevice, stream));
+        }
     }
     double read_dt() {
         unsigned long long bits = 0;
@@ -981,6 +1056,50 @@ int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_i
         return finish_faults(t, nf);
     });
 }
+int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out) {
+    if (nsteps < 0) return fail(t, BH_EINVAL, "nsteps < 0");
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->run_steps(cfl, nsteps);
+        t->dev->sync();
+        device_changed(t);
+        if (dt_out && nsteps) t->dev->dts.download(dt_out, nsteps, t->dev->stream);
+        t->dev->sync();
+        return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_tree_prepare(bh_tree* t) {
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->sync();
+        return BH_OK;
+    });
+}
+int bh_profile(bh_tree* t, int enable) {
+    return guarded(t, [&] {
+        if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+        t->dev->collect();
+        t->dev->profile = enable != 0;
+        if (enable) t->dev->acc.clear();
+        return BH_OK;
+    });
+}
+int bh_profile_report(bh_tree* t, char* buf, int len) {
+    return guarded(t, [&] {
+        std::string out;
+        if (t->dev) {
+            t->dev->collect();
+            char line[160];
+            for (auto& a : t->dev->acc) {
+                std::snprintf(line, sizeof line, "%s %ld %.6f\n", a.name.c_str(), a.n, a.ms);
+                out += line;
+            }
+        }
+        if (buf && len > 0) std::snprintf(buf, len, "%s", out.c_str());
+        return (int)out.size() < len ? BH_OK : BH_EINVAL;
+    });
+}
+int bh_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? BH_OK : BH_EDEVICE; }
 int bh_tree_take_faults(bh_tree* t, long long* n) {
     if (!t || !n) return BH_EINVAL;
     *n = t->faults;

@@ -223,14 +223,28 @@ BH_HD bool ctop(const GPt& m, double gm, const Cons& cv, Prim& pv, int& nf) {
     }
     double xi = (xi_l + xi_h) / 2.0;
     bool nr_ok = true;
-    for (int it = 0; it < maxiter; it++) {
-        double dF;
-        double F = c2p_fdf(gm, xi, Ssq, Bsq, SB, D, tau, dF);
-        if (fabs(F) < tol) break;
-        if (fabs(dF) < 1e-30) { nr_ok = false; break; }
-        double xin = xi - F / dF;
-        if (xin < xi_l || xin > xi_h) { nr_ok = false; break; }
-        xi = xin;
+    // Newton iteration.  The iterate is a pure function of the previous one,
+    // so once it is NaN, or repeats with period 1 or 2, every remaining pass
+    // is determined: the loop stops early with the value the full 500 passes
+    // would end on (NaN stays NaN; a 2-cycle ends on the parity of the
+    // remaining count).
+    double xprev = xi;
+    if (xi == xi) {
+        for (int it = 0; it < maxiter; it++) {
+            double dF;
+            double F = c2p_fdf(gm, xi, Ssq, Bsq, SB, D, tau, dF);
+            if (fabs(F) < tol) break;
+            if (fabs(dF) < 1e-30) { nr_ok = false; break; }
+            double xin = xi - F / dF;
+            if (xin < xi_l || xin > xi_h) { nr_ok = false; break; }
+            if (xin != xin || xin == xi) { xi = xin; break; }
+            if (it > 0 && xin == xprev) {
+                if (((maxiter - 1 - it) & 1) == 0) xi = xin;  // else the iterate stays at xi
+                break;
+            }
+            xprev = xi;
+            xi = xin;
+        }
     }
     if (!nr_ok) {
         double xib;

@@ -72,6 +72,19 @@ int bh_tree_floors(bh_tree* t);
 /* The time loop of src/fluid/main.cpp:169-173: step while t < t_end and
  * steps < nsteps; *t_io is advanced, *steps_out counts steps taken. */
 int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_io, int* steps_out);
+/* nsteps consecutive steps enqueued back to back with a single host
+ * synchronisation at the end (the step sequence of bh_rk2_advance without a
+ * t_end test); dt_out[nsteps] receives each step's dt.  No counterpart in
+ * the reference; used by the benchmark. */
+int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out);
+/* Build the device copy (geometry, ghost plan, state upload) ahead of the
+ * first step, so that set-up is not timed with it. */
+int bh_tree_prepare(bh_tree* t);
+/* Per-kernel timing with HIP events on the tree's stream: enable, then read
+ * "name launches total_ms" lines.  Measurement only. */
+int bh_profile(bh_tree* t, int enable);
+int bh_profile_report(bh_tree* t, char* buf, int len);
+int bh_device_sync(void);
 /* Equation-of-state domain faults (rho <= 0) since the last call. */
 int bh_tree_take_faults(bh_tree* t, long long* n);
 /* constrans::maxdivB of block b, src/fluid/grid/transport.cpp:77-89 */
