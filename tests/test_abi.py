@@ -1,0 +1,90 @@
+"""The C ABI: the library loads, exports every entry point include/bhfluid.h
+declares, and the host-side part of the API (tree construction, initial
+data, regrid, state exchange, error codes) works without a GPU.  The step
+itself must fail loudly when no device is present; there is no CPU path."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import bhfluid
+import configs
+
+HDR = Path(__file__).resolve().parents[1] / "include" / "bhfluid.h"
+
+
+def declared_symbols():
+    text = HDR.read_text()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char\*)\s+(bh_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    syms = declared_symbols()
+    assert "bh_rk2_step" in syms and "bh_tree_create" in syms and len(syms) >= 25
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(str(bhfluid.library_path()))
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_host_api_roundtrip():
+    lo, hi, nqlt, regrid, a, q = configs.R3
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    assert t.nblocks == nqlt ** 3
+    rho_max = bhfluid.torus_setup(t, regrid=True)
+    assert rho_max > 0
+    meta, edges = t.blocks()
+    assert meta.shape == (t.nblocks, 14) and edges.shape == (t.nblocks, 3, 17)
+    levels = set(meta[meta[:, 4] == 1, 0])
+    assert levels == {0, 1}  # two refinement levels among the leaves
+    # parents and children are consistent
+    for b, m in enumerate(meta):
+        for c in m[6:]:
+            if c >= 0:
+                assert meta[c, 5] == b and meta[c, 0] == m[0] + 1
+    U, W, Bf = t.state()
+    U2 = U.copy()
+    U2[0, 0, 0] = 123.0
+    t.set_state(U=U2)
+    assert t.state()[0][0, 0, 0] == 123.0
+    with pytest.raises(ValueError):
+        t.set_state(U=U[:1])
+
+
+def test_invalid_arguments_rejected():
+    with pytest.raises(ValueError):
+        bhfluid.AmrTree((0, 0), (1, 1, 1), 1)
+    with pytest.raises(ValueError):
+        bhfluid.AmrTree((0, 0, 0), (1, 1, 1), 0)
+    L = bhfluid.lib()
+    lo = np.zeros(3)
+    h = ctypes.c_void_p()
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert L.bh_tree_create(lo.ctypes.data_as(dp), lo.ctypes.data_as(dp), 0, 1.0, 0.0, 0.0, 5 / 3,
+                            ctypes.byref(h)) < 0
+    assert L.bh_last_error(None)
+
+
+def test_rkstg_rejects_bad_stage():
+    t = bhfluid.AmrTree(*configs.R0[:3], 1.0, 0.9, 0.1, configs.GAMMA)
+    with pytest.raises(bhfluid.BhError):
+        bhfluid.RK2Integrator().rkstg(t, 0.1, 2)
+
+
+@pytest.mark.skipif(Path("/dev/kfd").exists(), reason="a GPU is present")
+def test_step_without_gpu_fails_loudly():
+    t = bhfluid.AmrTree(*configs.R0[:3], 1.0, 0.9, 0.1, configs.GAMMA)
+    bhfluid.torus_setup(t)
+    with pytest.raises(bhfluid.BhError):
+        bhfluid.RK2Integrator(0.4).step(t)
+
+
+def test_fm_init_without_torus_returns_false():
+    # a box above the torus holds no torus cells: the reference prints an
+    # error and returns (initconds.cpp:483-486)
+    t = bhfluid.AmrTree((-5, -5, 30), (5, 5, 40), 1, 1.0, 0.9, 0.1, configs.GAMMA)
+    assert bhfluid.init.fm_init(t) is False
