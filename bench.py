@@ -63,10 +63,13 @@ ALGO_BYTES_PER_BLOCK = {
     "k_f2cb": (3 * 8400 + 6859 * 6) * W8,
     "k_floors": (4096 * 32) * W8,
     "k_dtcomp": (4096 * 5) * W8,
+    "k_logs": (8000 * 4) * W8,
+    "k_dtreduce": 0,
 }
 # launches per step and the blocks each launch covers ("leaf" or "all")
 KERNEL_SCOPE = {"k_flux": "all", "k_emf": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
-                "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf"}
+                "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf", "k_logs": "all",
+                "k_dtreduce": "leaf"}
 
 
 def build_tree(dom=DOM, nqlt=NQLT, regrid=True):

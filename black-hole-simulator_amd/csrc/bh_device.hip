@@ -69,6 +69,7 @@ struct KP {
     // U; a repeated con2prim of the same U is then skipped (it would return
     // the same W).  Every kernel that writes U keeps it current.
     unsigned char* cst;
+    double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
 };
 
 __device__ __forceinline__ double& Ua(const KP& P, int b, int v, int c) { return P.U[((size_t)b * 8 + v) * NCELL + c]; }
@@ -228,6 +229,18 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     add_faults(P, nf);
 }
 
+// ln(rho) and ln(eps) of every stored cell, for the log-space reconstruction
+// (reconstruction.cpp:49-61 takes the logs of the four stencil cells of
+// each face; each cell's value is the same for every face that uses it)
+__global__ __launch_bounds__(TPB) void k_logs(KP P) {
+    const int b = blockIdx.x / WG_CELL;
+    const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
+    if (c >= NCELL) return;
+    double* lr = P.lg + (size_t)b * 2 * NCELL;
+    lr[c] = cr_log(Wa(P, b, WRHO, c));
+    lr[NCELL + c] = cr_log(Wa(P, b, WEPS, c));
+}
+
 // ---------------------------------------------------------------- fluxes
 // patch::fluxcomp, grid.cpp:255-329: PLM reconstruction (log rho, log eps,
 // linear v and transverse B; reconstruction.cpp:40-97), the normal field
@@ -250,11 +263,12 @@ __global__ __launch_bounds__(TPB) void k_flux(KP P) {
     Prim L, R;
     {
         double fl, fr;
-        recon_scal(cr_log(Wa(P, b, WRHO, cm)), cr_log(Wa(P, b, WRHO, c0)), cr_log(Wa(P, b, WRHO, cp)),
-                   cr_log(Wa(P, b, WRHO, cpp)), P.limiter, fl, fr);
+        // ln(rho), ln(eps) of the four stencil cells, computed once per cell by k_logs
+        const double* lr = P.lg + (size_t)b * 2 * NCELL;
+        const double* le = lr + NCELL;
+        recon_scal(lr[cm], lr[c0], lr[cp], lr[cpp], P.limiter, fl, fr);
         L.rho = cr_exp(fl); R.rho = cr_exp(fr);
-        recon_scal(cr_log(Wa(P, b, WEPS, cm)), cr_log(Wa(P, b, WEPS, c0)), cr_log(Wa(P, b, WEPS, cp)),
-                   cr_log(Wa(P, b, WEPS, cpp)), P.limiter, fl, fr);
+        recon_scal(le[cm], le[c0], le[cp], le[cpp], P.limiter, fl, fr);
         L.eps = cr_exp(fl); R.eps = cr_exp(fr);
         for (int d = 0; d < 3; d++) {
             recon_scal(Wa(P, b, WV0 + d, cm), Wa(P, b, WV0 + d, c0), Wa(P, b, WV0 + d, cp), Wa(P, b, WV0 + d, cpp),
@@ -455,7 +469,8 @@ __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
 __global__ void k_dtinit(KP P) {
     if (threadIdx.x == 0 && blockIdx.x == 0) *P.dtbits = (unsigned long long)__double_as_longlong(DBL_MAX);
 }
-__global__ __launch_bounds__(TPB) void k_dtcomp(KP P, const int* __restrict__ leaves, double cfl) {
+__global__ __launch_bounds__(TPB) void k_dtcomp(KP P, const int* __restrict__ leaves, double cfl,
+                                                unsigned long long* __restrict__ partial) {
     const int b = leaves[blockIdx.x / WG_INT];
     const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
     const int c = lindx(a / 256, (a / 16) % 16, a % 16);
@@ -480,8 +495,32 @@ __global__ __launch_bounds__(TPB) void k_dtcomp(KP P, const int* __restrict__ le
         unsigned long long o = __shfl_xor(best, off);
         best = o < best ? o : best;
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(P.dtbits, best);
+    __shared__ unsigned long long wmin[TPB / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmin[0];
+        for (int w = 1; w < TPB / 64; w++) m = wmin[w] < m ? wmin[w] : m;
+        partial[blockIdx.x] = m;
+    }
     add_faults(P, nf);
+}
+// min over the per-workgroup candidates (one workgroup)
+__global__ __launch_bounds__(TPB) void k_dtreduce(KP P, const unsigned long long* __restrict__ partial, int n) {
+    unsigned long long best = (unsigned long long)__double_as_longlong(DBL_MAX);
+    for (int i = threadIdx.x; i < n; i += TPB) best = partial[i] < best ? partial[i] : best;
+    for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(best, off);
+        best = o < best ? o : best;
+    }
+    __shared__ unsigned long long wmin[TPB / 64];
+    if ((threadIdx.x & 63) == 0) wmin[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = wmin[0];
+        for (int w = 1; w < TPB / 64; w++) m = wmin[w] < m ? wmin[w] : m;
+        *P.dtbits = m;
+    }
 }
 
 // --------------------------------------------------------------- floors
@@ -628,6 +667,8 @@ public:
     DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
     DBuf<double> dts;               // per-step dt of a multi-step run
     DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
+    DBuf<unsigned long long> dtpart;  // per-workgroup dt candidates
+    DBuf<double> lg;                // [nb][2][8000] log(rho), log(eps)
 
     // Per-kernel timing with HIP events on this tree's stream.
     bool profile = false;
@@ -686,6 +727,7 @@ public:
         P.ghtab = ghtab.p; P.ghost_cells = ghost_cells.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
+        P.lg = lg.p;
         return P;
     }
 
@@ -731,6 +773,8 @@ public:
         scal.alloc(2);
         HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
         cst.alloc((size_t)nb * NCELL);
+        dtpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT));
+        lg.alloc((size_t)nb * 2 * NCELL);
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -754,9 +798,13 @@ public:
 
     void launch_dtcomp(double cfl) {
         KP P = kp();
-        launch("k_dtinit", k_dtinit, dim3(1), P);
         int nl = (int)plan.leaves.size();
-        if (nl) launch("k_dtcomp", k_dtcomp, dim3(nl * WG_INT), P, (const int*)leaves.p, cfl);
+        if (!nl) {
+            launch("k_dtinit", k_dtinit, dim3(1), P);
+            return;
+        }
+        launch("k_dtcomp", k_dtcomp, dim3(nl * WG_INT), P, (const int*)leaves.p, cfl, dtpart.p);
+        launch("k_dtreduce", k_dtreduce, dim3(1), P, (const unsigned long long*)dtpart.p, nl * WG_INT);
     }
     void set_dt(double dt) {
         unsigned long long bits;
@@ -772,6 +820,7 @@ public:
         }
         int nl = (int)plan.leaves.size();
         if (nl) launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
+        launch("k_logs", k_logs, dim3(nb * WG_CELL), P);
         launch("k_flux", k_flux, dim3(nb * WG_FACE), P);
         launch("k_emf", k_emf, dim3(nb * WG_EDGE), P);
         launch("k_bfupd", k_bfupd, dim3(nb * WG_BF), P);

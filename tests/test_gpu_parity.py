@@ -42,7 +42,9 @@ def test_device_math_correctly_rounded():
     O = oracle_lib()
     rng = np.random.default_rng(7)
     xs = {0: np.concatenate([np.exp(rng.uniform(-700, 700, 40000)), np.exp(rng.uniform(-40, 5, 40000)),
-                             1 + rng.uniform(-1e-6, 1e-6, 4000), np.array([1.0, 2.0, 0.5, 1e-300, 1e300])]),
+                             1 + rng.uniform(-1e-6, 1e-6, 4000), 1 + rng.uniform(-2 ** -7, 2 ** -7, 20000),
+                             1 + np.round(rng.uniform(-1e5, 1e5, 4000)) * 2.0 ** -52,
+                             np.array([1.0, 2.0, 0.5, 1e-300, 1e300])]),
           1: np.concatenate([rng.uniform(-745, 709, 40000), rng.uniform(-40, 5, 40000), rng.uniform(-1e-8, 1e-8, 4000),
                              np.array([0.0, -0.0, 1.0, 709.78, -745.0])])}
     for op, x in xs.items():
