@@ -44,6 +44,7 @@ constexpr int WG_FACE = (3 * NFC + TPB - 1) / TPB;   // 51
 constexpr int WG_EDGE = (19 * 19 * 19 + TPB - 1) / TPB;  // 27
 constexpr int WG_BF = (18 * 18 * 18 + TPB - 1) / TPB;    // 23
 constexpr int WG_INT = NINT / TPB;                   // 16
+constexpr int BRENT_WG = 2048;                       // grid of the Brent pass
 
 // Kernel view of the device tree.
 struct KP {
@@ -70,7 +71,15 @@ struct KP {
     // the same W).  Every kernel that writes U keeps it current.
     unsigned char* cst;
     double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
+    // Cells whose con2prim needs the Brent fallback are queued by the first
+    // pass (Newton for every cell) and finished by k_brent over a compact
+    // list, so that a few slow cells do not hold whole wavefronts.
+    unsigned int* qn;     // queue length
+    int* qb;              // block
+    int* qc;              // cell | kind << 16
+    double* qx;           // Newton bracket upper end
 };
+enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3 };
 
 __device__ __forceinline__ double& Ua(const KP& P, int b, int v, int c) { return P.U[((size_t)b * 8 + v) * NCELL + c]; }
 __device__ __forceinline__ double& Wa(const KP& P, int b, int v, int c) { return P.W[((size_t)b * 8 + v) * NCELL + c]; }
@@ -137,6 +146,42 @@ __device__ __forceinline__ bool c2p_cell(const KP& P, int b, int c, int& nf) {
 }
 __device__ __forceinline__ unsigned char& cst_at(const KP& P, int b, int c) { return P.cst[(size_t)b * NCELL + c]; }
 
+__device__ __forceinline__ void push_brent(const KP& P, int b, int c, int kind, double xi_h) {
+    unsigned int i = atomicAdd(P.qn, 1u);
+    P.qb[i] = b;
+    P.qc[i] = c | (kind << 16);
+    P.qx[i] = xi_h;
+}
+// c2p failure path (integrator.cpp:211-218): floor state with the cell's B
+__device__ __forceinline__ void c2p_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    Prim f = floor_prim(fl[c], fl[NCELL + c]);
+    f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
+    storeW(P, b, c, f);
+    storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+}
+// ghost con2prim failure path (grid.cpp:559-566): rho, eps floored, v and B kept
+__device__ __forceinline__ void ghost_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    Prim w = loadW(P, b, c);
+    w.rho = fl[c];
+    w.eps = fl[NCELL + c];
+    Wa(P, b, WRHO, c) = w.rho;
+    Wa(P, b, WEPS, c) = w.eps;
+    storeU(P, b, c, ptoc(m, P.gamma, w, nf));
+}
+// First con2prim pass: 1 = solved (W written when ok), 0 = failed, -1 = queued
+__device__ __forceinline__ int c2p_first(const KP& P, int b, int c, const GPt& m, int kind, Prim& pv, int& nf) {
+    C2P s;
+    ctop_setup(m, loadU(P, b, c), s);
+    double xi, xi_h;
+    if (!ctop_newton(P.gamma, s, xi, xi_h)) {
+        push_brent(P, b, c, kind, xi_h);
+        return -1;
+    }
+    return ctop_finish(P.gamma, s, xi, pv, nf) ? 1 : 0;
+}
+
 __device__ __forceinline__ int cell_of(int dim, int a, int t1, int t2) {
     return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
 }
@@ -191,29 +236,30 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                 double st2 = minmod(vc - Ua(P, nb, v, lindx(ci, cj, ck - 1)), Ua(P, nb, v, lindx(ci, cj, ck + 1)) - vc);
                 Ua(P, b, v, c) = vc + sn * on + st1 * ot1 + st2 * ot2;
             }
-            if (c2p_cell(P, b, c, nf)) {
+            GPt mp = cell_tree(P, b, c);
+            Prim pv;
+            int r = c2p_first(P, b, c, mp, Q_PROLONG, pv, nf);
+            if (r < 0) { add_faults(P, nf); return; }
+            if (r > 0) {
                 // the ghost con2prim below would repeat this one on the same U
+                storeW(P, b, c, pv);
                 cst_at(P, b, c) = 1;
                 add_faults(P, nf);
                 return;
             }
+            c2p_floor(P, b, c, mp, nf);
         }
     }
     // ghost con2prim, grid.cpp:538-570
     GPt m = cell_tree(P, b, c);
     Prim pv;
-    if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
+    int r = c2p_first(P, b, c, m, Q_GHOST, pv, nf);
+    if (r > 0) {
         storeW(P, b, c, pv);
         cst_at(P, b, c) = 1;
-    } else {
+    } else if (r == 0) {
         cst_at(P, b, c) = 0;
-        const double* fl = P.gfl + (size_t)b * 2 * NCELL;
-        Prim w = loadW(P, b, c);
-        w.rho = fl[c];
-        w.eps = fl[NCELL + c];
-        Wa(P, b, WRHO, c) = w.rho;
-        Wa(P, b, WEPS, c) = w.eps;
-        storeU(P, b, c, ptoc(m, P.gamma, w, nf));
+        ghost_floor(P, b, c, m, nf);
     }
     add_faults(P, nf);
 }
@@ -225,8 +271,79 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     if (c >= NCELL) return;
     if (cst_at(P, b, c)) return;
     int nf = 0;
-    cst_at(P, b, c) = c2p_cell(P, b, c, nf) ? 1 : 0;
+    GPt m = cell_tree(P, b, c);
+    Prim pv;
+    int r = c2p_first(P, b, c, m, Q_C2P, pv, nf);
+    if (r > 0) {
+        storeW(P, b, c, pv);
+        cst_at(P, b, c) = 1;
+    } else if (r == 0) {
+        c2p_floor(P, b, c, m, nf);
+    }
     add_faults(P, nf);
+}
+
+// floors semantics after con2prim of an active zone (grid.cpp:342-354)
+__device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const GPt& m, bool bhl, const Prim& pv,
+                                              bool have_pv, int& nf) {
+    unsigned char st = 0;
+    if (!bhl && have_pv) {
+        bool hl = !(pv.rho <= 0.0 || pv.eps <= 0.0);
+        if (hl) { storeW(P, b, c, pv); st = 1; }
+    }
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    if (bhl || Wa(P, b, WRHO, c) <= fl[c]) {
+        Prim f = floor_prim(fl[c], fl[NCELL + c]);
+        f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
+        storeW(P, b, c, f);
+        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+        st = 0;
+    }
+    cst_at(P, b, c) = st;
+}
+
+// Brent fallback for the queued cells, then the rest of each caller's
+// con2prim sequence.
+__global__ __launch_bounds__(TPB) void k_brent(KP P) {
+    const unsigned int n = *P.qn;
+    for (unsigned int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+        const int b = P.qb[i];
+        const int c = P.qc[i] & 0xffff;
+        const int kind = P.qc[i] >> 16;
+        int nf = 0;
+        GPt m = cell_tree(P, b, c);
+        C2P s;
+        ctop_setup(m, loadU(P, b, c), s);
+        double xi;
+        Prim pv;
+        bool ok = ctop_brent(P.gamma, s, P.qx[i], xi) && ctop_finish(P.gamma, s, xi, pv, nf);
+        if (kind == Q_FLOOR) {
+            floors_finish(P, b, c, m, !ok, pv, ok, nf);
+        } else if (ok) {
+            storeW(P, b, c, pv);
+            cst_at(P, b, c) = 1;
+        } else if (kind == Q_C2P) {
+            c2p_floor(P, b, c, m, nf);
+            cst_at(P, b, c) = 0;
+        } else if (kind == Q_GHOST) {
+            ghost_floor(P, b, c, m, nf);
+            cst_at(P, b, c) = 0;
+        } else {
+            // prolonged ghost: c2p floor, then the ghost con2prim of the new U
+            c2p_floor(P, b, c, m, nf);
+            if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
+                storeW(P, b, c, pv);
+                cst_at(P, b, c) = 1;
+            } else {
+                ghost_floor(P, b, c, m, nf);
+                cst_at(P, b, c) = 0;
+            }
+        }
+        add_faults(P, nf);
+    }
+}
+__global__ void k_qreset(KP P) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *P.qn = 0u;
 }
 
 // ln(rho) and ln(eps) of every stored cell, for the log-space reconstruction
@@ -533,24 +650,15 @@ __global__ __launch_bounds__(TPB) void k_floors(KP P) {
     GPt m = cell_tree(P, b, c);
     const double D = Ua(P, b, UD, c), tau = Ua(P, b, UTAU, c);
     bool bhl = !isfinite(D) || !isfinite(tau) || D <= 0.0 || tau <= 0.0;
-    unsigned char st = 0;
+    Prim pv;
+    bool have = false;
     if (!bhl) {
-        Prim pv;
-        bhl = !ctop(m, P.gamma, loadU(P, b, c), pv, nf);
-        if (!bhl) {
-            bool hl = !(pv.rho <= 0.0 || pv.eps <= 0.0);
-            if (hl) { storeW(P, b, c, pv); st = 1; }
-        }
+        int r = c2p_first(P, b, c, m, Q_FLOOR, pv, nf);
+        if (r < 0) { add_faults(P, nf); return; }
+        have = r > 0;
+        bhl = r == 0;
     }
-    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
-    if (bhl || Wa(P, b, WRHO, c) <= fl[c]) {
-        Prim f = floor_prim(fl[c], fl[NCELL + c]);
-        f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
-        storeW(P, b, c, f);
-        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
-        st = 0;
-    }
-    cst_at(P, b, c) = st;
+    floors_finish(P, b, c, m, bhl, pv, have, nf);
     add_faults(P, nf);
 }
 
@@ -669,6 +777,9 @@ public:
     DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
     DBuf<unsigned long long> dtpart;  // per-workgroup dt candidates
     DBuf<double> lg;                // [nb][2][8000] log(rho), log(eps)
+    DBuf<unsigned int> qn;          // Brent queue
+    DBuf<int> qb, qc;
+    DBuf<double> qx;
 
     // Per-kernel timing with HIP events on this tree's stream.
     bool profile = false;
@@ -728,6 +839,7 @@ public:
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
         P.lg = lg.p;
+        P.qn = qn.p; P.qb = qb.p; P.qc = qc.p; P.qx = qx.p;
         return P;
     }
 
@@ -775,6 +887,10 @@ public:
         cst.alloc((size_t)nb * NCELL);
         dtpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT));
         lg.alloc((size_t)nb * 2 * NCELL);
+        {
+            size_t qcap = std::max<size_t>({(size_t)1, plan.leaves.size() * NCELL, (size_t)nb * NINT});
+            qn.alloc(1); qb.alloc(qcap); qc.alloc(qcap); qx.alloc(qcap);
+        }
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -816,10 +932,17 @@ public:
         KP P = kp();
         for (size_t w = 0; w + 1 < wave_off.size(); w++) {
             int n = wave_off[w + 1] - wave_off[w];
-            if (n) launch("k_ghost", k_ghost, dim3(n * WG_GHOST), P, (const int*)(waveblk.p + wave_off[w]));
+            if (!n) continue;
+            launch("k_qreset", k_qreset, dim3(1), P);
+            launch("k_ghost", k_ghost, dim3(n * WG_GHOST), P, (const int*)(waveblk.p + wave_off[w]));
+            launch("k_brent", k_brent, dim3(BRENT_WG), P);
         }
         int nl = (int)plan.leaves.size();
-        if (nl) launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
+        if (nl) {
+            launch("k_qreset", k_qreset, dim3(1), P);
+            launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
+            launch("k_brent", k_brent, dim3(BRENT_WG), P);
+        }
         launch("k_logs", k_logs, dim3(nb * WG_CELL), P);
         launch("k_flux", k_flux, dim3(nb * WG_FACE), P);
         launch("k_emf", k_emf, dim3(nb * WG_EDGE), P);
@@ -829,7 +952,9 @@ public:
     }
     void launch_floors() {
         KP P = kp();
+        launch("k_qreset", k_qreset, dim3(1), P);
         launch("k_floors", k_floors, dim3(nb * WG_INT), P);
+        launch("k_brent", k_brent, dim3(BRENT_WG), P);
     }
     void launch_step(double cfl) {
         launch_dtcomp(cfl);

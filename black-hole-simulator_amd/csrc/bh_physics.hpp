@@ -185,58 +185,64 @@ BH_HD bool c2p_brent(double gm, double xa, double xb, double Ssq, double Bsq, do
     return false;
 }
 
-// Conserved-to-primitive inversion, conserved.cpp:135-230.
+// Conserved-to-primitive inversion, conserved.cpp:135-230, in three parts
+// so that kernels can run the Newton stage for every cell and the rare,
+// long Brent fallback over a compacted list of cells.
 // quirk Q6 with the uninitialised read at conserved.cpp:148-149: only the
 // j=2 term of gamma^{ij}S_j survives, and for i=0,1 S[2]/B[2] are read before
 // assignment; the deterministic build of the reference reads 0 there.
 // quirks Q9 (bracket expansion updates f_lo), Q10, Q11, Q12 (500 Newton
 // iterations without convergence count as success).
-BH_HD bool ctop(const GPt& m, double gm, const Cons& cv, Prim& pv, int& nf) {
-    const double gam[3][3] = {{m.g11, 0.0, m.g13}, {0.0, m.g22, 0.0}, {m.g13, 0.0, m.g33}};
+struct C2P {
+    double D, tau, Ssq, Bsq, SB, B[3], Bu[3], Su[3];
+};
+BH_HD void ctop_setup(const GPt& m, const Cons& cv, C2P& s) {
     const double gami2[3] = {m.gi13, 0.0, m.gi33};
+    double sg = m.sqrtdetgam;
+    s.D = cv.D / sg;
+    s.tau = cv.tau / sg;
+    double S[3];
+    s.Bsq = 0.0; s.Ssq = 0.0; s.SB = 0.0;
+    for (int i = 0; i < 3; i++) {
+        s.B[i] = cv.B[i] / sg;
+        S[i] = cv.S[i] / sg;
+        double B2 = (i == 2) ? s.B[2] : 0.0;
+        double S2 = (i == 2) ? S[2] : 0.0;
+        s.Bu[i] = gami2[i] * B2;
+        s.Su[i] = gami2[i] * S2;
+        s.Ssq += S[i] * s.Su[i];
+        s.Bsq += s.B[i] * s.Bu[i];
+        s.SB += S[i] * s.Bu[i];
+    }
+}
+// Bracket [xi_l, xi_h] and Newton iteration.  Returns false when the
+// reference falls back to Brent (then xi_h is the bracket to use).
+BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h) {
     const int maxiter = 500;
     const double tol = 1e-10;
-    double sg = m.sqrtdetgam;
-    double D = cv.D / sg;
-    double tau = cv.tau / sg;
-    double B[3], S[3];
-    double Bsq = 0.0, Ssq = 0.0, SB = 0.0;
-    double Bu[3], Su[3];
-    for (int i = 0; i < 3; i++) {
-        B[i] = cv.B[i] / sg;
-        S[i] = cv.S[i] / sg;
-        double B2 = (i == 2) ? B[2] : 0.0;
-        double S2 = (i == 2) ? S[2] : 0.0;
-        Bu[i] = gami2[i] * B2;
-        Su[i] = gami2[i] * S2;
-        Ssq += S[i] * Su[i];
-        Bsq += B[i] * Bu[i];
-        SB += S[i] * Bu[i];
-    }
+    const double D = s.D, tau = s.tau, Ssq = s.Ssq, Bsq = s.Bsq, SB = s.SB;
     double xi_l = D;
-    double xi_h = tau + D + Bsq / 2.0 + 1e3;
+    xi_h = tau + D + Bsq / 2.0 + 1e3;
     double fl = c2p_f(gm, xi_l, Ssq, Bsq, SB, D, tau);
     double fh = c2p_f(gm, xi_h, Ssq, Bsq, SB, D, tau);
     while (fl * fh > 0.0 && xi_h < 1e8 * D) {
         xi_h *= 2.0;
         fl = c2p_f(gm, xi_h, Ssq, Bsq, SB, D, tau);
     }
-    double xi = (xi_l + xi_h) / 2.0;
-    bool nr_ok = true;
-    // Newton iteration.  The iterate is a pure function of the previous one,
-    // so once it is NaN, or repeats with period 1 or 2, every remaining pass
-    // is determined: the loop stops early with the value the full 500 passes
-    // would end on (NaN stays NaN; a 2-cycle ends on the parity of the
-    // remaining count).
+    xi = (xi_l + xi_h) / 2.0;
+    // The iterate is a pure function of the previous one, so once it is NaN,
+    // or repeats with period 1 or 2, every remaining pass is determined: the
+    // loop stops early with the value the full 500 passes would end on (NaN
+    // stays NaN; a 2-cycle ends on the parity of the remaining count).
     double xprev = xi;
     if (xi == xi) {
         for (int it = 0; it < maxiter; it++) {
             double dF;
             double F = c2p_fdf(gm, xi, Ssq, Bsq, SB, D, tau, dF);
             if (fabs(F) < tol) break;
-            if (fabs(dF) < 1e-30) { nr_ok = false; break; }
+            if (fabs(dF) < 1e-30) return false;
             double xin = xi - F / dF;
-            if (xin < xi_l || xin > xi_h) { nr_ok = false; break; }
+            if (xin < xi_l || xin > xi_h) return false;
             if (xin != xin || xin == xi) { xi = xin; break; }
             if (it > 0 && xin == xprev) {
                 if (((maxiter - 1 - it) & 1) == 0) xi = xin;  // else the iterate stays at xi
@@ -246,30 +252,41 @@ BH_HD bool ctop(const GPt& m, double gm, const Cons& cv, Prim& pv, int& nf) {
             xi = xin;
         }
     }
-    if (!nr_ok) {
-        double xib;
-        if (!c2p_brent(gm, xi_l, xi_h, Ssq, Bsq, SB, D, tau, xib, tol)) return false;
-        xi = xib;
-    }
+    return true;
+}
+BH_HD bool ctop_brent(double gm, const C2P& s, double xi_h, double& xi) {
+    return c2p_brent(gm, s.D, xi_h, s.Ssq, s.Bsq, s.SB, s.D, s.tau, xi, 1e-10);
+}
+// Primitive recovery from the root xi (conserved.cpp:196-229)
+BH_HD bool ctop_finish(double gm, const C2P& s, double xi, Prim& pv, int& nf) {
+    const double Bsq = s.Bsq, SB = s.SB, Ssq = s.Ssq;
     double xpQ = xi + Bsq;
     double sdb2 = SB * SB;
     double vsq = (Ssq * xpQ * xpQ + sdb2 * (2.0 * xi + Bsq)) / (xpQ * xpQ * xi * xi);
     vsq = smin(vsq, 1.0 - 1e-10);
     double ltz2 = 1.0 / (1.0 - vsq);
     double ltz = sqrt(ltz2);
-    pv.rho = D / ltz;
+    pv.rho = s.D / ltz;
     double h = xi / (pv.rho * ltz2);
     pv.eps = (h - 1.0) / (gm);
     double p = eos_press(gm, pv.rho, pv.eps, nf);
     eos_enth(gm, pv.rho, pv.eps, nf);
     eos_temp(pv.rho, p, nf);
     for (int i = 0; i < 3; i++) {
-        pv.v[i] = (Su[i] + (SB / xi) * Bu[i]) / xpQ;
-        pv.B[i] = B[i];
+        pv.v[i] = (s.Su[i] + (SB / xi) * s.Bu[i]) / xpQ;
+        pv.B[i] = s.B[i];
     }
-    (void)gam;
     if (pv.rho <= 0.0 || pv.eps <= 0.0 || p <= 0.0) return false;
     return true;
+}
+BH_HD bool ctop(const GPt& m, double gm, const Cons& cv, Prim& pv, int& nf) {
+    C2P s;
+    ctop_setup(m, cv, s);
+    double xi, xi_h;
+    if (!ctop_newton(gm, s, xi, xi_h)) {
+        if (!ctop_brent(gm, s, xi_h, xi)) return false;
+    }
+    return ctop_finish(gm, s, xi, pv, nf);
 }
 
 // ------------------------------------------------------- reconstruction

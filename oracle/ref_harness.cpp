@@ -21,7 +21,9 @@
 //   Bf[b][3][8400]  (Bfx, Bfy, Bfz in the reference's face indexing)
 // with the cell index (i+2)*400+(j+2)*20+(k+2), identical to patch::lindx.
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <new>
 #include <stdexcept>
 #include <vector>
 #include <array>
@@ -34,6 +36,23 @@
 #include "initial.hpp"
 #include "recon.hpp"
 #include "hlld.hpp"
+
+// Zero-filled heap for the reference's objects.  grid::patch leaves its
+// refine/erase flags uninitialised (src/fluid/grid.hpp:98-99, constructor at
+// src/fluid/grid/grid.cpp:19) and amrtree::regrid reads them
+// (grid.cpp:803-815), so the compiled reference refines depending on stale
+// heap bytes.  Linked with -Bsymbolic, these replace operator new/delete for
+// the reference code in this library only, making those flags start false.
+void* operator new(std::size_t n) {
+    void* p = std::calloc(1, n ? n : 1);
+    if (!p) throw std::bad_alloc();
+    return p;
+}
+void* operator new[](std::size_t n) { return operator new(n); }
+void operator delete(void* p) noexcept { std::free(p); }
+void operator delete[](void* p) noexcept { std::free(p); }
+void operator delete(void* p, std::size_t) noexcept { std::free(p); }
+void operator delete[](void* p, std::size_t) noexcept { std::free(p); }
 
 namespace {
 constexpr int NCELL = cpn * cpn * cpn;
