@@ -118,6 +118,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nqlt", type=int, default=NQLT)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="before the timed region, run the traffic-counter calibration copy of MIB MiB")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -125,6 +127,12 @@ def main():
         sys.exit(2)
 
     import bhfluid
+
+    copy = None
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        ms = bhfluid.copy_probe(nbytes)
+        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBps_rw": 2 * nbytes / ms / 1e6}
 
     t_setup = time.time()
     tree = build_tree(nqlt=args.nqlt)
@@ -225,6 +233,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if copy is not None:
+        line["copy_probe"] = copy
     if dts is not None:
         line["config"]["dt_first_last"] = [float(dts[0]), float(dts[-1])]
     print(json.dumps(line), flush=True)

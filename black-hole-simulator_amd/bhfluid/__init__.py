@@ -112,6 +112,7 @@ def lib():
         _sig(L, "bh_hlld_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
+        _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
         _lib = L
     return _lib
 
@@ -357,6 +358,14 @@ def run_steps(tree: AmrTree, nsteps: int, cfl: float = 0.4) -> np.ndarray:
 def device_sync():
     if lib().bh_device_sync() != 0:
         raise BhError("hipDeviceSynchronize failed")
+
+
+def copy_probe(nbytes: int, reps: int = 5) -> float:
+    """Milliseconds per plain device copy of nbytes (HBM read + write)."""
+    ms = ctypes.c_double(0.0)
+    if lib().bh_copy_probe(int(nbytes), int(reps), ctypes.byref(ms)) != 0:
+        raise BhError(lib().bh_last_error(None).decode())
+    return ms.value
 
 
 def device_info() -> str:

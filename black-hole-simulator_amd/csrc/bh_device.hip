@@ -54,7 +54,7 @@ struct KP {
     double* W;           // [nb][8][8000]
     double* Bf;          // [nb][3][8400]
     double* Us;          // [nb][5][4096] RK store of D, S, tau
-    double* F;           // [nb][3][8][8400] fluxes, zero outside computed faces
+    double* F;           // [nb][3][8][4352] fluxes on the computed faces
     double* E;           // [nb][3][8820] EMFs
     const double* gct;   // [nb][8][8000]
     const double* gcc;   // [nb][11][8000]
@@ -80,6 +80,25 @@ struct KP {
     double* qx;           // Newton bracket upper end
 };
 enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3 };
+
+constexpr int chr_slot(int m, int n, int l) {
+    int lo = n < l ? n : l, hi = n < l ? l : n;
+    for (int z = 0; z < NCHR; z++)
+        if (kChrIdx[z][0] == m && kChrIdx[z][1] == lo && kChrIdx[z][2] == hi) return z;
+    return -1;
+}
+
+// Fluxes are stored compactly on the computed faces only, [b][dir][comp][4352]
+// (x-faces i in [-1,15], j,k in [0,15]; y and z likewise).  The reference
+// keeps them in padded arrays whose other entries stay zero and are read as
+// zero by the EMF average; fB() returns that zero.
+__device__ __forceinline__ int face_lf(int dir, int i, int j, int k) {
+    return dir == 0 ? ((i + 1) * 16 + j) * 16 + k : (dir == 1 ? (i * 17 + (j + 1)) * 16 + k : (i * 16 + j) * 17 + (k + 1));
+}
+__device__ __forceinline__ bool face_in(int dir, int i, int j, int k) {
+    int lo0 = dir == 0 ? -1 : 0, lo1 = dir == 1 ? -1 : 0, lo2 = dir == 2 ? -1 : 0;
+    return i >= lo0 && i < BLK && j >= lo1 && j < BLK && k >= lo2 && k < BLK;
+}
 
 __device__ __forceinline__ double& Ua(const KP& P, int b, int v, int c) { return P.U[((size_t)b * 8 + v) * NCELL + c]; }
 __device__ __forceinline__ double& Wa(const KP& P, int b, int v, int c) { return P.W[((size_t)b * 8 + v) * NCELL + c]; }
@@ -416,10 +435,9 @@ __global__ __launch_bounds__(TPB) void k_flux(KP P) {
     Cons UR = ptoc(m, P.gamma, R, nf);
     double ct;
     Cons Fl = hlld(L, R, UL, UR, m, P.gamma, 0, ct, nf);
-    const int fidx = dir == 0 ? ixf(i, j, k) : (dir == 1 ? iyf(i, j, k) : izf(i, j, k));
-    double* Fo = P.F + ((size_t)b * 3 + dir) * 8 * NFACE + fidx;
-    Fo[0] = Fl.D; Fo[NFACE] = Fl.S[0]; Fo[2 * NFACE] = Fl.S[1]; Fo[3 * NFACE] = Fl.S[2];
-    Fo[4 * NFACE] = Fl.tau; Fo[5 * NFACE] = Fl.B[0]; Fo[6 * NFACE] = Fl.B[1]; Fo[7 * NFACE] = Fl.B[2];
+    double* Fo = P.F + ((size_t)b * 3 + dir) * 8 * NFC + lf;
+    Fo[0] = Fl.D; Fo[NFC] = Fl.S[0]; Fo[2 * NFC] = Fl.S[1]; Fo[3 * NFC] = Fl.S[2];
+    Fo[4 * NFC] = Fl.tau; Fo[5 * NFC] = Fl.B[0]; Fo[6 * NFC] = Fl.B[1]; Fo[7 * NFC] = Fl.B[2];
     add_faults(P, nf);
 }
 
@@ -431,16 +449,14 @@ __global__ __launch_bounds__(TPB) void k_emf(KP P) {
     const int t = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
     if (t >= 19 * 19 * 19) return;
     const int i = t / 361 - GH, j = (t / 19) % 19 - GH, k = t % 19 - GH;
-    const double* Fx = P.F + ((size_t)b * 3 + 0) * 8 * NFACE;
-    const double* Fy = P.F + ((size_t)b * 3 + 1) * 8 * NFACE;
-    const double* Fz = P.F + ((size_t)b * 3 + 2) * 8 * NFACE;
+    const double* F = P.F + (size_t)b * 3 * 8 * NFC;
+    auto fB = [&](int dir, int comp, int ii, int jj, int kk) -> double {
+        return face_in(dir, ii, jj, kk) ? F[(dir * 8 + 5 + comp) * NFC + face_lf(dir, ii, jj, kk)] : 0.0;
+    };
     double* E = P.E + (size_t)b * 3 * NEDGE;
-    E[iex(i, j, k)] = (Fy[7 * NFACE + iyf(i, j, k)] + Fy[7 * NFACE + iyf(i, j, k + 1)] - Fz[6 * NFACE + izf(i, j, k)] -
-                       Fz[6 * NFACE + izf(i, j + 1, k)]) / 4;
-    E[NEDGE + izf(i, j, k)] = (Fz[5 * NFACE + izf(i, j, k)] + Fz[5 * NFACE + izf(i + 1, j, k)] -
-                               Fx[7 * NFACE + ixf(i, j, k)] - Fx[7 * NFACE + ixf(i, j, k + 1)]) / 4;
-    E[2 * NEDGE + iyf(i, j, k)] = (Fx[6 * NFACE + ixf(i, j, k)] + Fx[6 * NFACE + ixf(i, j + 1, k)] -
-                                   Fy[5 * NFACE + iyf(i, j, k)] - Fy[5 * NFACE + iyf(i + 1, j, k)]) / 4;
+    E[iex(i, j, k)] = (fB(1, 2, i, j, k) + fB(1, 2, i, j, k + 1) - fB(2, 1, i, j, k) - fB(2, 1, i, j + 1, k)) / 4;
+    E[NEDGE + izf(i, j, k)] = (fB(2, 0, i, j, k) + fB(2, 0, i + 1, j, k) - fB(0, 2, i, j, k) - fB(0, 2, i, j, k + 1)) / 4;
+    E[2 * NEDGE + iyf(i, j, k)] = (fB(0, 1, i, j, k) + fB(0, 1, i, j + 1, k) - fB(1, 0, i, j, k) - fB(1, 0, i + 1, j, k)) / 4;
 }
 
 // constrans::Bfupdate, transport.cpp:38-58 (not RK-averaged, quirk Q21)
@@ -526,11 +542,22 @@ __device__ __forceinline__ void source_terms(const KP& P, int b, int c, int a, c
     double T[4][4];
     for (int m = 0; m < 4; m++)
         for (int n = 0; n < 4; n++) T[m][n] = rhohb2 * u4[m] * u4[n] + px * gi[m][n] - b4[m] * b4[n];
-    const double* G = P.gchr + (size_t)b * NCHR * NINT + a;
+    double G[NCHR];
+    const double* gp = P.gchr + (size_t)b * NCHR * NINT + a;
+#pragma unroll
+    for (int z = 0; z < NCHR; z++) G[z] = gp[(size_t)z * NINT];
+#pragma unroll
     for (int m = 0; m < 4; m++) {
         double s = 0.0;
+#pragma unroll
         for (int n = 0; n < 4; n++)
-            for (int l = 0; l < 4; l++) s += T[n][l] * G[(size_t)(m * 10 + sym4(n, l)) * NINT];
+#pragma unroll
+            for (int l = 0; l < 4; l++) {
+                // the 26 structurally zero entries enter as +0.0 (the product
+                // still propagates a non-finite T like the reference's)
+                const int z = chr_slot(m, n, l);
+                s += T[n][l] * (z >= 0 ? G[z] : 0.0);
+            }
         src[m] = s * sgc;
     }
 }
@@ -546,19 +573,19 @@ __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
     const double dt = __longlong_as_double((long long)*P.dtbits);
     const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
     const double sg = cell_sqrtdetg(P, b, c);
-    const double* Fx = P.F + ((size_t)b * 3 + 0) * 8 * NFACE;
-    const double* Fy = P.F + ((size_t)b * 3 + 1) * 8 * NFACE;
-    const double* Fz = P.F + ((size_t)b * 3 + 2) * 8 * NFACE;
-    const int xR = ixf(i, j, k), xL = ixf(i - 1, j, k);
-    const int yR = iyf(i, j, k), yL = iyf(i, j - 1, k);
-    const int zR = izf(i, j, k), zL = izf(i, j, k - 1);
+    const double* Fx = P.F + ((size_t)b * 3 + 0) * 8 * NFC;
+    const double* Fy = P.F + ((size_t)b * 3 + 1) * 8 * NFC;
+    const double* Fz = P.F + ((size_t)b * 3 + 2) * 8 * NFC;
+    const int xR = face_lf(0, i, j, k), xL = face_lf(0, i - 1, j, k);
+    const int yR = face_lf(1, i, j, k), yL = face_lf(1, i, j - 1, k);
+    const int zR = face_lf(2, i, j, k), zL = face_lf(2, i, j, k - 1);
     int nf = 0;
     Prim W = loadW(P, b, c);
     double src[4];
     source_terms(P, b, c, a, W, src, nf);
     // U component v in {D, S0, S1, S2, tau}; source slot: tau <- src0, S_d <- src_{d+1}
     for (int v = 0; v < 5; v++) {
-        const size_t o = (size_t)v * NFACE;
+        const size_t o = (size_t)v * NFC;
         double div = -((Fx[o + xR] - Fx[o + xL]) / dx + (Fy[o + yR] - Fy[o + yL]) / dy +
                        (Fz[o + zR] - Fz[o + zL]) / dz) / sg;
         double s = v == UD ? 0.0 : (v == UTAU ? src[0] : src[v]);
@@ -731,6 +758,11 @@ __global__ void k_hlld_batch(int n, double gm, int dim, const double* g, const d
     ct[t] = c;
     if (nf) atomicAdd(faults, (unsigned long long)nf);
 }
+// Plain device copy of a known size: the calibration for the HBM traffic
+// counters (FETCH_SIZE / WRITE_SIZE per dispatch).
+__global__ __launch_bounds__(TPB) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) b[i] = a[i];
+}
 __global__ void k_math_batch(int op, int n, const double* x, double* y) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -854,7 +886,7 @@ public:
         W.alloc((size_t)nb * 8 * NCELL);
         Bf.alloc((size_t)nb * 3 * NFACE);
         Us.alloc((size_t)nb * 5 * NINT);
-        F.alloc((size_t)nb * 3 * 8 * NFACE);
+        F.alloc((size_t)nb * 3 * 8 * NFC);
         E.alloc((size_t)nb * 3 * NEDGE);
         HIPCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
         HIPCHK(hipMemsetAsync(E.p, 0, E.n * sizeof(double), stream));
@@ -1385,6 +1417,34 @@ int bh_math_batch(int op, int n, const double* x, double* y, int on_device) {
         if (n) hipLaunchKernelGGL(k_math_batch, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        return BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_copy_probe(long long nbytes, int reps, double* ms_per_copy) {
+    if (nbytes < 32 || reps < 1) return BH_EINVAL;
+    try {
+        size_t n = (size_t)nbytes / sizeof(double4);
+        double4 *a = nullptr, *b = nullptr;
+        HIPCHK(hipMalloc(&a, n * sizeof(double4)));
+        HIPCHK(hipMalloc(&b, n * sizeof(double4)));
+        HIPCHK(hipMemset(a, 0, n * sizeof(double4)));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, 0, a, b, n);
+        HIPCHK(hipEventRecord(e0, 0));
+        for (int r = 0; r < reps; r++) hipLaunchKernelGGL(k_copy, dim3(4096), dim3(TPB), 0, 0, a, b, n);
+        HIPCHK(hipEventRecord(e1, 0));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_per_copy) *ms_per_copy = ms / reps;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipFree(a);
+        (void)hipFree(b);
         return BH_OK;
     } catch (const DevError& e) {
         return fail(nullptr, BH_EDEVICE, e.msg);

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <limits>
 #include <sstream>
+#include <stdexcept>
 
 namespace bh {
 
@@ -688,7 +689,8 @@ void build_geometry(const HostTree& t, GeomHost& g) {
     g.face.assign((size_t)nb * 3 * NGFC * NFC, 0.0);
     g.chr.assign((size_t)nb * NCHR * NINT, 0.0);
     g.dxyz.assign((size_t)nb * 3, 0.0);
-#pragma omp parallel for schedule(dynamic)
+    int bad_chr = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad_chr)
     for (int b = 0; b < nb; b++) {
         const Patch& p = *t.quilt[b];
         g.dxyz[3 * b] = p.dx(); g.dxyz[3 * b + 1] = p.dy(); g.dxyz[3 * b + 2] = p.dz();
@@ -738,10 +740,18 @@ void build_geometry(const HostTree& t, GeomHost& g) {
                             for (int l = n; l < 4; l++) {
                                 double s = 0.0;
                                 for (int q = 0; q < 4; q++) s += m.g[mm][q] * m.G[q][n][l];
-                                ch[(size_t)(mm * 10 + sym4(n, l)) * NINT + a] = s;
+                                int slot = -1;
+                                for (int z = 0; z < NCHR; z++)
+                                    if (kChrIdx[z][0] == mm && kChrIdx[z][1] == n && kChrIdx[z][2] == l) slot = z;
+                                if (slot >= 0) {
+                                    ch[(size_t)slot * NINT + a] = s;
+                                } else if (s != 0.0 || std::signbit(s)) {
+                                    bad_chr = 1;
+                                }
                             }
                 }
     }
+    if (bad_chr) throw std::runtime_error("a lowered Christoffel of the cell metric expected to be +0.0 is not");
 }
 
 void build_ghost_plan(const HostTree& t, GhostPlan& g) {

@@ -94,13 +94,19 @@ private:
 constexpr int NGCT = 8;   // tree metric at cells: sqrtdetgam, alpha, g11, g13, g22, g33, gi13, gi33
 constexpr int NGCC = 11;  // cell metric at cells: GCell fields
 constexpr int NGFC = 8;   // tree metric at faces: sqrtdetg, sqrtdetgam, alpha, beta0, g11, g13, g22, g33
-constexpr int NCHR = 40;  // lowered Christoffels Gamma_{m(nl)} of the cell metric, n<=l
+// Lowered Christoffels Gamma_{m n l} = g_{m s} Gamma^s_{n l} of the cell metric
+// (a = 0): only these 14 (m, n<=l) entries can be non-zero; the other 26 are
+// exactly +0.0 (sums of zero products started from +0.0), which
+// build_geometry verifies cell by cell.
+constexpr int NCHR = 14;
+constexpr int kChrIdx[NCHR][3] = {{0, 0, 0}, {0, 0, 1}, {0, 1, 1}, {0, 2, 2}, {0, 3, 3}, {1, 0, 0}, {1, 0, 1},
+                                  {1, 1, 1}, {1, 2, 2}, {1, 3, 3}, {2, 1, 2}, {2, 3, 3}, {3, 1, 3}, {3, 2, 3}};
 struct GeomHost {
     std::vector<double> cell_tree;  // [nb][NGCT][8000]
     std::vector<double> cell_cell;  // [nb][NGCC][8000]
     std::vector<double> floors;     // [nb][2][8000]  rho_floor, eps_floor
     std::vector<double> face;       // [nb][3][NGFC][4352]
-    std::vector<double> chr;        // [nb][NCHR][4096]
+    std::vector<double> chr;        // [nb][NCHR][4096] the non-zero lowered Christoffels
     std::vector<double> dxyz;       // [nb][3]
 };
 void build_geometry(const HostTree& t, GeomHost& g);

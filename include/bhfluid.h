@@ -105,6 +105,10 @@ int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* 
 /* op 0: log, 1: exp, correctly rounded; on_device 0 evaluates the same code on the host. */
 int bh_math_batch(int op, int n, const double* x, double* y, int on_device);
 
+/* Measurement helper without a reference counterpart: times a plain device
+ * copy of nbytes (read nbytes, write nbytes), the calibration kernel for the
+ * HBM traffic counters. */
+int bh_copy_probe(long long nbytes, int reps, double* ms_per_copy);
 /* Device and build information. */
 int bh_device_info(char* buf, int len);
 
