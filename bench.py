@@ -44,6 +44,7 @@ A_SPIN, Q_CHARGE, GAMMA, CFL = 0.9, 0.1, 5.0 / 3.0, 0.4
 #   k_flux   : W on the 7168 stencil cells (8 words) + 13056 normal face B
 #              in, 13056 faces x 8 flux words out
 #   k_c2p    : U in, W out on 8000 cells
+#   k_c2p_pre: U in, W out on 4096 cells
 #   k_ghost  : U, W of 3072 face-ghost sources in, U, W of 3904 ghost cells out
 #   k_update : U, W, RK store in (4096 x 21 words), U and store out (4096 x 13),
 #              fluxes 5 words on 3 x 4352 faces in
@@ -56,6 +57,7 @@ W8 = 8
 ALGO_BYTES_PER_BLOCK = {
     "k_flux": (7168 * 8 + 13056 + 13056 * 8) * W8,
     "k_c2p": (8000 * 16) * W8,
+    "k_c2p_pre": (4096 * 16) * W8,
     "k_ghost": (3072 * 16 + 3904 * 16) * W8,
     "k_update": (4096 * (21 + 13) + 3 * 4352 * 5) * W8,
     "k_emf": (3 * 4352 * 6 + 3 * 6859) * W8,
@@ -65,11 +67,31 @@ ALGO_BYTES_PER_BLOCK = {
     "k_dtcomp": (4096 * 5) * W8,
     "k_logs": (8000 * 4) * W8,
     "k_dtreduce": 0,
+    "k_brent": 0,
+    "k_qreset": 0,
 }
 # launches per step and the blocks each launch covers ("leaf" or "all")
 KERNEL_SCOPE = {"k_flux": "all", "k_emf": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
                 "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf", "k_logs": "all",
-                "k_dtreduce": "leaf"}
+                "k_dtreduce": "leaf", "k_c2p_pre": "leaf", "k_brent": "leaf", "k_qreset": "leaf"}
+# HBM bytes per dispatch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE,
+# each scaled by what the counter reads for a copy of known size)
+TRAFFIC_TABLE = ROOT / "profiles" / "traffic_main_torus.md"
+
+
+def measured_traffic(kernel):
+    """Scaled HBM bytes per dispatch of `kernel` from the committed counter
+    summary, or None when the summary has no row for it."""
+    if not TRAFFIC_TABLE.exists():
+        return None
+    for line in TRAFFIC_TABLE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 7 and cols[0] == kernel:
+            try:
+                return float(cols[6]) * 1e9
+            except ValueError:
+                return None
+    return None
 
 
 def build_tree(dom=DOM, nqlt=NQLT, regrid=True):
@@ -165,8 +187,10 @@ def main():
         dts = None
     bhfluid.device_sync()
     el = time.perf_counter() - t0
-    prof = tree.profile_report()
+    report = tree.profile_report()
     tree.profile(False)
+    prof = {k: v for k, v in report.items() if not k.startswith("queue:")}
+    queued = {k[6:]: v[0] for k, v in report.items() if k.startswith("queue:")}
     U, _, _ = tree.state()
     leaf = meta[:, 4] == 1
     Ui = U.reshape(nb, 8, 20, 20, 20)[leaf][:, :, 2:18, 2:18, 2:18]
@@ -182,7 +206,9 @@ def main():
     blocks = nb if KERNEL_SCOPE[dom_name] == "all" else nleaf
     if dom_name == "k_ghost":
         blocks = nleaf / max(1, n_launch / (2 * args.steps))  # one launch per wave
-    algo = ALGO_BYTES_PER_BLOCK[dom_name] * blocks
+    algo = ALGO_BYTES_PER_BLOCK.get(dom_name, 0) * blocks
+    if dom_name == "k_brent":  # U in, W out for each queued cell
+        algo = sum(queued.values()) / n_launch * 16 * W8
     achieved = algo / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "avg_ms": round(v[1] / v[0], 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
@@ -219,6 +245,7 @@ def main():
             "nonfinite_leaf_zones_after": nonfinite,
             "setup_s": round(setup_s, 1),
             "kernels": kernels,
+            "brent_queued_cells": queued,
         },
         "roofline": {
             "bound": "hbm",
@@ -227,7 +254,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(dom_name),
+            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per dispatch, copy-calibrated "
+                              "(profiles/traffic_main_torus.md)",
             "algo_bytes_per_launch": algo,
             "avg_launch_ms": avg_ms,
         },

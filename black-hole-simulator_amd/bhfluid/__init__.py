@@ -261,7 +261,9 @@ class AmrTree:
         self._check(self._L.bh_profile(self._h, 1 if enable else 0), "profile")
 
     def profile_report(self):
-        """{kernel: (launches, total_ms)} accumulated since profiling was enabled."""
+        """{kernel: (launches, total_ms)} accumulated since profiling was enabled,
+        plus ``queue:<pass>: (cells, 0.0)``, the cells each con2prim pass queued
+        for the Brent fallback over the same period."""
         buf = ctypes.create_string_buffer(1 << 16)
         self._check(self._L.bh_profile_report(self._h, buf, 1 << 16), "profile_report")
         out = {}

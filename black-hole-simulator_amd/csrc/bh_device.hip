@@ -63,12 +63,22 @@ struct KP {
     const double* gchr;  // [nb][40][4096]
     const double* dxyz;  // [nb][3]
     const int* ghtab;    // [nb][12]
-    const short* ghost_cells;  // [3904]
+    const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
+    // [3904] ghost cells in storage order (the fill copies whole rows and
+    // planes: neighbouring lanes touch neighbouring addresses)
+    const short* ghost_order;
+    // [8000] con2prim visiting order: the ghost cells grouped by face side,
+    // then edges and corners, then the 4096 active zones (lanes of a
+    // wavefront share the side, hence the fill kind and the typical cost)
+    const short* cell_order;
     unsigned long long* dtbits;  // dt as an ordered bit pattern
     unsigned long long* faults;
     // cst[b][c] = 1 when W holds the successful con2prim of the cell's current
     // U; a repeated con2prim of the same U is then skipped (it would return
-    // the same W).  Every kernel that writes U keeps it current.
+    // the same W).  Every kernel that writes U keeps it current.  Between the
+    // early con2prim pass and the con2prim phase, an active zone may also hold
+    // CST_FAIL (+1 when the failing call counted equation-of-state faults,
+    // always 3 then): con2prim of the current U fails; W and U are untouched.
     unsigned char* cst;
     double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
     // Cells whose con2prim needs the Brent fallback are queued by the first
@@ -78,8 +88,14 @@ struct KP {
     int* qb;              // block
     int* qc;              // cell | kind << 16
     double* qx;           // Newton bracket upper end
+    unsigned long long* qstat;  // queued cells per pass (diagnostic)
 };
-enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3 };
+enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
+constexpr unsigned char CST_FAIL = 2;
+// status of a failed con2prim and the faults it counted (0 or 3: the three
+// equation-of-state calls of the primitive recovery all see the same rho)
+__device__ __forceinline__ unsigned char fail_status(int nf) { return CST_FAIL + (nf > 0 ? 1 : 0); }
+__device__ __forceinline__ int fail_faults(unsigned char st) { return st > CST_FAIL ? 3 : 0; }
 
 constexpr int chr_slot(int m, int n, int l) {
     int lo = n < l ? n : l, hi = n < l ? l : n;
@@ -165,8 +181,18 @@ __device__ __forceinline__ bool c2p_cell(const KP& P, int b, int c, int& nf) {
 }
 __device__ __forceinline__ unsigned char& cst_at(const KP& P, int b, int c) { return P.cst[(size_t)b * NCELL + c]; }
 
+// Queue slots are claimed once per wavefront: the lowest pushing lane adds
+// the number of pushing lanes to the queue length and each lane takes the
+// slot given by its rank among them (one atomic per wavefront instead of
+// one per cell on a single contended counter).
 __device__ __forceinline__ void push_brent(const KP& P, int b, int c, int kind, double xi_h) {
-    unsigned int i = atomicAdd(P.qn, 1u);
+    const unsigned long long mask = __ballot(1);
+    const int lane = (int)__lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    unsigned int base = 0;
+    if (lane == leader) base = atomicAdd(P.qn, (unsigned int)__popcll(mask));
+    base = __shfl(base, leader);
+    const unsigned int i = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
     P.qb[i] = b;
     P.qc[i] = c | (kind << 16);
     P.qx[i] = xi_h;
@@ -214,7 +240,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
     const int b = blocks[blockIdx.x / WG_GHOST];
     const int t = (blockIdx.x % WG_GHOST) * TPB + threadIdx.x;
     if (t >= NGHOST) return;
-    const int c = P.ghost_cells[t];
+    const int c = P.ghost_order[t];
     const int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
     int nf = 0;
     const int oi = (i < 0 || i >= BLK), oj = (j < 0 || j >= BLK), ok = (k < 0 || k >= BLK);
@@ -234,6 +260,23 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             for (int v = 0; v < 8; v++) {
                 Ua(P, b, v, c) = Ua(P, src_b, v, src_c);
                 Wa(P, b, v, c) = Wa(P, src_b, v, src_c);
+            }
+            // The source holds this same U at the same metric bits, so its
+            // known con2prim outcome is the ghost's: success returns the
+            // source's W (copied above), failure takes the ghost floor path.
+            if (kind == GH_COPY && P.ghsame[b * 6 + 2 * dim + s]) {
+                const unsigned char st = cst_at(P, src_b, src_c);
+                if (st == 1) {
+                    cst_at(P, b, c) = 1;
+                    return;
+                }
+                if (st >= CST_FAIL) {
+                    nf = fail_faults(st);
+                    cst_at(P, b, c) = 0;
+                    ghost_floor(P, b, c, cell_tree(P, b, c), nf);
+                    add_faults(P, nf);
+                    return;
+                }
             }
         } else {
             // gh_prolong, grid.cpp:427-492 (quirk Q25: coarse index ignores octant and side)
@@ -268,6 +311,10 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             }
             c2p_floor(P, b, c, mp, nf);
         }
+    } else if (cst_at(P, b, c) == 1) {
+        // an edge or corner ghost (not filled) whose W is already the
+        // con2prim of its U
+        return;
     }
     // ghost con2prim, grid.cpp:538-570
     GPt m = cell_tree(P, b, c);
@@ -283,14 +330,26 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
     add_faults(P, nf);
 }
 
-// con2prim on every stored cell of every leaf, integrator.cpp:201-222
+// con2prim on every stored cell of every leaf, integrator.cpp:201-222.
+// Ghost cells first, then active zones (mostly settled by the early pass),
+// so that wavefronts hold cells of like cost.
 __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leaves) {
     const int b = leaves[blockIdx.x / WG_CELL];
-    const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
-    if (c >= NCELL) return;
-    if (cst_at(P, b, c)) return;
+    const int t = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
+    if (t >= NCELL) return;
+    const int c = P.cell_order[t];
+    const unsigned char st = cst_at(P, b, c);
+    if (st == 1) return;
     int nf = 0;
     GPt m = cell_tree(P, b, c);
+    if (st >= CST_FAIL) {
+        // failure found by the early pass on this same U
+        nf = fail_faults(st);
+        c2p_floor(P, b, c, m, nf);
+        cst_at(P, b, c) = 0;
+        add_faults(P, nf);
+        return;
+    }
     Prim pv;
     int r = c2p_first(P, b, c, m, Q_C2P, pv, nf);
     if (r > 0) {
@@ -300,6 +359,45 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
         c2p_floor(P, b, c, m, nf);
     }
     add_faults(P, nf);
+}
+
+// Active zones whose U a ghost fill mirrors (domain-boundary sides): their
+// pre-con2prim W is read by the fill, so the early pass leaves them alone.
+__device__ __forceinline__ bool mirror_source(const KP& P, int b, int i, int j, int k) {
+    const int* gt = P.ghtab + b * 12;
+    const int x[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (gt[(2 * d) * 2] == GH_BNDY && x[d] < GH) return true;
+        if (gt[(2 * d + 1) * 2] == GH_BNDY && x[d] >= BLK - GH) return true;
+    }
+    return false;
+}
+
+// The con2prim of the active zones of every leaf, run before the ghost fill.
+// The fill reads active zones' U (copies, prolongation) and W (copies, whose
+// ghost con2prim then replaces it or keeps its v and B on failure), never a
+// result of the con2prim phase.  So a success may be stored early (W <-
+// ctop(U), U unchanged, flagged current; the con2prim phase skips it), and a
+// failure is only recorded (its floor state changes U, so the con2prim phase
+// applies it); copied ghosts reuse either outcome.  Zones a boundary mirror
+// reads keep their W and are left to the con2prim phase.
+__global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ leaves) {
+    const int b = leaves[blockIdx.x / WG_INT];
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int c = lindx(i, j, k);
+    if (cst_at(P, b, c) || mirror_source(P, b, i, j, k)) return;
+    int nf = 0;
+    GPt m = cell_tree(P, b, c);
+    Prim pv;
+    const int r = c2p_first(P, b, c, m, Q_PRE, pv, nf);
+    if (r > 0) {
+        storeW(P, b, c, pv);
+        cst_at(P, b, c) = 1;
+    } else if (r == 0) {
+        cst_at(P, b, c) = fail_status(nf);
+    }
 }
 
 // floors semantics after con2prim of an active zone (grid.cpp:342-354)
@@ -323,8 +421,9 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
 
 // Brent fallback for the queued cells, then the rest of each caller's
 // con2prim sequence.
-__global__ __launch_bounds__(TPB) void k_brent(KP P) {
+__global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
     const unsigned int n = *P.qn;
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(P.qstat + pass, (unsigned long long)n);
     for (unsigned int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
         const int b = P.qb[i];
         const int c = P.qc[i] & 0xffff;
@@ -336,6 +435,16 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P) {
         double xi;
         Prim pv;
         bool ok = ctop_brent(P.gamma, s, P.qx[i], xi) && ctop_finish(P.gamma, s, xi, pv, nf);
+        if (kind == Q_PRE) {
+            // early con2prim pass: store a success, record a failure
+            if (ok) {
+                storeW(P, b, c, pv);
+                cst_at(P, b, c) = 1;
+            } else {
+                cst_at(P, b, c) = fail_status(nf);
+            }
+            continue;
+        }
         if (kind == Q_FLOOR) {
             floors_finish(P, b, c, m, !ok, pv, ok, nf);
         } else if (ok) {
@@ -420,11 +529,15 @@ __global__ __launch_bounds__(TPB) void k_flux(KP P) {
             L.B[d] = fl; R.B[d] = fr;
         }
     }
-    // normal field from the face (grid.cpp:263-264, 287-288, 311-312)
+    // normal field from the face (grid.cpp:263-264, 287-288, 311-312); the
+    // unrolled select keeps L.B and R.B in registers
     const double* Bf = P.Bf + ((size_t)b * 3 + dir) * NFACE;
     const int bidx = dir == 0 ? ixf(i + 1, j, k) : (dir == 1 ? iyf(i, j + 1, k) : izf(i, j, k + 1));
-    L.B[dir] = Bf[bidx];
-    R.B[dir] = Bf[bidx];
+    const double bn = Bf[bidx];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d == dir) { L.B[d] = bn; R.B[d] = bn; }
+    }
     // tree metric at the face
     const double* gf = P.gfc + ((size_t)b * 3 + dir) * NGFC * NFC + lf;
     GPt m;
@@ -802,8 +915,9 @@ public:
     GhostPlan plan;
     DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz;
     DBuf<int> ghtab, leaves, waveblk;
+    DBuf<unsigned char> ghsame;
     std::vector<int> wave_off;  // offsets into waveblk
-    DBuf<short> ghost_cells;
+    DBuf<short> cell_order;
     DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
     DBuf<double> dts;               // per-step dt of a multi-step run
     DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
@@ -812,6 +926,9 @@ public:
     DBuf<unsigned int> qn;          // Brent queue
     DBuf<int> qb, qc;
     DBuf<double> qx;
+    // cells queued for the Brent pass, per pass
+    enum { QS_PRE = 0, QS_GHOST = 1, QS_C2P = 2, QS_FLOOR = 3, QS_N = 4 };
+    DBuf<unsigned long long> qstat;
 
     // Per-kernel timing with HIP events on this tree's stream.
     bool profile = false;
@@ -867,11 +984,13 @@ public:
         P.gamma = gamma; P.limiter = limiter;
         P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p; P.E = E.p;
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p;
-        P.ghtab = ghtab.p; P.ghost_cells = ghost_cells.p;
+        P.ghtab = ghtab.p; P.ghsame = ghsame.p;
+        P.ghost_order = cell_order.p; P.cell_order = cell_order.p + NGHOST;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
         P.lg = lg.p;
         P.qn = qn.p; P.qb = qb.p; P.qc = qc.p; P.qx = qx.p;
+        P.qstat = qstat.p;
         return P;
     }
 
@@ -882,6 +1001,7 @@ public:
         GeomHost g;
         build_geometry(t, g);
         build_ghost_plan(t, plan);
+        mark_exact_copies(g, plan);
         U.alloc((size_t)nb * 8 * NCELL);
         W.alloc((size_t)nb * 8 * NCELL);
         Bf.alloc((size_t)nb * 3 * NFACE);
@@ -898,6 +1018,7 @@ public:
         gchr.alloc(g.chr.size()); gchr.upload(g.chr.data(), g.chr.size(), stream);
         dxyz.alloc(g.dxyz.size()); dxyz.upload(g.dxyz.data(), g.dxyz.size(), stream);
         ghtab.alloc(plan.tab.size()); ghtab.upload(plan.tab.data(), plan.tab.size(), stream);
+        ghsame.alloc(plan.same.size()); ghsame.upload(plan.same.data(), plan.same.size(), stream);
         leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
         if (!plan.leaves.empty()) leaves.upload(plan.leaves.data(), plan.leaves.size(), stream);
         std::vector<int> wb;
@@ -908,12 +1029,17 @@ public:
         }
         waveblk.alloc(std::max<size_t>(1, wb.size()));
         if (!wb.empty()) waveblk.upload(wb.data(), wb.size(), stream);
-        std::vector<short> gc;
+        // storage-order ghost list, then the con2prim order
+        std::vector<short> order;
         for (int c = 0; c < NCELL; c++) {
             int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
-            if (i < 0 || i >= BLK || j < 0 || j >= BLK || k < 0 || k >= BLK) gc.push_back((short)c);
+            if (i < 0 || i >= BLK || j < 0 || j >= BLK || k < 0 || k >= BLK) order.push_back((short)c);
         }
-        ghost_cells.alloc(gc.size()); ghost_cells.upload(gc.data(), gc.size(), stream);
+        const std::vector<short> by_side = ghost_cell_order();
+        if ((int)order.size() != NGHOST || (int)by_side.size() != NGHOST) throw DevError("ghost cell table size");
+        order.insert(order.end(), by_side.begin(), by_side.end());
+        for (int a = 0; a < NINT; a++) order.push_back((short)lindx(a / 256, (a / 16) % 16, a % 16));
+        cell_order.alloc(order.size()); cell_order.upload(order.data(), order.size(), stream);
         scal.alloc(2);
         HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
         cst.alloc((size_t)nb * NCELL);
@@ -923,6 +1049,8 @@ public:
             size_t qcap = std::max<size_t>({(size_t)1, plan.leaves.size() * NCELL, (size_t)nb * NINT});
             qn.alloc(1); qb.alloc(qcap); qc.alloc(qcap); qx.alloc(qcap);
         }
+        qstat.alloc(QS_N);
+        HIPCHK(hipMemsetAsync(qstat.p, 0, QS_N * sizeof(unsigned long long), stream));
         HIPCHK(hipStreamSynchronize(stream));
     }
 
@@ -962,18 +1090,23 @@ public:
     }
     void launch_rkstg(int stage) {
         KP P = kp();
+        int nl = (int)plan.leaves.size();
+        if (nl) {
+            launch("k_qreset", k_qreset, dim3(1), P);
+            launch("k_c2p_pre", k_c2p_pre, dim3(nl * WG_INT), P, (const int*)leaves.p);
+            launch("k_brent", k_brent, dim3(BRENT_WG), P, (int)QS_PRE);
+        }
         for (size_t w = 0; w + 1 < wave_off.size(); w++) {
             int n = wave_off[w + 1] - wave_off[w];
             if (!n) continue;
             launch("k_qreset", k_qreset, dim3(1), P);
             launch("k_ghost", k_ghost, dim3(n * WG_GHOST), P, (const int*)(waveblk.p + wave_off[w]));
-            launch("k_brent", k_brent, dim3(BRENT_WG), P);
+            launch("k_brent", k_brent, dim3(BRENT_WG), P, (int)QS_GHOST);
         }
-        int nl = (int)plan.leaves.size();
         if (nl) {
             launch("k_qreset", k_qreset, dim3(1), P);
             launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
-            launch("k_brent", k_brent, dim3(BRENT_WG), P);
+            launch("k_brent", k_brent, dim3(BRENT_WG), P, (int)QS_C2P);
         }
         launch("k_logs", k_logs, dim3(nb * WG_CELL), P);
         launch("k_flux", k_flux, dim3(nb * WG_FACE), P);
@@ -986,7 +1119,7 @@ public:
         KP P = kp();
         launch("k_qreset", k_qreset, dim3(1), P);
         launch("k_floors", k_floors, dim3(nb * WG_INT), P);
-        launch("k_brent", k_brent, dim3(BRENT_WG), P);
+        launch("k_brent", k_brent, dim3(BRENT_WG), P, (int)QS_FLOOR);
     }
     void launch_step(double cfl) {
         launch_dtcomp(cfl);
@@ -1286,7 +1419,12 @@ int bh_profile(bh_tree* t, int enable) {
         if (!t->dev) t->dev = std::make_unique<DeviceTree>();
         t->dev->collect();
         t->dev->profile = enable != 0;
-        if (enable) t->dev->acc.clear();
+        if (enable) {
+            t->dev->acc.clear();
+            if (t->dev->qstat.p)
+                HIPCHK(hipMemsetAsync(t->dev->qstat.p, 0, DeviceTree::QS_N * sizeof(unsigned long long),
+                                      t->dev->stream));
+        }
         return BH_OK;
     });
 }
@@ -1298,6 +1436,14 @@ int bh_profile_report(bh_tree* t, char* buf, int len) {
             char line[160];
             for (auto& a : t->dev->acc) {
                 std::snprintf(line, sizeof line, "%s %ld %.6f\n", a.name.c_str(), a.n, a.ms);
+                out += line;
+            }
+            // cells queued for the Brent pass since profiling was enabled
+            unsigned long long q[DeviceTree::QS_N] = {0, 0, 0, 0};
+            if (t->dev->qstat.p) HIPCHK(hipMemcpy(q, t->dev->qstat.p, sizeof q, hipMemcpyDeviceToHost));
+            const char* qn[DeviceTree::QS_N] = {"pre", "ghost", "c2p", "floor"};
+            for (int i = 0; i < DeviceTree::QS_N; i++) {
+                std::snprintf(line, sizeof line, "queue:%s %llu 0\n", qn[i], q[i]);
                 out += line;
             }
         }

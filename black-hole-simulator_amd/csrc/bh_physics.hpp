@@ -165,16 +165,24 @@ BH_HD bool c2p_brent(double gm, double xa, double xb, double Ssq, double Bsq, do
     double s = 0.0, d = 0.0;
     for (int i = 0; i < 100; i++) {
         if (fabs(xb - xa) < tol) { root = xb; return true; }
-        if (fa != fc && fb != fc) {
-            s = (xa * fb * fc) / ((fa - fb) - (fa - fc)) + (xb * fa * fc) / ((fb - fa) - (fb - fc)) +
-                (xc * fa * fb) / ((fc - fa) - (fc - fb));
-        } else {
-            s = xb - fb * (xb - xa) / (fb - fa);
+        // The interpolated point is kept only if lo < s < xb.  When lo < xb
+        // fails no s can pass (NaN included), the step bisects, and the
+        // interpolant, which nothing else reads, is not evaluated.
+        const double lo = (3 * xa + 4) / 4;
+        bool bisect = true;
+        if (lo < xb) {
+            if (fa != fc && fb != fc) {
+                s = (xa * fb * fc) / ((fa - fb) - (fa - fc)) + (xb * fa * fc) / ((fb - fa) - (fb - fc)) +
+                    (xc * fa * fb) / ((fc - fa) - (fc - fb));
+            } else {
+                s = xb - fb * (xb - xa) / (fb - fa);
+            }
+            bool c1 = !(lo < s && s < xb);
+            bool c2 = mflag && fabs(s - xb) >= fabs(xb - xc) / 2;
+            bool c3 = !mflag && fabs(s - xb) >= fabs(xc - d) / 2;
+            bisect = c1 || c2 || c3;
         }
-        bool c1 = !((3 * xa + 4) / 4 < s && s < xb);
-        bool c2 = mflag && fabs(s - xb) >= fabs(xb - xc) / 2;
-        bool c3 = !mflag && fabs(s - xb) >= fabs(xc - d) / 2;
-        if (c1 || c2 || c3) { s = (xa + xb) / 2; mflag = true; }
+        if (bisect) { s = (xa + xb) / 2; mflag = true; }
         else mflag = false;
         double fs = c2p_f(gm, s, Ssq, Bsq, SB, D, tau);
         d = xc; xc = xb; fc = fb;

@@ -817,6 +817,57 @@ void build_ghost_plan(const HostTree& t, GhostPlan& g) {
     }
     g.waves.assign(nw, {});
     for (int b : g.leaves) g.waves[wave[b]].push_back(b);
+    g.same.assign((size_t)nb * 6, 0);
+}
+
+namespace {
+int cell_on_axis(int dim, int a, int t1, int t2) {
+    return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
+}
+}  // namespace
+
+// A copied ghost cell (grid.cpp:390-425) holds its source's U; its con2prim
+// sees the same U and, when the two cells' metric bits agree, returns the
+// source's result.  Cell centres on either side of a same-level face are
+// normally computed to the same bits; this checks it for every such pair.
+void mark_exact_copies(const GeomHost& geo, GhostPlan& g) {
+    const size_t nb = g.tab.size() / 12;
+    g.same.assign(nb * 6, 0);
+    for (int b : g.leaves)
+        for (int side = 0; side < 6; side++) {
+            const int kind = g.tab[(size_t)b * 12 + side * 2], q = g.tab[(size_t)b * 12 + side * 2 + 1];
+            if (kind != GH_COPY) continue;
+            const int dim = side / 2, s = side % 2;
+            bool eq = true;
+            for (int gl = 0; gl < GH && eq; gl++)
+                for (int t1 = 0; t1 < BLK && eq; t1++)
+                    for (int t2 = 0; t2 < BLK && eq; t2++) {
+                        const int c = cell_on_axis(dim, s ? BLK + gl : gl - GH, t1, t2);
+                        const int sc = cell_on_axis(dim, (s ? 0 : BLK - GH) + gl, t1, t2);
+                        for (int v = 0; v < NGCT; v++) {
+                            const double x = geo.cell_tree[((size_t)b * NGCT + v) * NCELL + c];
+                            const double y = geo.cell_tree[((size_t)q * NGCT + v) * NCELL + sc];
+                            if (std::memcmp(&x, &y, sizeof x) != 0) { eq = false; break; }
+                        }
+                    }
+            g.same[(size_t)b * 6 + side] = eq ? 1 : 0;
+        }
+}
+
+std::vector<short> ghost_cell_order() {
+    std::vector<short> out;
+    for (int side = 0; side < 6; side++) {
+        const int dim = side / 2, s = side % 2;
+        for (int gl = 0; gl < GH; gl++)
+            for (int t1 = 0; t1 < BLK; t1++)
+                for (int t2 = 0; t2 < BLK; t2++) out.push_back((short)cell_on_axis(dim, s ? BLK + gl : gl - GH, t1, t2));
+    }
+    for (int c = 0; c < NCELL; c++) {
+        const int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
+        const int o = (i < 0 || i >= BLK) + (j < 0 || j >= BLK) + (k < 0 || k >= BLK);
+        if (o >= 2) out.push_back((short)c);
+    }
+    return out;
 }
 
 }  // namespace bh

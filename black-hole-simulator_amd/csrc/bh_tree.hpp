@@ -121,7 +121,15 @@ struct GhostPlan {
     std::vector<int> leaves;              // leaf block ids in quilt order
     std::vector<int> tab;                 // [nb][6][2] (kind, neighbour), side = 2*dim + (side>0)
     std::vector<std::vector<int>> waves;  // leaf ids per wave
+    // [nb][6]: 1 when every ghost cell a same-level copy fills on that side
+    // has the same tree-metric bits as its source cell, so that the ghost's
+    // con2prim equals the source's (see mark_exact_copies)
+    std::vector<unsigned char> same;
 };
 void build_ghost_plan(const HostTree& t, GhostPlan& g);
+void mark_exact_copies(const GeomHost& geo, GhostPlan& g);
+// Ghost cells of a block: the 6 x 512 face-ghost cells grouped by side
+// (dim 0 low, dim 0 high, dim 1 low, ...), then the 832 edge and corner cells.
+std::vector<short> ghost_cell_order();
 
 }  // namespace bh

@@ -81,7 +81,9 @@ int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out);
  * first step, so that set-up is not timed with it. */
 int bh_tree_prepare(bh_tree* t);
 /* Per-kernel timing with HIP events on the tree's stream: enable, then read
- * "name launches total_ms" lines.  Measurement only. */
+ * "name launches total_ms" lines, followed by "queue:<pass> cells 0" lines
+ * counting the cells each con2prim pass handed to the Brent fallback.
+ * Measurement only. */
 int bh_profile(bh_tree* t, int enable);
 int bh_profile_report(bh_tree* t, char* buf, int len);
 int bh_device_sync(void);
