@@ -203,12 +203,12 @@ def main():
     dom_name = max(prof, key=lambda k: prof[k][1])
     n_launch, tot_ms = prof[dom_name]
     avg_ms = tot_ms / n_launch
-    blocks = nb if KERNEL_SCOPE[dom_name] == "all" else nleaf
+    blocks = nb if KERNEL_SCOPE.get(dom_name, "leaf") == "all" else nleaf
     if dom_name == "k_ghost":
         blocks = nleaf / max(1, n_launch / (2 * args.steps))  # one launch per wave
     algo = ALGO_BYTES_PER_BLOCK.get(dom_name, 0) * blocks
-    if dom_name == "k_brent":  # U in, W out for each queued cell
-        algo = sum(queued.values()) / n_launch * 16 * W8
+    if dom_name == "k_brent":  # staged input (11 words) and bracket in, W out for each queued cell
+        algo = sum(queued.values()) / n_launch * 20 * W8
     achieved = algo / (avg_ms * 1e-3) / 1e9
     kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "avg_ms": round(v[1] / v[0], 4)}
                for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}

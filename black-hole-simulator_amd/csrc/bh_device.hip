@@ -45,6 +45,15 @@ constexpr int WG_EDGE = (19 * 19 * 19 + TPB - 1) / TPB;  // 27
 constexpr int WG_BF = (18 * 18 * 18 + TPB - 1) / TPB;    // 23
 constexpr int WG_INT = NINT / TPB;                   // 16
 constexpr int BRENT_WG = 2048;                       // grid of the Brent pass
+// The Brent queue is split into NSUBQ sub-queues, one counter each on its
+// own 128-byte line (see subq_of).  Appends to one counter serialise at its
+// L2 channel (one wavefront atomic at a time), which made a single counter
+// the bottleneck of the con2prim passes; NSUBQ counters spread them over
+// the channels.
+constexpr int NSUBQ = 256;
+constexpr int QN_STRIDE = 32;                        // uints per counter line
+static_assert(NSUBQ <= TPB, "k_qreset clears one counter per thread");
+static_assert(BRENT_WG % NSUBQ == 0, "every sub-queue gets the same number of Brent workgroups");
 
 // Kernel view of the device tree.
 struct KP {
@@ -84,10 +93,15 @@ struct KP {
     // Cells whose con2prim needs the Brent fallback are queued by the first
     // pass (Newton for every cell) and finished by k_brent over a compact
     // list, so that a few slow cells do not hold whole wavefronts.
-    unsigned int* qn;     // queue length
+    unsigned int* qn;     // sub-queue lengths, qn[k * QN_STRIDE]
+    unsigned int qcap;    // capacity of each sub-queue
     int* qb;              // block
     int* qc;              // cell | kind << 16
     double* qx;           // Newton bracket upper end
+    // con2prim input of each queued cell, [QS_WORDS][NSUBQ * qcap]: U, then
+    // sqrt(gamma), gamma^13, gamma^33 of the tree metric, stored by the pusher
+    // so that the Brent pass reads its inputs contiguously
+    double* qs;
     unsigned long long* qstat;  // queued cells per pass (diagnostic)
 };
 enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
@@ -181,21 +195,51 @@ __device__ __forceinline__ bool c2p_cell(const KP& P, int b, int c, int& nf) {
 }
 __device__ __forceinline__ unsigned char& cst_at(const KP& P, int b, int c) { return P.cst[(size_t)b * NCELL + c]; }
 
+// Sub-queue of a pushing workgroup: each run of NSUBQ consecutive workgroups
+// covers every sub-queue once, rotated by 37 per run, so that a sub-queue
+// collects workgroups from different positions within their blocks (cells
+// that need the fallback cluster by position) and at most
+// ceil(grid / NSUBQ) workgroups.
+__device__ __forceinline__ unsigned int subq_of(unsigned int wg) {
+    return (wg % NSUBQ + (wg / NSUBQ) * 37u) % NSUBQ;
+}
+constexpr int QS_WORDS = 11;
+
 // Queue slots are claimed once per wavefront: the lowest pushing lane adds
-// the number of pushing lanes to the queue length and each lane takes the
-// slot given by its rank among them (one atomic per wavefront instead of
-// one per cell on a single contended counter).
-__device__ __forceinline__ void push_brent(const KP& P, int b, int c, int kind, double xi_h) {
+// the number of pushing lanes to its workgroup's sub-queue length and each
+// lane takes the slot given by its rank among them.  A sub-queue receives
+// the cells of at most ceil(grid / NSUBQ) workgroups, which its capacity
+// covers.
+__device__ __forceinline__ void push_brent(const KP& P, int b, int c, int kind, double xi_h, const Cons& u,
+                                           const GPt& m) {
     const unsigned long long mask = __ballot(1);
     const int lane = (int)__lane_id();
     const int leader = __ffsll((unsigned long long)mask) - 1;
+    const unsigned int q = subq_of(blockIdx.x);
     unsigned int base = 0;
-    if (lane == leader) base = atomicAdd(P.qn, (unsigned int)__popcll(mask));
+    if (lane == leader) base = atomicAdd(P.qn + q * QN_STRIDE, (unsigned int)__popcll(mask));
     base = __shfl(base, leader);
-    const unsigned int i = base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
+    const size_t i = (size_t)q * P.qcap + base + (unsigned int)__popcll(mask & ((1ull << lane) - 1ull));
     P.qb[i] = b;
     P.qc[i] = c | (kind << 16);
     P.qx[i] = xi_h;
+    const size_t st = (size_t)NSUBQ * P.qcap;
+    double* e = P.qs + i;
+    e[0] = u.D; e[st] = u.S[0]; e[2 * st] = u.S[1]; e[3 * st] = u.S[2]; e[4 * st] = u.tau;
+    e[5 * st] = u.B[0]; e[6 * st] = u.B[1]; e[7 * st] = u.B[2];
+    e[8 * st] = m.sqrtdetgam; e[9 * st] = m.gi13; e[10 * st] = m.gi33;
+}
+// con2prim set-up of queue entry i from its stored input
+__device__ __forceinline__ void queued_setup(const KP& P, size_t i, C2P& s) {
+    const size_t st = (size_t)NSUBQ * P.qcap;
+    const double* q = P.qs + i;
+    Cons u;
+    u.D = q[0]; u.S[0] = q[st]; u.S[1] = q[2 * st]; u.S[2] = q[3 * st]; u.tau = q[4 * st];
+    u.B[0] = q[5 * st]; u.B[1] = q[6 * st]; u.B[2] = q[7 * st];
+    GPt m;
+    m.sqrtdetg = 0.0; m.alpha = 0.0; m.beta0 = 0.0; m.g11 = 0.0; m.g13 = 0.0; m.g22 = 0.0; m.g33 = 0.0;
+    m.sqrtdetgam = q[8 * st]; m.gi13 = q[9 * st]; m.gi33 = q[10 * st];
+    ctop_setup(m, u, s);
 }
 // c2p failure path (integrator.cpp:211-218): floor state with the cell's B
 __device__ __forceinline__ void c2p_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
@@ -218,10 +262,11 @@ __device__ __forceinline__ void ghost_floor(const KP& P, int b, int c, const GPt
 // First con2prim pass: 1 = solved (W written when ok), 0 = failed, -1 = queued
 __device__ __forceinline__ int c2p_first(const KP& P, int b, int c, const GPt& m, int kind, Prim& pv, int& nf) {
     C2P s;
-    ctop_setup(m, loadU(P, b, c), s);
+    const Cons u = loadU(P, b, c);
+    ctop_setup(m, u, s);
     double xi, xi_h;
     if (!ctop_newton(P.gamma, s, xi, xi_h)) {
-        push_brent(P, b, c, kind, xi_h);
+        push_brent(P, b, c, kind, xi_h, u, m);
         return -1;
     }
     return ctop_finish(P.gamma, s, xi, pv, nf) ? 1 : 0;
@@ -421,20 +466,38 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
 
 // Brent fallback for the queued cells, then the rest of each caller's
 // con2prim sequence.
+// Workgroup w serves sub-queue w % NSUBQ together with the other
+// BRENT_WG / NSUBQ workgroups of that residue.
+#define FOR_QUEUED(i)                                                                 \
+    const unsigned int q_ = blockIdx.x % NSUBQ;                                       \
+    const unsigned int n_ = P.qn[q_ * QN_STRIDE];                                     \
+    const unsigned int part_ = blockIdx.x / NSUBQ, nparts_ = gridDim.x / NSUBQ;       \
+    for (unsigned int j_ = part_ * TPB + threadIdx.x; j_ < n_; j_ += nparts_ * TPB) { \
+        const size_t i = (size_t)q_ * P.qcap + j_;
+
+// Brent fallback (conserved.cpp:232-285) for the queued cells, from the
+// con2prim input their pushers stored, then the rest of each caller's
+// con2prim sequence: primitive recovery from the root and what the caller
+// does on success or failure.
 __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
-    const unsigned int n = *P.qn;
-    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(P.qstat + pass, (unsigned long long)n);
-    for (unsigned int i = blockIdx.x * TPB + threadIdx.x; i < n; i += gridDim.x * TPB) {
+    if (blockIdx.x < NSUBQ && threadIdx.x == 0) {
+        const unsigned int n = P.qn[blockIdx.x * QN_STRIDE];
+        if (n) atomicAdd(P.qstat + pass, (unsigned long long)n);
+    }
+    FOR_QUEUED(i)
         const int b = P.qb[i];
-        const int c = P.qc[i] & 0xffff;
-        const int kind = P.qc[i] >> 16;
+        const int qc = P.qc[i];
+        const int c = qc & 0xffff;
+        const int kind = qc >> 16;
         int nf = 0;
-        GPt m = cell_tree(P, b, c);
-        C2P s;
-        ctop_setup(m, loadU(P, b, c), s);
-        double xi;
         Prim pv;
-        bool ok = ctop_brent(P.gamma, s, P.qx[i], xi) && ctop_finish(P.gamma, s, xi, pv, nf);
+        bool ok = false;
+        {
+            C2P s;
+            queued_setup(P, i, s);
+            double xi;
+            ok = ctop_brent(P.gamma, s, P.qx[i], xi) && ctop_finish(P.gamma, s, xi, pv, nf);
+        }
         if (kind == Q_PRE) {
             // early con2prim pass: store a success, record a failure
             if (ok) {
@@ -445,6 +508,7 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
             }
             continue;
         }
+        const GPt m = cell_tree(P, b, c);
         if (kind == Q_FLOOR) {
             floors_finish(P, b, c, m, !ok, pv, ok, nf);
         } else if (ok) {
@@ -470,8 +534,9 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
         add_faults(P, nf);
     }
 }
+#undef FOR_QUEUED
 __global__ void k_qreset(KP P) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *P.qn = 0u;
+    if (threadIdx.x < NSUBQ) P.qn[threadIdx.x * QN_STRIDE] = 0u;
 }
 
 // ln(rho) and ln(eps) of every stored cell, for the log-space reconstruction
@@ -923,9 +988,10 @@ public:
     DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
     DBuf<unsigned long long> dtpart;  // per-workgroup dt candidates
     DBuf<double> lg;                // [nb][2][8000] log(rho), log(eps)
-    DBuf<unsigned int> qn;          // Brent queue
+    DBuf<unsigned int> qn;          // Brent sub-queue lengths
+    unsigned int qcap = 0;          // capacity of each sub-queue
     DBuf<int> qb, qc;
-    DBuf<double> qx;
+    DBuf<double> qx, qs;
     // cells queued for the Brent pass, per pass
     enum { QS_PRE = 0, QS_GHOST = 1, QS_C2P = 2, QS_FLOOR = 3, QS_N = 4 };
     DBuf<unsigned long long> qstat;
@@ -989,7 +1055,7 @@ public:
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
         P.lg = lg.p;
-        P.qn = qn.p; P.qb = qb.p; P.qc = qc.p; P.qx = qx.p;
+        P.qn = qn.p; P.qcap = qcap; P.qb = qb.p; P.qc = qc.p; P.qx = qx.p; P.qs = qs.p;
         P.qstat = qstat.p;
         return P;
     }
@@ -1046,8 +1112,13 @@ public:
         dtpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT));
         lg.alloc((size_t)nb * 2 * NCELL);
         {
-            size_t qcap = std::max<size_t>({(size_t)1, plan.leaves.size() * NCELL, (size_t)nb * NINT});
-            qn.alloc(1); qb.alloc(qcap); qc.alloc(qcap); qx.alloc(qcap);
+            // the largest pushing grid, in workgroups: c2p over leaves, floors over all blocks
+            const size_t wg = std::max<size_t>({(size_t)1, plan.leaves.size() * WG_CELL, (size_t)nb * WG_INT});
+            qcap = (unsigned int)(((wg + NSUBQ - 1) / NSUBQ) * TPB);
+            const size_t total = (size_t)NSUBQ * qcap;
+            qn.alloc((size_t)NSUBQ * QN_STRIDE);
+            HIPCHK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(unsigned int), stream));
+            qb.alloc(total); qc.alloc(total); qx.alloc(total); qs.alloc(total * QS_WORDS);
         }
         qstat.alloc(QS_N);
         HIPCHK(hipMemsetAsync(qstat.p, 0, QS_N * sizeof(unsigned long long), stream));
