@@ -55,6 +55,19 @@ constexpr int QN_STRIDE = 32;                        // uints per counter line
 static_assert(NSUBQ <= TPB, "k_qreset clears one counter per thread");
 static_assert(BRENT_WG % NSUBQ == 0, "every sub-queue gets the same number of Brent workgroups");
 
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup p runs on
+// XCD p % 8) and each XCD has its own L2.  Kernels whose neighbouring work
+// items share inputs (shared faces) renumber their workgroups so that each
+// XCD takes a contiguous range of the work: the workgroups of one block then
+// run on one XCD and share its L2 instead of fetching the block's data into
+// several.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_local(int p, int G) {
+    const int per = G / NXCD;
+    if (p >= per * NXCD) return p;  // the last G % 8 workgroups keep their number
+    return (p % NXCD) * per + p / NXCD;
+}
+
 // Kernel view of the device tree.
 struct KP {
     double gamma;
@@ -665,8 +678,9 @@ __global__ __launch_bounds__(TPB) void k_bfupd(KP P) {
 // constrans::f2cB, transport.cpp:60-75: cell B = face average; U.B uses the
 // cell metric's sqrt(-g) (quirk Q4)
 __global__ __launch_bounds__(TPB) void k_f2cb(KP P) {
-    const int b = blockIdx.x / WG_EDGE;
-    const int t = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
+    const int wg = xcd_local(blockIdx.x, gridDim.x);
+    const int b = wg / WG_EDGE;
+    const int t = (wg % WG_EDGE) * TPB + threadIdx.x;
     if (t >= 19 * 19 * 19) return;
     const int i = t / 361 - 1, j = (t / 19) % 19 - 1, k = t % 19 - 1;
     const double* Bx = P.Bf + (size_t)b * 3 * NFACE;
@@ -744,8 +758,9 @@ __device__ __forceinline__ void source_terms(const KP& P, int b, int c, int a, c
 // quirks Q20, Q22, Q31): dU = div F + S; stage 0 stores U and takes an Euler
 // step, stage 1 averages; U.B is reset from the cell-centred field.
 __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
-    const int b = blockIdx.x / WG_INT;
-    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int wg = xcd_local(blockIdx.x, gridDim.x);
+    const int b = wg / WG_INT;
+    const int a = (wg % WG_INT) * TPB + threadIdx.x;
     const int i = a / 256, j = (a / 16) % 16, k = a % 16;
     const int c = lindx(i, j, k);
     const double dt = __longlong_as_double((long long)*P.dtbits);
