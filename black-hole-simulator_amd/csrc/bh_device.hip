@@ -315,15 +315,22 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             int src_b, src_c;
             if (kind == GH_COPY) { src_b = nb; src_c = cell_of(dim, (s ? 0 : BLK - GH) + g, t1, t2); }
             else { src_b = b; src_c = cell_of(dim, s ? BLK - 1 - g : g, t1, t2); }
+            // all sixteen loads (and the source's con2prim status) ahead of
+            // the stores into the same arrays
+            double u[8], w[8];
             for (int v = 0; v < 8; v++) {
-                Ua(P, b, v, c) = Ua(P, src_b, v, src_c);
-                Wa(P, b, v, c) = Wa(P, src_b, v, src_c);
+                u[v] = Ua(P, src_b, v, src_c);
+                w[v] = Wa(P, src_b, v, src_c);
+            }
+            const unsigned char st = cst_at(P, src_b, src_c);
+            for (int v = 0; v < 8; v++) {
+                Ua(P, b, v, c) = u[v];
+                Wa(P, b, v, c) = w[v];
             }
             // The source holds this same U at the same metric bits, so its
             // known con2prim outcome is the ghost's: success returns the
             // source's W (copied above), failure takes the ghost floor path.
             if (kind == GH_COPY && P.ghsame[b * 6 + 2 * dim + s]) {
-                const unsigned char st = cst_at(P, src_b, src_c);
                 if (st == 1) {
                     cst_at(P, b, c) = 1;
                     return;
@@ -349,13 +356,15 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             else if (dim == 1) { on = offt1; ot1 = offg; ot2 = offt2; }
             else { on = offt1; ot1 = offt2; ot2 = offg; }
             const int cc = lindx(ci, cj, ck);
+            double nu[8];
             for (int v = 0; v < 8; v++) {
                 const double vc = Ua(P, nb, v, cc);
                 double sn = minmod(vc - Ua(P, nb, v, lindx(ci - 1, cj, ck)), Ua(P, nb, v, lindx(ci + 1, cj, ck)) - vc);
                 double st1 = minmod(vc - Ua(P, nb, v, lindx(ci, cj - 1, ck)), Ua(P, nb, v, lindx(ci, cj + 1, ck)) - vc);
                 double st2 = minmod(vc - Ua(P, nb, v, lindx(ci, cj, ck - 1)), Ua(P, nb, v, lindx(ci, cj, ck + 1)) - vc);
-                Ua(P, b, v, c) = vc + sn * on + st1 * ot1 + st2 * ot2;
+                nu[v] = vc + sn * on + st1 * ot1 + st2 * ot2;
             }
+            for (int v = 0; v < 8; v++) Ua(P, b, v, c) = nu[v];
             GPt mp = cell_tree(P, b, c);
             Prim pv;
             int r = c2p_first(P, b, c, mp, Q_PROLONG, pv, nf);
@@ -560,8 +569,10 @@ __global__ __launch_bounds__(TPB) void k_logs(KP P) {
     const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
     if (c >= NCELL) return;
     double* lr = P.lg + (size_t)b * 2 * NCELL;
-    lr[c] = cr_log(Wa(P, b, WRHO, c));
-    lr[NCELL + c] = cr_log(Wa(P, b, WEPS, c));
+    const double rho = Wa(P, b, WRHO, c), eps = Wa(P, b, WEPS, c);
+    const double l0 = cr_log(rho), l1 = cr_log(eps);
+    lr[c] = l0;
+    lr[NCELL + c] = l1;
 }
 
 // ---------------------------------------------------------------- fluxes
@@ -644,10 +655,15 @@ __global__ __launch_bounds__(TPB) void k_emf(KP P) {
     auto fB = [&](int dir, int comp, int ii, int jj, int kk) -> double {
         return face_in(dir, ii, jj, kk) ? F[(dir * 8 + 5 + comp) * NFC + face_lf(dir, ii, jj, kk)] : 0.0;
     };
+    // all loads ahead of the stores (E and F are both double arrays, so a
+    // store could otherwise hold back the loads after it)
+    const double a0 = fB(1, 2, i, j, k), a1 = fB(1, 2, i, j, k + 1), a2 = fB(2, 1, i, j, k), a3 = fB(2, 1, i, j + 1, k);
+    const double b0 = fB(2, 0, i, j, k), b1 = fB(2, 0, i + 1, j, k), b2 = fB(0, 2, i, j, k), b3 = fB(0, 2, i, j, k + 1);
+    const double c0 = fB(0, 1, i, j, k), c1 = fB(0, 1, i, j + 1, k), c2 = fB(1, 0, i, j, k), c3 = fB(1, 0, i + 1, j, k);
     double* E = P.E + (size_t)b * 3 * NEDGE;
-    E[iex(i, j, k)] = (fB(1, 2, i, j, k) + fB(1, 2, i, j, k + 1) - fB(2, 1, i, j, k) - fB(2, 1, i, j + 1, k)) / 4;
-    E[NEDGE + izf(i, j, k)] = (fB(2, 0, i, j, k) + fB(2, 0, i + 1, j, k) - fB(0, 2, i, j, k) - fB(0, 2, i, j, k + 1)) / 4;
-    E[2 * NEDGE + iyf(i, j, k)] = (fB(0, 1, i, j, k) + fB(0, 1, i, j + 1, k) - fB(1, 0, i, j, k) - fB(1, 0, i + 1, j, k)) / 4;
+    E[iex(i, j, k)] = (a0 + a1 - a2 - a3) / 4;
+    E[NEDGE + izf(i, j, k)] = (b0 + b1 - b2 - b3) / 4;
+    E[2 * NEDGE + iyf(i, j, k)] = (c0 + c1 - c2 - c3) / 4;
 }
 
 // constrans::Bfupdate, transport.cpp:38-58 (not RK-averaged, quirk Q21)
@@ -664,15 +680,20 @@ __global__ __launch_bounds__(TPB) void k_bfupd(KP P) {
     double* Bx = P.Bf + (size_t)b * 3 * NFACE;
     double* By = Bx + NFACE;
     double* Bz = Bx + 2 * NFACE;
-    double dEz_dy = (Ez[iyf(i, j, k)] - Ez[iyf(i, j - 1, k)]) / dy;
-    double dEy_dz = (Ey[izf(i, j, k)] - Ey[izf(i, j, k - 1)]) / dz;
-    Bx[ixf(i, j, k)] += dt * (dEz_dy - dEy_dz);
-    double dEz_dx = (Ez[iyf(i, j, k)] - Ez[iyf(i - 1, j, k)]) / dx;
-    double dEx_dz = (Ex[iex(i, j, k)] - Ex[iex(i, j, k - 1)]) / dz;
-    By[iyf(i, j, k)] += dt * (dEx_dz - dEz_dx);
-    double dEy_dx = (Ey[izf(i, j, k)] - Ey[izf(i - 1, j, k)]) / dx;
-    double dEx_dy = (Ex[iex(i, j, k)] - Ex[iex(i, j - 1, k)]) / dy;
-    Bz[izf(i, j, k)] += dt * (dEy_dx - dEx_dy);
+    // all loads ahead of the stores
+    const double ez = Ez[iyf(i, j, k)], ez_jm = Ez[iyf(i, j - 1, k)], ez_im = Ez[iyf(i - 1, j, k)];
+    const double ey = Ey[izf(i, j, k)], ey_km = Ey[izf(i, j, k - 1)], ey_im = Ey[izf(i - 1, j, k)];
+    const double ex = Ex[iex(i, j, k)], ex_km = Ex[iex(i, j, k - 1)], ex_jm = Ex[iex(i, j - 1, k)];
+    const double bx = Bx[ixf(i, j, k)], by = By[iyf(i, j, k)], bz = Bz[izf(i, j, k)];
+    double dEz_dy = (ez - ez_jm) / dy;
+    double dEy_dz = (ey - ey_km) / dz;
+    double dEz_dx = (ez - ez_im) / dx;
+    double dEx_dz = (ex - ex_km) / dz;
+    double dEy_dx = (ey - ey_im) / dx;
+    double dEx_dy = (ex - ex_jm) / dy;
+    Bx[ixf(i, j, k)] = bx + dt * (dEz_dy - dEy_dz);
+    By[iyf(i, j, k)] = by + dt * (dEx_dz - dEz_dx);
+    Bz[izf(i, j, k)] = bz + dt * (dEy_dx - dEx_dy);
 }
 
 // constrans::f2cB, transport.cpp:60-75: cell B = face average; U.B uses the
@@ -690,8 +711,8 @@ __global__ __launch_bounds__(TPB) void k_f2cb(KP P) {
     double b0 = (Bx[ixf(i, j, k)] + Bx[ixf(i - 1, j, k)]) / 2;
     double b1 = (By[iyf(i, j, k)] + By[iyf(i, j - 1, k)]) / 2;
     double b2 = (Bz[izf(i, j, k)] + Bz[izf(i, j, k - 1)]) / 2;
+    const double sg = cell_sqrtdetg(P, b, c);  // loaded before the stores
     Wa(P, b, WB0, c) = b0; Wa(P, b, WB1, c) = b1; Wa(P, b, WB2, c) = b2;
-    double sg = cell_sqrtdetg(P, b, c);
     Ua(P, b, UB0, c) = sg * b0; Ua(P, b, UB1, c) = sg * b1; Ua(P, b, UB2, c) = sg * b2;
     cst_at(P, b, c) = 0;
 }
@@ -776,23 +797,31 @@ __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
     Prim W = loadW(P, b, c);
     double src[4];
     source_terms(P, b, c, a, W, src, nf);
-    // U component v in {D, S0, S1, S2, tau}; source slot: tau <- src0, S_d <- src_{d+1}
+    // U component v in {D, S0, S1, S2, tau}; source slot: tau <- src0, S_d <- src_{d+1}.
+    // Every load precedes the first store (U, Us and F are all double arrays).
+    double fxr[5], fxl[5], fyr[5], fyl[5], fzr[5], fzl[5], u[5], us[5];
+    double* usp = P.Us + (size_t)b * 5 * NINT + a;
+#pragma unroll
     for (int v = 0; v < 5; v++) {
         const size_t o = (size_t)v * NFC;
-        double div = -((Fx[o + xR] - Fx[o + xL]) / dx + (Fy[o + yR] - Fy[o + yL]) / dy +
-                       (Fz[o + zR] - Fz[o + zL]) / dz) / sg;
+        fxr[v] = Fx[o + xR]; fxl[v] = Fx[o + xL];
+        fyr[v] = Fy[o + yR]; fyl[v] = Fy[o + yL];
+        fzr[v] = Fz[o + zR]; fzl[v] = Fz[o + zL];
+        u[v] = Ua(P, b, v, c);
+        us[v] = stage == 0 ? 0.0 : usp[(size_t)v * NINT];
+    }
+    double nu[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        double div = -((fxr[v] - fxl[v]) / dx + (fyr[v] - fyl[v]) / dy + (fzr[v] - fzl[v]) / dz) / sg;
         double s = v == UD ? 0.0 : (v == UTAU ? src[0] : src[v]);
         double dU = div + s;
-        double u = Ua(P, b, v, c);
-        double* us = P.Us + ((size_t)b * 5 + v) * NINT + a;
-        double nu;
-        if (stage == 0) {
-            *us = u;
-            nu = u + dU * dt;
-        } else {
-            nu = ((*us + u) + dU * dt) * 0.5;
-        }
-        Ua(P, b, v, c) = nu;
+        nu[v] = stage == 0 ? u[v] + dU * dt : ((us[v] + u[v]) + dU * dt) * 0.5;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++) {
+        if (stage == 0) usp[(size_t)v * NINT] = u[v];
+        Ua(P, b, v, c) = nu[v];
     }
     for (int d = 0; d < 3; d++) Ua(P, b, UB0 + d, c) = sg * W.B[d];
     cst_at(P, b, c) = 0;
