@@ -33,6 +33,10 @@ sys.path.insert(0, str(ROOT / "tests"))
 METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
     "zone-updates/s"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
+# compulsory FP64 state traffic of one zone-update (8 evolved words read and
+# written per stage plus the stored hydro words): 37 words
+BYTES_PER_ZONE_UPDATE = 296
 
 DOM = 50.0
 NQLT = 8
@@ -77,6 +81,23 @@ KERNEL_SCOPE = {"k_flux": "all", "k_emf": "all", "k_bfupd": "all", "k_f2cb": "al
 # HBM bytes per dispatch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE,
 # each scaled by what the counter reads for a copy of known size)
 TRAFFIC_TABLE = ROOT / "profiles" / "traffic_main_torus.md"
+
+
+FP64_TABLE = ROOT / "profiles" / "fp64_main_torus.md"
+
+
+def measured_step_flops():
+    """Executed FP64 flops of one step from the committed counter summary."""
+    if not FP64_TABLE.exists():
+        return None
+    for line in FP64_TABLE.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) == 7 and cols[0] == "**step**":
+            try:
+                return float(cols[2])
+            except ValueError:
+                return None
+    return None
 
 
 def measured_traffic(kernel):
@@ -131,6 +152,21 @@ def cpu_baseline():
         "sample": f"1 step of the oracle on the central [-25M,25M]^3 of the same grid (4^3 roots + 1 regrid, "
                   f"{meta.shape[0]} patches, {leaf_zones} leaf zones), {el:.2f} s; setup {t1 - t0:.1f} s untimed",
     }
+
+
+def step_roofline(value, ms_per_step, bench_grid):
+    """Whole-step bounds: compulsory state bytes per zone-update against HBM,
+    and the executed FP64 flops of one step (counter summary, bench grid only)
+    against the FP64 vector peak."""
+    gbs = value * BYTES_PER_ZONE_UPDATE / 1e9
+    out = {"bytes_per_zone_update": BYTES_PER_ZONE_UPDATE, "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
+    flops = measured_step_flops() if bench_grid else None
+    if flops:
+        tfs = flops / (ms_per_step * 1e-3) / 1e12
+        out.update({"fp64_flops_per_step": flops, "fp64_TFLOPs": tfs, "fp64_peak": FP64_PEAK_TFS,
+                    "fp64_frac": tfs / FP64_PEAK_TFS,
+                    "fp64_source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes (profiles/fp64_main_torus.md)"})
+    return out
 
 
 def main():
@@ -260,6 +296,7 @@ def main():
             "algo_bytes_per_launch": algo,
             "avg_launch_ms": avg_ms,
         },
+        "step_roofline": step_roofline(value, ms_per_step, args.nqlt == NQLT),
         "cpu_baseline": cpu,
     }
     if copy is not None:
