@@ -111,6 +111,7 @@ def lib():
         _sig(L, "bh_ctop_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _dp, _dp, _ip])
         _sig(L, "bh_hlld_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
+        _sig(L, "bh_c2p_sign_batch", _i, [_d, _i, _dp, _ip, _dp, _i])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
         _lib = L

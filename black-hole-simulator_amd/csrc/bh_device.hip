@@ -985,6 +985,15 @@ __global__ void k_hlld_batch(int n, double gm, int dim, const double* g, const d
 __global__ __launch_bounds__(TPB) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, size_t n) {
     for (size_t i = (size_t)blockIdx.x * TPB + threadIdx.x; i < n; i += (size_t)gridDim.x * TPB) b[i] = a[i];
 }
+// Proven sign of the con2prim residual next to the residual itself:
+// in[i] = (xi, Ssq, Bsq, SB, D, tau), sign[i] = c2p_f_sign, f[i] = c2p_f.
+__global__ void k_c2p_sign_batch(int n, double gm, const double* in, int* sign, double* f) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const double* a = in + 6 * (size_t)t;
+    sign[t] = c2p_f_sign(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
+    f[t] = c2p_f(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
+}
 __global__ void k_math_batch(int op, int n, const double* x, double* y) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -1678,6 +1687,30 @@ int bh_math_batch(int op, int n, const double* x, double* y, int on_device) {
         if (n) hipLaunchKernelGGL(k_math_batch, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        return BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_c2p_sign_batch(double gm, int n, const double* in, int* sign, double* f, int on_device) {
+    if (n < 0 || (n && (!in || !sign || !f))) return BH_EINVAL;
+    if (!on_device) {
+        for (int i = 0; i < n; i++) {
+            const double* a = in + 6 * (size_t)i;
+            sign[i] = c2p_f_sign(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
+            f[i] = c2p_f(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
+        }
+        return BH_OK;
+    }
+    try {
+        Scratch s;
+        double* din = s.put(in, (size_t)6 * n);
+        int* ds = s.put<int>(nullptr, n);
+        double* df = s.put<double>(nullptr, n);
+        if (n) hipLaunchKernelGGL(k_c2p_sign_batch, dim3((n + 255) / 256), dim3(256), 0, 0, n, gm, din, ds, df);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(sign, ds, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(f, df, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
         return BH_OK;
     } catch (const DevError& e) {
         return fail(nullptr, BH_EDEVICE, e.msg);

@@ -150,6 +150,89 @@ BH_HD double c2p_f(double gm, double xi, double Ssq, double Bsq, double SB, doub
     return xi - taun - Dn - pstar;
 }
 
+// Reciprocal and square root with relative error below 2^-50 (device: the
+// hardware estimate refined by two Newton steps; host: the correctly rounded
+// operations).  Only used where an error bound covers them.
+BH_HD double approx_rcp(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / x;
+#endif
+}
+BH_HD double approx_sqrt(double x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r = __builtin_amdgcn_rsq(x);
+    double g = x * r, h = 0.5 * r;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    return fma(g, e, g);
+#else
+    return sqrt(x);
+#endif
+}
+
+// Sign of c2p_f(gm, xi, ...) without its five divisions and square root:
+// +1 when c2p_f returns a value > 0 (its 1e30 and 1e32 exits included), -1
+// when it returns a value < 0, 0 when the cheap evaluation cannot prove
+// which (the caller then evaluates c2p_f).
+//
+// The numerator and denominator of v^2 are formed with c2p_f's own
+// operations (same bits).  From there the quantities carry approximation
+// error: v^2 within 2^-49.8 relative, w = 1 - v^2 within 2^-49.7 (1 + kappa),
+// kappa = v^2 / w (cancellation in w), and every later term within a few
+// 2^-53 more; c2p_f's own roundings are of the same order.  Each decision
+// below requires a margin of 2^-44 (1 + kappa) times the magnitudes that
+// enter it (32x the first-order bound) plus 2^-1000 (so that a sign cannot
+// be lost to underflow); kappa is capped at 2^30 so that first order holds.
+BH_HD int c2p_f_sign(double gm, double xi, double Ssq, double Bsq, double SB, double Dn, double taun) {
+    if (!(Dn >= 0.0)) return 0;  // rho = Dn / ltz < 0, or NaN: exact path
+    double Yi = SB * SB;
+    double K = Bsq;
+    double X = xi * xi;
+    double xipK = xi + K;
+    const double num = Ssq * X + Yi * (2.0 * xi + K);
+    const double den = X + xipK * xipK;
+    if (!(den >= 0x1p-900 && den <= 0x1p+900 && num >= 0.0 && num <= 0x1p+900)) return 0;
+    const double vsq = num * approx_rcp(den);
+    if (!(vsq < 1.0 - 0x1p-46)) return vsq >= 1.0 + 0x1p-46 ? 1 : 0;  // c2p_f returns 1e30 when vsq >= 1
+    const double w = 1.0 - vsq;                                       // 1 / ltz2
+    const double kappa = vsq * approx_rcp(w);
+    if (!(kappa <= 0x1p+30)) return 0;
+    const double tol = 0x1p-44 * (1.0 + kappa);
+    const double t1 = xi * w;                // xi / ltz2
+    const double t2 = Dn * approx_sqrt(w);   // Dn / ltz
+    const double d = t1 - t2;                // sign of p
+    const double ep = tol * (fabs(t1) + fabs(t2)) + 0x1p-1000;
+    if (d < -ep) return 1;                   // p < 0: c2p_f returns 1e32
+    if (!(d > ep)) return 0;
+    const double c = (gm - 1.0) / gm;
+    const double pr = c * d;
+    const double t3 = K * w * 0.5;           // K / (2 ltz2)
+    if (!(X >= 0x1p-900 && X <= 0x1p+900)) return 0;
+    const double t4 = Yi * approx_rcp(2.0 * X);  // Yi / (2 X)
+    const double pstar = pr + t3 + t4;
+    const double f = xi - taun - Dn - pstar;
+    const double e = tol * (fabs(c) * (fabs(t1) + fabs(t2)) + fabs(t3)) +
+                     0x1p-44 * (fabs(t4) + fabs(pr) + fabs(t3) + fabs(pstar) + fabs(f)) + 0x1p-1000;
+    if (!(fabs(f) <= 0x1p+1000 && e <= 0x1p+1000)) return 0;
+    if (f > e) return 1;
+    if (f < -e) return -1;
+    return 0;
+}
+// c2p_f where only its sign is used: the sign as +-1 when c2p_f_sign proves
+// it, else c2p_f itself.
+BH_HD double c2p_f_signed(double gm, double xi, double Ssq, double Bsq, double SB, double Dn, double taun) {
+    const int sg = c2p_f_sign(gm, xi, Ssq, Bsq, SB, Dn, taun);
+    return sg ? (double)sg : c2p_f(gm, xi, Ssq, Bsq, SB, Dn, taun);
+}
+
 BH_HD void dswap(double& a, double& b) { double t = a; a = b; b = t; }
 
 // Brent-Dekker fallback, conserved.cpp:232-285 (quirk Q13: bisection test
@@ -184,7 +267,9 @@ BH_HD bool c2p_brent(double gm, double xa, double xb, double Ssq, double Bsq, do
         }
         if (bisect) { s = (xa + xb) / 2; mflag = true; }
         else mflag = false;
-        double fs = c2p_f(gm, s, Ssq, Bsq, SB, D, tau);
+        // fs only enters through the sign of fa * fs (fa and fb are never
+        // refreshed, quirk Q13)
+        double fs = c2p_f_signed(gm, s, Ssq, Bsq, SB, D, tau);
         d = xc; xc = xb; fc = fb;
         if (fa * fs < 0.0) xb = s;
         else xa = s;
@@ -235,7 +320,7 @@ BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h) {
     double fh = c2p_f(gm, xi_h, Ssq, Bsq, SB, D, tau);
     while (fl * fh > 0.0 && xi_h < 1e8 * D) {
         xi_h *= 2.0;
-        fl = c2p_f(gm, xi_h, Ssq, Bsq, SB, D, tau);
+        fl = c2p_f_signed(gm, xi_h, Ssq, Bsq, SB, D, tau);  // fl only enters through fl * fh > 0
     }
     xi = (xi_l + xi_h) / 2.0;
     // The iterate is a pure function of the previous one, so once it is NaN,

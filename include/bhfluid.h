@@ -106,6 +106,11 @@ int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* 
                   double* ctspeed);
 /* op 0: log, 1: exp, correctly rounded; on_device 0 evaluates the same code on the host. */
 int bh_math_batch(int op, int n, const double* x, double* y, int on_device);
+/* The con2prim energy residual (conserved::fres, src/fluid/physics/conserved.cpp:80-123)
+ * at in[i] = (xi, Ssq, Bsq, SB, D, tau): f[i] = the residual, sign[i] = its sign as
+ * proved by the division-free evaluation the Brent and bracket loops use (+1, -1, or 0
+ * when unproved).  For parity tests of that evaluation; no reference counterpart. */
+int bh_c2p_sign_batch(double gm, int n, const double* in, int* sign, double* f, int on_device);
 
 /* Measurement helper without a reference counterpart: times a plain device
  * copy of nbytes (read nbytes, write nbytes), the calibration kernel for the

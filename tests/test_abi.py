@@ -88,3 +88,16 @@ def test_fm_init_without_torus_returns_false():
     # error and returns (initconds.cpp:483-486)
     t = bhfluid.AmrTree((-5, -5, 30), (5, 5, 40), 1, 1.0, 0.9, 0.1, configs.GAMMA)
     assert bhfluid.init.fm_init(t) is False
+
+
+def test_residual_sign_proof_on_host():
+    """Host build of the division-free residual sign (exact reciprocal and
+    square root there): every proved sign agrees with the residual, and most
+    points away from the root get one."""
+    import c2p_sign_cases as cs
+    L = bhfluid.lib()
+    pts, near = cs.points(L, np.random.default_rng(5), n_sets=600)
+    sign, f = cs.residual(L, pts, 0)
+    assert cs.check(sign, f) == 0
+    far = ~near & np.isfinite(f)
+    assert (sign[far] != 0).mean() > 0.9

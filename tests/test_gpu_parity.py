@@ -143,3 +143,17 @@ def test_step_R2_regridded_bitwise():
 
 def test_step_R3_two_levels_bitwise():
     _run_parity(configs.R3, 2, "R3")
+
+
+def test_residual_sign_proof_on_device():
+    """The division-free sign of the con2prim residual that the Brent and
+    bracket loops use agrees with the residual wherever it claims a sign, and
+    claims one away from the root."""
+    import c2p_sign_cases as cs
+    L = bhfluid.lib()
+    pts, near = cs.points(L, np.random.default_rng(11))
+    sign, f = cs.residual(L, pts, 1)
+    assert cs.check(sign, f) == 0
+    far = ~near & np.isfinite(f)
+    assert (sign[far] != 0).mean() > 0.9
+    assert (sign[near] == 0).any()
