@@ -227,6 +227,13 @@ def main():
     tree.profile(False)
     prof = {k: v for k, v in report.items() if not k.startswith("queue:")}
     queued = {k[6:]: v[0] for k, v in report.items() if k.startswith("queue:")}
+    # the analyser's state-only diagnostics on the final state (device
+    # reductions), timed separately from the steps
+    diag = tree.diagnostics()
+    t_d = time.perf_counter()
+    for _ in range(5):
+        diag = tree.diagnostics()
+    diag_ms = 1e3 * (time.perf_counter() - t_d) / 5
     U, _, _ = tree.state()
     leaf = meta[:, 4] == 1
     Ui = U.reshape(nb, 8, 20, 20, 20)[leaf][:, :, 2:18, 2:18, 2:18]
@@ -282,6 +289,8 @@ def main():
             "setup_s": round(setup_s, 1),
             "kernels": kernels,
             "brent_queued_cells": queued,
+            "diagnostics_ms": round(diag_ms, 3),
+            "diagnostics_after": {k: float(v) for k, v in diag.items()},
         },
         "roofline": {
             "bound": "hbm",

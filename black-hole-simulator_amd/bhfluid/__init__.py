@@ -112,6 +112,7 @@ def lib():
         _sig(L, "bh_hlld_batch", _i, [_d, _d, _d, _d, _i, _dp, _dp, _i, _dp, _dp, _dp, _dp, _dp, _dp])
         _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
         _sig(L, "bh_c2p_sign_batch", _i, [_d, _i, _dp, _ip, _dp, _i])
+        _sig(L, "bh_tree_diagnostics", _i, [_v, _dp])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
         _lib = L
@@ -272,6 +273,17 @@ class AmrTree:
             name, n, ms = line.split()
             out[name] = (int(n), float(ms))
         return out
+
+    DIAGNOSTICS = ("mass", "energy", "L_pol", "Edot", "Phi_B", "thermE", "Hscal")
+
+    def diagnostics(self) -> dict:
+        """The ``analysis::analyser`` quantities that read only the evolved
+        state (src/fluid/data/quantitatives.cpp), reduced on the GPU over the
+        leaf active zones: mass, energy, L_pol, Edot and Phi_B at the
+        horizon, thermE, Hscal at 10 M."""
+        out = np.zeros(7)
+        self._check(self._L.bh_tree_diagnostics(self._h, _p(out)), "diagnostics")
+        return dict(zip(self.DIAGNOSTICS, out.tolist()))
 
     def maxdivB(self, b: int) -> float:
         """``constrans::maxdivB`` of block b (grid/transport.cpp:77-89)."""

@@ -84,6 +84,7 @@ struct KP {
     const double* gfc;   // [nb][3][8][4352]
     const double* gchr;  // [nb][40][4096]
     const double* dxyz;  // [nb][3]
+    const double* org;   // [nb][3] first x, y, z cell edge of each block
     const int* ghtab;    // [nb][12]
     const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
     // [3904] ghost cells in storage order (the fill copies whole rows and
@@ -911,6 +912,72 @@ __global__ __launch_bounds__(TPB) void k_floors(KP P) {
     add_faults(P, nf);
 }
 
+// ---------------------------------------------------------- diagnostics
+// analysis::analyser quantities that read only the evolved state
+// (src/fluid/data/quantitatives.cpp): per active zone of every leaf, the
+// terms of mass (:82-86), energy (:87-91), L_pol (:92-96), thermE
+// (:129-133) and, on the shells |r - r_s| <= |(dx, dy, dz)| of surf_int
+// (:61-79), Edot (:103-107) and Phi_B (:114-118) at the horizon and the
+// Hscal numerator and denominator (:199-208) at rs; each term is formed with
+// the reference's operations (cell centre, r and volume as patch::cell_init,
+// grid.cpp:149-175; sqrt(-g), alpha, beta of the cell metric).  Sums run per
+// workgroup, then over workgroups, in a fixed order.
+constexpr int NDIAG = 8;
+__device__ __forceinline__ void block_sum(double v[NDIAG], double* lds) {
+    // fixed-order tree over the workgroup: lds[NDIAG][TPB]
+    for (int q = 0; q < NDIAG; q++) lds[q * TPB + threadIdx.x] = v[q];
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h)
+            for (int q = 0; q < NDIAG; q++) lds[q * TPB + threadIdx.x] += lds[q * TPB + threadIdx.x + h];
+        __syncthreads();
+    }
+}
+__global__ __launch_bounds__(TPB) void k_diag(KP P, const int* __restrict__ leaves, double hrz, double rs,
+                                              double* __restrict__ partial) {
+    __shared__ double lds[NDIAG * TPB];
+    const int b = leaves[blockIdx.x / WG_INT];
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int c = lindx(i, j, k);
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const double xc = P.org[3 * b] + (i + 0.5) * dx, yc = P.org[3 * b + 1] + (j + 0.5) * dy,
+                 zc = P.org[3 * b + 2] + (k + 0.5) * dz;
+    const double vol = dx * dy * dz;
+    double r = sqrt(xc * xc + yc * yc + zc * zc);
+    if (r < 1e-10) r = 1e-10;
+    const double shell = sqrt(dx * dx + dy * dy + dz * dz);
+    const double sg = gcc_at(P, b, 0, c), alpha = gcc_at(P, b, 1, c), beta0 = gcc_at(P, b, 2, c);
+    const double D = Ua(P, b, UD, c), S0 = Ua(P, b, US0, c), S1 = Ua(P, b, US1, c), tau = Ua(P, b, UTAU, c);
+    const Prim w = loadW(P, b, c);
+    const double vr = (xc * w.v[0] + yc * w.v[1] + zc * w.v[2]) / r;
+    const double Br = (xc * w.B[0] + yc * w.B[1] + zc * w.B[2]) / r;
+    const bool on_hrz = !(fabs(r - hrz) > shell), on_rs = !(fabs(r - rs) > shell);
+    double v[NDIAG];
+    v[0] = D * vol;
+    v[1] = (tau + D) * vol;
+    v[2] = (xc * S1 - yc * S0) * vol;
+    // beta^1 = beta^2 = +0.0 in the cell metric
+    v[3] = on_hrz ? -((tau + D) * (alpha * vr - (beta0 * xc + 0.0 * yc + 0.0 * zc) / r)) * vol : 0.0;
+    v[4] = on_hrz ? fabs(Br) * sg * vol : 0.0;
+    v[5] = w.rho * w.eps * sg * vol;
+    v[6] = on_rs ? fabs(zc) * w.rho * sg * vol : 0.0;
+    v[7] = on_rs ? w.rho * sg * vol : 0.0;
+    block_sum(v, lds);
+    if (threadIdx.x < NDIAG) partial[(size_t)blockIdx.x * NDIAG + threadIdx.x] = lds[threadIdx.x * TPB];
+}
+// one workgroup: sum the per-workgroup partials in a fixed order
+__global__ __launch_bounds__(TPB) void k_diag_reduce(const double* __restrict__ partial, int n,
+                                                     double* __restrict__ out) {
+    __shared__ double lds[NDIAG * TPB];
+    double v[NDIAG];
+    for (int q = 0; q < NDIAG; q++) v[q] = 0.0;
+    for (int p = threadIdx.x; p < n; p += TPB)
+        for (int q = 0; q < NDIAG; q++) v[q] += partial[(size_t)p * NDIAG + q];
+    block_sum(v, lds);
+    if (threadIdx.x < NDIAG) out[threadIdx.x] = lds[threadIdx.x * TPB];
+}
+
 // ---------------------------------------------------- pointwise batches
 __global__ void k_ptoc_batch(int n, double gm, const double* g, const double* W, double* U,
                              unsigned long long* faults) {
@@ -1031,7 +1098,8 @@ public:
     int limiter = LIM_VANLEER;
     hipStream_t stream = nullptr;
     GhostPlan plan;
-    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz;
+    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz, org;
+    DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
     DBuf<int> ghtab, leaves, waveblk;
     DBuf<unsigned char> ghsame;
     std::vector<int> wave_off;  // offsets into waveblk
@@ -1102,7 +1170,7 @@ public:
         KP P;
         P.gamma = gamma; P.limiter = limiter;
         P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p; P.E = E.p;
-        P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p;
+        P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
         P.ghtab = ghtab.p; P.ghsame = ghsame.p;
         P.ghost_order = cell_order.p; P.cell_order = cell_order.p + NGHOST;
         P.dtbits = scal.p; P.faults = scal.p + 1;
@@ -1136,6 +1204,15 @@ public:
         gfc.alloc(g.face.size()); gfc.upload(g.face.data(), g.face.size(), stream);
         gchr.alloc(g.chr.size()); gchr.upload(g.chr.data(), g.chr.size(), stream);
         dxyz.alloc(g.dxyz.size()); dxyz.upload(g.dxyz.data(), g.dxyz.size(), stream);
+        {
+            std::vector<double> o((size_t)nb * 3);
+            for (int b = 0; b < nb; b++) {
+                const Patch& p = *t.quilt[b];
+                o[3 * b] = p.xe[0]; o[3 * b + 1] = p.ye[0]; o[3 * b + 2] = p.ze[0];
+            }
+            org.alloc(o.size()); org.upload(o.data(), o.size(), stream);
+        }
+        diagpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT) * NDIAG + NDIAG);
         ghtab.alloc(plan.tab.size()); ghtab.upload(plan.tab.data(), plan.tab.size(), stream);
         ghsame.alloc(plan.same.size()); ghsame.upload(plan.same.data(), plan.same.size(), stream);
         leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
@@ -1257,6 +1334,25 @@ public:
             launch_step(cfl);
             HIPCHK(hipMemcpyAsync(dts.p + s, scal.p, sizeof(double), hipMemcpyDeviceToDevice, stream));
         }
+    }
+    // mass, energy, L_pol, Edot, Phi_B, thermE, Hscal (see k_diag)
+    void diagnostics(double M, double a, double Q, double out[7]) {
+        const double hrz = M + std::sqrt(M * M - a * a - Q * Q);
+        const double rs = 10.0 * M;
+        KP P = kp();
+        const int nl = (int)plan.leaves.size();
+        double s[NDIAG] = {0, 0, 0, 0, 0, 0, 0, 0};
+        if (nl) {
+            double* fin = diagpart.p + (size_t)nl * WG_INT * NDIAG;
+            launch("k_diag", k_diag, dim3(nl * WG_INT), P, (const int*)leaves.p, hrz, rs, diagpart.p);
+            launch("k_diag_reduce", k_diag_reduce, dim3(1), (const double*)diagpart.p, nl * WG_INT, fin);
+            HIPCHK(hipMemcpyAsync(s, fin, sizeof s, hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
+        out[4] = 0.5 * s[4];
+        out[5] = s[5];
+        out[6] = (s[7] > 1e-20) ? (s[6] / s[7]) / rs : 0.0;
     }
     double read_dt() {
         unsigned long long bits = 0;
@@ -1576,6 +1672,15 @@ int bh_profile_report(bh_tree* t, char* buf, int len) {
     });
 }
 int bh_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? BH_OK : BH_EDEVICE; }
+int bh_tree_diagnostics(bh_tree* t, double* out) {
+    if (!out) return fail(t, BH_EINVAL, "null output");
+    return guarded(t, [&] {
+        need_device(t);
+        const TreeParams& m = t->host->prm;
+        t->dev->diagnostics(m.M, m.a, m.Q, out);
+        return BH_OK;
+    });
+}
 int bh_tree_take_faults(bh_tree* t, long long* n) {
     if (!t || !n) return BH_EINVAL;
     *n = t->faults;

@@ -89,6 +89,14 @@ int bh_profile_report(bh_tree* t, char* buf, int len);
 int bh_device_sync(void);
 /* Equation-of-state domain faults (rho <= 0) since the last call. */
 int bh_tree_take_faults(bh_tree* t, long long* n);
+/* The analysis::analyser quantities that read only the evolved state,
+ * src/fluid/data/quantitatives.cpp, as device reductions over the leaf active
+ * zones: out[7] = mass (:82-86), energy (:87-91), L_pol (:92-96), Edot and
+ * Phi_B at the horizon (:103-107, :114-118), thermE (:129-133), Hscal at
+ * 10 M (:199-208).  Each zone's term is the reference's; the sums run in a
+ * fixed order of their own (results agree with the reference's sequential
+ * sums to rounding and are identical from run to run). */
+int bh_tree_diagnostics(bh_tree* t, double* out);
 /* constrans::maxdivB of block b, src/fluid/grid/transport.cpp:77-89 */
 int bh_tree_maxdivB(bh_tree* t, int b, double* out);
 

@@ -1290,6 +1290,64 @@ long long or_update_terms(void* h, int b, double* dF, double* S) {
     return take_faults(*t);
 }
 double or_maxdivB(void* h, int b) { return maxdivB(*static_cast<Tree*>(h), b); }
+
+// analysis::analyser quantities that depend only on the evolved state,
+// src/fluid/data/quantitatives.cpp: vol_int / surf_int (:46-79) over leaf
+// active zones in quilt order, summed sequentially like the reference, of
+//   mass (:82-86), energy (:87-91), L_pol (:92-96), Edot (:103-107),
+//   Phi_B (:114-118), thermE (:129-133), Hscal (:199-208, rs = 10 M)
+// with r_horizon = M + sqrt(M^2 - a^2 - Q^2) (:18-21), val_vr / val_Br
+// (:23-36) and sg = sqrt(-g) of the cell metric (a = 0, quirk Q2).
+// abs[q] = the sum of |term| for q's integral (the conditioning of the sum).
+void or_diagnostics(void* h, double* out, double* abs) {
+    Tree& t = *static_cast<Tree*>(h);
+    const double M = t.ks.M, a = t.ks.a, Q = t.ks.Q;
+    const double hrz = M + std::sqrt(M * M - a * a - Q * Q);
+    const double rs = 10.0 * M;
+    double s[8] = {0, 0, 0, 0, 0, 0, 0, 0}, sa[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int b = 0; b < t.nb; b++) {
+        if (!t.blk[b].leaf) continue;
+        const double dx = t.blk[b].dx(), dy = t.blk[b].dy(), dz = t.blk[b].dz();
+        const double shell = std::sqrt(dx * dx + dy * dy + dz * dz);
+        for (int i = 0; i < BLK; i++)
+            for (int j = 0; j < BLK; j++)
+                for (int k = 0; k < BLK; k++) {
+                    const int c = lindx(i, j, k);
+                    const Geo& g = t.gc(b, c);
+                    const MC mc = metric_comp(t.cell_ks, g.r, g.th);
+                    const double sg = mc.sqrtdetg;
+                    const double D = t.u(b, 0, c), S0 = t.u(b, 1, c), S1 = t.u(b, 2, c), tau = t.u(b, 4, c);
+                    const double rho = t.w(b, 0, c), eps = t.w(b, 1, c);
+                    const double vr = (g.xc * t.w(b, 2, c) + g.yc * t.w(b, 3, c) + g.zc * t.w(b, 4, c)) / g.r;
+                    const double Br = (g.xc * t.w(b, 5, c) + g.yc * t.w(b, 6, c) + g.zc * t.w(b, 7, c)) / g.r;
+                    double term[8];
+                    term[0] = D * g.vol;
+                    term[1] = (tau + D) * g.vol;
+                    term[2] = (g.xc * S1 - g.yc * S0) * g.vol;
+                    term[5] = rho * eps * sg * g.vol;
+                    const bool on_hrz = !(std::abs(g.r - hrz) > shell);
+                    const bool on_rs = !(std::abs(g.r - rs) > shell);
+                    term[3] = on_hrz ? -((tau + D) * (mc.alpha * vr - (mc.beta[0] * g.xc + mc.beta[1] * g.yc +
+                                                                       mc.beta[2] * g.zc) / g.r)) * g.vol
+                                     : 0.0;
+                    term[4] = on_hrz ? std::abs(Br) * sg * g.vol : 0.0;
+                    term[6] = on_rs ? std::abs(g.zc) * rho * sg * g.vol : 0.0;
+                    term[7] = on_rs ? rho * sg * g.vol : 0.0;
+                    const bool in[8] = {true, true, true, on_hrz, on_hrz, true, on_rs, on_rs};
+                    for (int q = 0; q < 8; q++)
+                        if (in[q]) { s[q] += term[q]; sa[q] += std::abs(term[q]); }
+                }
+    }
+    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
+    out[4] = 0.5 * s[4];
+    out[5] = s[5];
+    out[6] = (s[7] > 1e-20) ? (s[6] / s[7]) / rs : 0.0;
+    for (int q = 0; q < 6; q++) abs[q] = sa[q];
+    abs[4] = 0.5 * sa[4];
+    abs[6] = sa[6];
+    abs[7] = sa[7];
+    abs[5] = sa[5];
+}
 void or_cell_geom(void* h, int b, double* out) {
     Tree* t = static_cast<Tree*>(h);
     for (int c = 0; c < NCELL; c++) {

@@ -39,6 +39,9 @@ long long or_flux_ct_patch(void* t, int b, double dt, double* F, double* E);
 long long or_update_terms(void* t, int b, double* dF, double* S);
 double or_maxdivB(void* t, int b);
 void or_cell_geom(void* t, int b, double* out);
+/* out[7] = mass, energy, L_pol, Edot, Phi_B, thermE, Hscal (quantitatives.cpp);
+ * abs[8] = sum of |term| of each integral (Hscal: numerator [6], denominator [7]). */
+void or_diagnostics(void* t, double* out, double* abs);
 void or_set_threads(int n);
 
 /* pointwise */

@@ -15,6 +15,7 @@
 //   rk2integrator::{step,dtcomp,rkstg,div_flux,source}  src/fluid/solver/integrator.cpp
 //   amrtree::{ghosts,regrid}, patch::{fluxcomp,floors}  src/fluid/grid/grid.cpp
 //   constrans::*                  src/fluid/grid/transport.cpp
+//   analysis::analyser            src/fluid/data/quantitatives.cpp
 // The state exchange layout (SoA per block) is the one the product uses:
 //   U[b][8][20^3]   (D, S0, S1, S2, tau, B0, B1, B2)
 //   W[b][8][20^3]   (rho, eps, v0, v1, v2, B0, B1, B2)
@@ -36,6 +37,7 @@
 #include "initial.hpp"
 #include "recon.hpp"
 #include "hlld.hpp"
+#include "analysis.hpp"
 
 // Zero-filled heap for the reference's objects.  grid::patch leaves its
 // refine/erase flags uninitialised (src/fluid/grid.hpp:98-99, constructor at
@@ -373,4 +375,22 @@ void ref_reconfp(void* h, int b, int i, int j, int k, int dim, int lim, double* 
 }
 // patch::B_init on patch b
 void ref_B_init(void* h, int b) { static_cast<RefTree*>(h)->tree->quilt[b]->B_init(); }
+
+// analysis::analyser quantities that read only the evolved state
+// (src/fluid/data/quantitatives.cpp): mass, energy, L_pol, Edot, Phi_B,
+// thermE, Hscal at the horizon r_horizon (:18-21) and rs = 10 M as batch_
+// (:220-247) passes them.
+void ref_diagnostics(void* h, double* out) {
+    grid::amrtree& t = *static_cast<RefTree*>(h)->tree;
+    analysis::analyser an;
+    const double M = t.mtr.M, a = t.mtr.a, Q = t.mtr.Q;
+    const double hrz = M + std::sqrt(M * M - a * a - Q * Q);
+    out[0] = an.mass(t);
+    out[1] = an.energy(t);
+    out[2] = an.L_pol(t);
+    out[3] = an.Edot(t, hrz);
+    out[4] = an.Phi_B(t, hrz);
+    out[5] = an.thermE(t);
+    out[6] = an.Hscal(t, 10.0 * M);
+}
 }

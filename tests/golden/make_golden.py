@@ -10,7 +10,10 @@ the reference sources); the GPU box never runs it.  Writes:
   state (block table and U, W, Bf after src/fluid/main.cpp:94-168) and of the
   state after every phase of every step (dtcomp, rkstg 0, rkstg 1, floors),
   with dt, the reference's exception if one was thrown, and non-finite
-  counts.  NaNs are canonicalised before hashing.
+  counts, and the reference analyser's state-only diagnostics (mass,
+  energy, L_pol, Edot, Phi_B, thermE, Hscal; exact values as float.hex)
+  initially and after every completed step.  NaNs are canonicalised before
+  hashing.
 
 Usage: python tests/golden/make_golden.py
 """
@@ -119,7 +122,8 @@ def tree_record(name, cfg, nsteps):
     rec = {"config": {"dom_l": lo, "dom_h": hi, "nqlt": nqlt, "regrid": regrid, "a": a, "Q": q,
                       "gamma": configs.GAMMA, "cfl": configs.CFL},
            "nblocks": s.nb, "leaves": int(s.meta[:, 4].sum()),
-           "blocks": digest(s.meta, s.edges), "initial": state_digest(s), "steps": []}
+           "blocks": digest(s.meta, s.edges), "initial": state_digest(s),
+           "diag_initial": [float(x).hex() for x in t.diagnostics()], "steps": []}
     for step in range(nsteps):
         st = {}
         dt, rc = t.dtcomp(configs.CFL)
@@ -136,6 +140,7 @@ def tree_record(name, cfg, nsteps):
         if "threw" not in st:
             U = s.U.reshape(s.nb, 8, 20, 20, 20)[:, :, 2:18, 2:18, 2:18]
             st["nonfinite_active"] = int((~np.isfinite(U)).any(axis=1).sum())
+            st["diag"] = [float(x).hex() for x in t.diagnostics()]
         rec["steps"].append(st)
         print(name, step, {k: (v[:12] if isinstance(v, str) else v) for k, v in st.items()}, flush=True)
         if "threw" in st:

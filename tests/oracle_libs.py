@@ -74,6 +74,7 @@ def oracle_lib():
         _sig(L, "or_update_terms", _ll, [_v, _i, _dp, _dp])
         _sig(L, "or_maxdivB", _d, [_v, _i])
         _sig(L, "or_cell_geom", None, [_v, _i, _dp])
+        _sig(L, "or_diagnostics", None, [_v, _dp, _dp])
         _sig(L, "or_set_threads", None, [_i])
         _sig(L, "or_metric", None, [_d, _d, _d, _d, _d, _dp])
         _sig(L, "or_ptoc", _i, [_d, _d, _d, _d, _dp, _d, _d, _dp, _dp])
@@ -118,6 +119,7 @@ def ref_lib():
         _sig(L, "ref_update_terms", _i, [_v, _i, _dp, _dp])
         _sig(L, "ref_maxdivB", _d, [_v, _i])
         _sig(L, "ref_cell_geom", None, [_v, _i, _dp])
+        _sig(L, "ref_diagnostics", None, [_v, _dp])
         _sig(L, "ref_metric", None, [_d, _d, _d, _d, _d, _dp])
         _sig(L, "ref_ptoc", _i, [_d, _d, _d, _d, _dp, _d, _d, _dp, _dp])
         _sig(L, "ref_ctop", _i, [_d, _d, _d, _d, _dp, _d, _d, _dp, _dp])
@@ -222,6 +224,12 @@ class RefTree:
     def ghosts_all(self):
         return self.L.ref_ghosts_all(self.h)
 
+    def diagnostics(self):
+        """mass, energy, L_pol, Edot, Phi_B, thermE, Hscal (quantitatives.cpp)."""
+        out = np.zeros(7)
+        self.L.ref_diagnostics(self.h, _p(out))
+        return out
+
 
 class OracleTree:
     """The oracle restatement, loaded with a block table and a state."""
@@ -274,6 +282,16 @@ class OracleTree:
 
     def ghosts_all(self):
         return self.L.or_ghosts_all(self.h)
+
+    def diagnostics(self):
+        """(values[7], sum of |terms|[8]) as or_diagnostics."""
+        out = np.zeros(7)
+        ab = np.zeros(8)
+        self.L.or_diagnostics(self.h, _p(out), _p(ab))
+        return out, ab
+
+
+DIAG_NAMES = ("mass", "energy", "L_pol", "Edot", "Phi_B", "thermE", "Hscal")
 
 
 def bits_equal(a, b):

@@ -157,3 +157,33 @@ def test_residual_sign_proof_on_device():
     far = ~near & np.isfinite(f)
     assert (sign[far] != 0).mean() > 0.9
     assert (sign[near] == 0).any()
+
+
+@pytest.mark.parametrize("name,nsteps", [("R4", 2), ("R3", 0), ("R1", 3)])
+def test_diagnostics_against_oracle(name, nsteps):
+    """Device reductions of the analyser's state-only quantities against the
+    oracle's sequential sums (which equal the reference's bit for bit).  The
+    terms are the same; only the summation order differs, so each result is
+    held to the bound on recursive summation error, n * 2^-52 * sum |terms|
+    (n terms; the fixed-order tree adds log2 n * 2^-53), NaN where the
+    oracle's is NaN."""
+    t, params = build(getattr(configs, name))
+    for _ in range(nsteps):
+        try:
+            bhfluid.RK2Integrator(configs.CFL).step(t)
+        except bhfluid.DomainError:
+            pass
+    dev = np.array(list(t.diagnostics().values()))
+    ot = OracleTree.from_state(snapshot(t), params, libm=1)
+    ref, ab = ot.diagnostics()
+    meta, _ = t.blocks()
+    n = int(meta[:, 4].sum()) * 4096
+    tol = n * 2.0 ** -52 * np.array([ab[0], ab[1], ab[2], ab[3], ab[4], ab[5], 0.0])
+    for q, nm in enumerate(bhfluid.AmrTree.DIAGNOSTICS):
+        if np.isnan(ref[q]):
+            assert np.isnan(dev[q]), (name, nm)
+        elif nm == "Hscal":
+            # ratio of two sums of non-negative terms: relative bound on each
+            assert abs(dev[q] - ref[q]) <= 4 * n * 2.0 ** -52 * abs(ref[q]) + 1e-300, (name, nm, dev[q], ref[q])
+        else:
+            assert abs(dev[q] - ref[q]) <= tol[q] + 1e-300, (name, nm, dev[q], ref[q], tol[q])

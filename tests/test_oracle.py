@@ -103,16 +103,22 @@ def _product_initial(cfg):
     return State(meta, edges, U, W, Bf), (1.0, a, q, configs.GAMMA)
 
 
+def _golden_diag(hexes):
+    return np.array([float.fromhex(h) for h in hexes])
+
+
 @pytest.mark.parametrize("name", ["R0", "R1", "R4", "R2", "R3"])
 def test_oracle_phases_match_reference_digests(trees, name):
     """Initial data from the product's host code, then every phase of every
-    step through the oracle, against the reference's digests."""
+    step through the oracle, against the reference's digests; the oracle's
+    state-only diagnostics against the reference analyser's, bit for bit."""
     rec = trees[name]
     cfg = getattr(configs, name)
     s0, params = _product_initial(cfg)
     assert digest(s0.meta, s0.edges) == rec["blocks"]
     assert digest(s0.U, s0.W, s0.Bf) == rec["initial"]
     ot = OracleTree.from_state(s0, params, libm=0)
+    assert bits_equal(ot.diagnostics()[0], _golden_diag(rec["diag_initial"])), name
     for n, st in enumerate(rec["steps"]):
         dt, f = ot.dtcomp(configs.CFL)
         assert dt == st["dt"], (name, n, dt, st["dt"])
@@ -126,6 +132,8 @@ def test_oracle_phases_match_reference_digests(trees, name):
             assert faults == 0, (name, n, phase, faults)
             s = ot.state()
             assert digest(s.U, s.W, s.Bf) == st[phase], (name, n, phase)
+        if "diag" in st:
+            assert bits_equal(ot.diagnostics()[0], _golden_diag(st["diag"])), (name, n)
 
 
 # ------------------------------------------------- against the compiled reference
