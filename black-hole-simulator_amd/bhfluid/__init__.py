@@ -113,6 +113,7 @@ def lib():
         _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
         _sig(L, "bh_c2p_sign_batch", _i, [_d, _i, _dp, _ip, _dp, _i])
         _sig(L, "bh_tree_diagnostics", _i, [_v, _dp])
+        _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
         _lib = L
@@ -285,6 +286,11 @@ class AmrTree:
         self._check(self._L.bh_tree_diagnostics(self._h, _p(out)), "diagnostics")
         return dict(zip(self.DIAGNOSTICS, out.tolist()))
 
+    def write_dump(self, path, time: float = 0.0, step: int = 0):
+        """Write W of every leaf's active zones to a dump file (format in
+        include/bhfluid.h; read it back with ``read_dump``)."""
+        self._check(self._L.bh_tree_write_dump(self._h, str(path).encode(), float(time), int(step)), "write_dump")
+
     def maxdivB(self, b: int) -> float:
         """``constrans::maxdivB`` of block b (grid/transport.cpp:77-89)."""
         out = ctypes.c_double(0.0)
@@ -368,6 +374,32 @@ def run_steps(tree: AmrTree, nsteps: int, cfl: float = 0.4) -> np.ndarray:
     dts = np.zeros(max(int(nsteps), 1))
     tree._check(tree._L.bh_rk2_steps(tree._h, float(cfl), int(nsteps), _p(dts)), "steps")
     return dts[:nsteps]
+
+
+DUMP_MAGIC = b"BHFLUIDW"
+
+
+def read_dump(path) -> dict:
+    """Read a dump written by ``AmrTree.write_dump``: header fields, the leaf
+    table (level, ijk, lo, hi) and W as a read-only array [leaf][8][16][16][16]
+    (rho, eps, v0, v1, v2, B0, B1, B2; indices i, j, k)."""
+    raw = np.memmap(path, dtype=np.uint8, mode="r")
+    if bytes(raw[:8]) != DUMP_MAGIC:
+        raise BhError(f"{path}: not a bhfluid dump")
+    version, hbytes = np.frombuffer(raw[8:16], "<u4")
+    if version != 1:
+        raise BhError(f"{path}: dump version {version} not supported")
+    nl, nvar, nz, _ = np.frombuffer(raw[16:32], "<i4")
+    M, a, Q, gm, time = np.frombuffer(raw[32:72], "<f8")
+    step = int(np.frombuffer(raw[72:80], "<i8")[0])
+    tab = raw[hbytes:hbytes + 64 * nl].reshape(nl, 64)
+    ints = np.frombuffer(tab[:, :16].tobytes(), "<i4").reshape(nl, 4)
+    box = np.frombuffer(tab[:, 16:].tobytes(), "<f8").reshape(nl, 6)
+    off = int(hbytes) + 64 * int(nl)
+    W = np.frombuffer(raw, dtype="<f8", count=int(nl) * int(nvar) * int(nz), offset=off)
+    return {"version": int(version), "M": float(M), "a": float(a), "Q": float(Q), "gamma": float(gm),
+            "time": float(time), "step": step, "level": ints[:, 0], "ijk": ints[:, 1:4],
+            "lo": box[:, :3], "hi": box[:, 3:], "W": W.reshape(int(nl), int(nvar), BLOCK, BLOCK, BLOCK)}
 
 
 def device_sync():

@@ -978,6 +978,22 @@ __global__ __launch_bounds__(TPB) void k_diag_reduce(const double* __restrict__ 
     if (threadIdx.x < NDIAG) out[threadIdx.x] = lds[threadIdx.x * TPB];
 }
 
+// ----------------------------------------------------------- state dump
+// W of the active zones of leaves l0 .. l0 + n - 1, packed [n][8][4096]
+// (k fastest) for the dump file.
+__global__ __launch_bounds__(TPB) void k_pack_w(KP P, const int* __restrict__ leaves, int l0,
+                                                double* __restrict__ out) {
+    const int l = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int b = leaves[l0 + l];
+    const int c = lindx(a / 256, (a / 16) % 16, a % 16);
+    double w[8];
+#pragma unroll
+    for (int v = 0; v < 8; v++) w[v] = Wa(P, b, v, c);
+#pragma unroll
+    for (int v = 0; v < 8; v++) out[((size_t)l * 8 + v) * NINT + a] = w[v];
+}
+
 // ---------------------------------------------------- pointwise batches
 __global__ void k_ptoc_batch(int n, double gm, const double* g, const double* W, double* U,
                              unsigned long long* faults) {
@@ -1354,6 +1370,15 @@ public:
         out[5] = s[5];
         out[6] = (s[7] > 1e-20) ? (s[6] / s[7]) / rs : 0.0;
     }
+    // packed W of leaves [l0, l0 + n) into host memory h ([n][8][4096])
+    void pack_w(int l0, int n, double* dbuf, double* h) {
+        if (n <= 0) return;
+        KP P = kp();
+        hipLaunchKernelGGL(k_pack_w, dim3(n * WG_INT), dim3(TPB), 0, stream, P, (const int*)leaves.p, l0, dbuf);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h, dbuf, (size_t)n * 8 * NINT * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+    }
     double read_dt() {
         unsigned long long bits = 0;
         HIPCHK(hipMemcpyAsync(&bits, scal.p, sizeof bits, hipMemcpyDeviceToHost, stream));
@@ -1678,6 +1703,72 @@ int bh_tree_diagnostics(bh_tree* t, double* out) {
         need_device(t);
         const TreeParams& m = t->host->prm;
         t->dev->diagnostics(m.M, m.a, m.Q, out);
+        return BH_OK;
+    });
+}
+// Dump file, version 1 (little-endian):
+//   header, 128 bytes: "BHFLUIDW", u32 version, u32 header bytes, i32 leaf
+//     count, i32 variables (8), i32 zones per block (4096), i32 0, f64 M, a,
+//     Q, gamma, time, i64 step, zero padding
+//   block table, 64 bytes per leaf: i32 level, i32 i, j, k, f64 lo[3], hi[3]
+//   data: per leaf, per variable (rho, eps, v0, v1, v2, B0, B1, B2), the
+//     4096 active zones, k fastest
+// Leaves in quilt order.
+int bh_tree_write_dump(bh_tree* t, const char* path, double time, long long step) {
+    if (!path) return fail(t, BH_EINVAL, "null path");
+    return guarded(t, [&] {
+        std::vector<int> leaves;
+        for (int b = 0; b < t->host->nblocks(); b++)
+            if (t->host->quilt[b]->leaf) leaves.push_back(b);
+        const int nl = (int)leaves.size();
+        FILE* f = std::fopen(path, "wb");
+        if (!f) return fail(t, BH_EINVAL, std::string("cannot open ") + path);
+        std::unique_ptr<FILE, int (*)(FILE*)> guard(f, std::fclose);
+        unsigned char hdr[128];
+        std::memset(hdr, 0, sizeof hdr);
+        std::memcpy(hdr, "BHFLUIDW", 8);
+        const uint32_t ver = 1, hb = 128;
+        const int32_t ints[4] = {nl, 8, NINT, 0};
+        const TreeParams& m = t->host->prm;
+        const double dbl[5] = {m.M, m.a, m.Q, m.gamma, time};
+        const int64_t st = step;
+        std::memcpy(hdr + 8, &ver, 4);
+        std::memcpy(hdr + 12, &hb, 4);
+        std::memcpy(hdr + 16, ints, 16);
+        std::memcpy(hdr + 32, dbl, 40);
+        std::memcpy(hdr + 72, &st, 8);
+        bool ok = std::fwrite(hdr, 1, sizeof hdr, f) == sizeof hdr;
+        for (int b : leaves) {
+            const Patch& p = *t->host->quilt[b];
+            unsigned char e[64];
+            const int32_t li[4] = {p.lvl, p.ijk[0], p.ijk[1], p.ijk[2]};
+            const double lohi[6] = {p.xe[0], p.ye[0], p.ze[0], p.xe[BLK], p.ye[BLK], p.ze[BLK]};
+            std::memcpy(e, li, 16);
+            std::memcpy(e + 16, lohi, 48);
+            ok = ok && std::fwrite(e, 1, sizeof e, f) == sizeof e;
+        }
+        // data in chunks of up to 256 leaves, from whichever copy is current
+        const int chunk = 256;
+        std::vector<double> buf((size_t)std::min(nl, chunk) * 8 * NINT);
+        const bool from_device = !t->host_valid;
+        DBuf<double> dbuf;
+        if (from_device) dbuf.alloc(buf.size());
+        for (int l0 = 0; l0 < nl && ok; l0 += chunk) {
+            const int n = std::min(chunk, nl - l0);
+            if (from_device) {
+                t->dev->pack_w(l0, n, dbuf.p, buf.data());
+            } else {
+                for (int l = 0; l < n; l++) {
+                    Patch& p = *t->host->quilt[leaves[l0 + l]];
+                    for (int v = 0; v < 8; v++)
+                        for (int a = 0; a < NINT; a++)
+                            buf[((size_t)l * 8 + v) * NINT + a] = p.w(v, lindx(a / 256, (a / 16) % 16, a % 16));
+                }
+            }
+            const size_t cnt = (size_t)n * 8 * NINT;
+            ok = std::fwrite(buf.data(), sizeof(double), cnt, f) == cnt;
+        }
+        if (!ok) return fail(t, BH_EINVAL, std::string("write failed: ") + path);
         return BH_OK;
     });
 }

@@ -97,6 +97,14 @@ int bh_tree_take_faults(bh_tree* t, long long* n);
  * fixed order of their own (results agree with the reference's sequential
  * sums to rounding and are identical from run to run). */
 int bh_tree_diagnostics(bh_tree* t, double* out);
+/* Write W of every leaf's active zones to a versioned binary file for the
+ * renderer (the reference leaves this format as a stub, src/fluid/data.hpp):
+ * a 128-byte header ("BHFLUIDW", version 1, leaf count, 8 variables, 4096
+ * zones per leaf, M, a, Q, gamma, time, step), a 64-byte entry per leaf
+ * (level, i, j, k, lower and upper corner), then per leaf rho, eps, v0..2,
+ * B0..2 over the active zones (k fastest), leaves in quilt order.  The data
+ * are packed on the GPU when the device holds the current state. */
+int bh_tree_write_dump(bh_tree* t, const char* path, double time, long long step);
 /* constrans::maxdivB of block b, src/fluid/grid/transport.cpp:77-89 */
 int bh_tree_maxdivB(bh_tree* t, int b, double* out);
 

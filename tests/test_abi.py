@@ -101,3 +101,25 @@ def test_residual_sign_proof_on_host():
     assert cs.check(sign, f) == 0
     far = ~near & np.isfinite(f)
     assert (sign[far] != 0).mean() > 0.9
+
+
+def test_dump_roundtrip_from_host_state(tmp_path):
+    """The dump of a tree whose current state is on the host holds every
+    leaf's active-zone W and the leaf table, as read_dump returns them."""
+    lo, hi, nqlt, regrid, a, Q = configs.R3
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    bhfluid.torus_setup(t, regrid=regrid)
+    path = tmp_path / "state.bhd"
+    t.write_dump(path, time=1.25, step=7)
+    d = bhfluid.read_dump(path)
+    meta, edges = t.blocks()
+    leaf = meta[:, 4] == 1
+    _, W, _ = t.state()
+    Wl = W[leaf].reshape(-1, 8, 20, 20, 20)[:, :, 2:18, 2:18, 2:18]
+    assert d["version"] == 1 and d["step"] == 7 and d["time"] == 1.25
+    assert (d["M"], d["a"], d["Q"], d["gamma"]) == (1.0, a, Q, configs.GAMMA)
+    assert np.array_equal(d["level"], meta[leaf, 0]) and np.array_equal(d["ijk"], meta[leaf, 1:4])
+    e = edges.reshape(-1, 3, 17)[leaf]
+    assert np.array_equal(d["lo"], e[:, :, 0]) and np.array_equal(d["hi"], e[:, :, 16])
+    assert np.array_equal(d["W"].view(np.int64), Wl.view(np.int64))
+    assert path.stat().st_size == 128 + 64 * leaf.sum() + leaf.sum() * 8 * 4096 * 8

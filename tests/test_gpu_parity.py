@@ -187,3 +187,18 @@ def test_diagnostics_against_oracle(name, nsteps):
             assert abs(dev[q] - ref[q]) <= 4 * n * 2.0 ** -52 * abs(ref[q]) + 1e-300, (name, nm, dev[q], ref[q])
         else:
             assert abs(dev[q] - ref[q]) <= tol[q] + 1e-300, (name, nm, dev[q], ref[q], tol[q])
+
+
+def test_dump_packed_on_device(tmp_path):
+    """After device steps the dump is packed on the GPU; it equals the
+    active-zone W the host reads back."""
+    t, params = build(configs.R4)
+    for _ in range(2):
+        bhfluid.RK2Integrator(configs.CFL).step(t)
+    path = tmp_path / "r4.bhd"
+    t.write_dump(path, time=0.5, step=2)
+    d = bhfluid.read_dump(path)
+    meta, _ = t.blocks()
+    _, W, _ = t.state()
+    Wl = W[meta[:, 4] == 1].reshape(-1, 8, 20, 20, 20)[:, :, 2:18, 2:18, 2:18]
+    assert np.array_equal(d["W"].view(np.int64), Wl.view(np.int64))
