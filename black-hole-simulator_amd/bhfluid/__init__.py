@@ -113,6 +113,7 @@ def lib():
         _sig(L, "bh_math_batch", _i, [_i, _i, _dp, _dp, _i])
         _sig(L, "bh_c2p_sign_batch", _i, [_d, _i, _dp, _ip, _dp, _i])
         _sig(L, "bh_tree_diagnostics", _i, [_v, _dp])
+        _sig(L, "bh_tree_regrid_device", _i, [_v, _ip])
         _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
@@ -246,6 +247,15 @@ class AmrTree:
     def ptoc_interior(self):
         """U = ptoc(W) on active zones (src/fluid/main.cpp:148-159)."""
         self._check(self._L.bh_tree_ptoc_interior(self._h), "ptoc_interior")
+
+    def regrid_device(self) -> int:
+        """``amrtree::regrid`` with the state kept on the GPU (flags and data
+        work on the device, block-table changes on the host); identical
+        result to ``regrid``.  Returns blocks refined plus parents
+        flattened."""
+        n = ctypes.c_int(0)
+        self._check(self._L.bh_tree_regrid_device(self._h, ctypes.byref(n)), "regrid_device")
+        return n.value
 
     def regrid(self):
         """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""

@@ -16,6 +16,7 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "bh_common.hpp"
@@ -978,6 +979,221 @@ __global__ __launch_bounds__(TPB) void k_diag_reduce(const double* __restrict__ 
     if (threadIdx.x < NDIAG) out[threadIdx.x] = lds[threadIdx.x * TPB];
 }
 
+// ---------------------------------------------------------------- regrid
+// The flagging loop of amrtree::regrid (grid.cpp:744-795) over the active
+// zones of every leaf: bit 0 of flags[leaf] = some zone asks for refinement
+// (r < 3 at level < 4, or a relative rho or B gradient above 0.5 at level
+// < 4), bit 1 = some zone forbids flattening (the same, or a relative rho
+// gradient above 0.1).  r, dx, dy, dz as patch::cell_init.
+__global__ __launch_bounds__(TPB) void k_regrid_flags(KP P, const int* __restrict__ leaves,
+                                                      const int* __restrict__ lvl, unsigned int* __restrict__ flags) {
+    __shared__ unsigned int wg_flags;
+    if (threadIdx.x == 0) wg_flags = 0u;
+    __syncthreads();
+    const int l = blockIdx.x / WG_INT;
+    const int b = leaves[l];
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int c = lindx(i, j, k);
+    const int level = lvl[b];
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const double xc = P.org[3 * b] + (i + 0.5) * dx, yc = P.org[3 * b + 1] + (j + 0.5) * dy,
+                 zc = P.org[3 * b + 2] + (k + 0.5) * dz;
+    double r = sqrt(xc * xc + yc * yc + zc * zc);
+    if (r < 1e-10) r = 1e-10;
+    unsigned int f = 0u;
+    if (r < 3.0 && level < 4) {
+        f = 3u;
+    } else {
+        const double chro = Wa(P, b, WRHO, c);
+        if (chro > 0.0) {
+            double drho_dx = fabs(Wa(P, b, WRHO, lindx(i + 1, j, k)) - Wa(P, b, WRHO, lindx(i - 1, j, k))) / (2 * dx);
+            double drho_dy = fabs(Wa(P, b, WRHO, lindx(i, j + 1, k)) - Wa(P, b, WRHO, lindx(i, j - 1, k))) / (2 * dy);
+            double drho_dz = fabs(Wa(P, b, WRHO, lindx(i, j, k + 1)) - Wa(P, b, WRHO, lindx(i, j, k - 1))) / (2 * dz);
+            double drho = sqrt(drho_dx * drho_dx + drho_dy * drho_dy + drho_dz * drho_dz);
+            double rgrh = drho * dx / chro;
+            if (rgrh > 0.5 && level < 4) f |= 3u;
+            if (rgrh > 0.1) f |= 2u;
+        }
+        const double B0 = Wa(P, b, WB0, c), B1 = Wa(P, b, WB1, c), B2 = Wa(P, b, WB2, c);
+        double Bmag = sqrt(B0 * B0 + B1 * B1 + B2 * B2);
+        if (Bmag > 1e-14) {
+            double dB_dx = fabs(Wa(P, b, WB0, lindx(i + 1, j, k)) - Wa(P, b, WB0, lindx(i - 1, j, k))) / (2 * dx);
+            double dB_dy = fabs(Wa(P, b, WB1, lindx(i, j + 1, k)) - Wa(P, b, WB1, lindx(i, j - 1, k))) / (2 * dy);
+            double dB_dz = fabs(Wa(P, b, WB2, lindx(i, j, k + 1)) - Wa(P, b, WB2, lindx(i, j, k - 1))) / (2 * dz);
+            double rdB = (dB_dx * dx + dB_dy * dy + dB_dz * dz) / Bmag;
+            if (rdB > 0.5 && level < 4) f |= 3u;
+        }
+    }
+    if (f) atomicOr(&wg_flags, f);
+    __syncthreads();
+    if (threadIdx.x == 0 && wg_flags) atomicOr(flags + l, wg_flags);
+}
+
+// Copy per-block slabs of `words` doubles into their new slots: dst block b
+// takes src block map[b] (map[b] < 0: left alone).
+__global__ __launch_bounds__(TPB) void k_gather(double* __restrict__ dst, const double* __restrict__ src,
+                                                const int* __restrict__ map, long words) {
+    const int b = blockIdx.y;
+    const int s = map[b];
+    if (s < 0) return;
+    for (long w = (long)blockIdx.x * TPB + threadIdx.x; w < words; w += (long)gridDim.x * TPB)
+        dst[(size_t)b * words + w] = src[(size_t)s * words + w];
+}
+
+// floor_or_keep of the regrid's data work: con2prim of the zone's U with
+// the tree metric; on failure the floor state keeps the zone's B and U is
+// recomputed (grid.cpp:637-647, 705-714).
+__device__ __forceinline__ void regrid_c2p(const KP& P, int b, int c, int& nf) {
+    GPt m = cell_tree(P, b, c);
+    Prim pv;
+    if (ctop(m, P.gamma, loadU(P, b, c), pv, nf)) {
+        storeW(P, b, c, pv);
+    } else {
+        const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+        Prim f = floor_prim(fl[c], fl[NCELL + c]);
+        f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
+        storeW(P, b, c, f);
+        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+    }
+}
+// amrtree::refine's data (grid.cpp:599-653, quirk Q24: the slope uses the +
+// neighbour as "minus"): child zone U from the parent's U (old layout, slot
+// par[n]) by limited linear prolongation, then floor_or_keep.  One
+// workgroup row per (refined parent n, child h); ch[8 n + h] = child slot.
+__global__ __launch_bounds__(TPB) void k_refine_zones(KP P, const double* __restrict__ Uold,
+                                                      const int* __restrict__ par, const int* __restrict__ ch) {
+    const int n = blockIdx.y / 8, h = blockIdx.y % 8;
+    const int a = blockIdx.x * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
+    const int pi = (i + ix * BLK) / 2, pj = (j + iy * BLK) / 2, pk = (k + iz * BLK) / 2;
+    const double ox = (i % 2 == 0) ? -0.25 : 0.25;
+    const double oy = (j % 2 == 0) ? -0.25 : 0.25;
+    const double oz = (k % 2 == 0) ? -0.25 : 0.25;
+    const int pc = lindx(pi, pj, pk);
+    const int ppx = lindx(pi + 1, pj, pk), pmx = lindx(pi - 1, pj, pk);
+    const int ppy = lindx(pi, pj + 1, pk), pmy = lindx(pi, pj - 1, pk);
+    const int ppz = lindx(pi, pj, pk + 1), pmz = lindx(pi, pj, pk - 1);
+    const double* up = Uold + (size_t)par[n] * 8 * NCELL;
+    const int b = ch[8 * n + h];
+    const int cc = lindx(i, j, k);
+    double nu[8];
+    for (int v = 0; v < 8; v++) {
+        const double* u = up + (size_t)v * NCELL;
+        const double vc = u[pc];
+        double sx = minmod(vc - u[ppx], u[pmx] - vc);
+        double sy = minmod(vc - u[ppy], u[pmy] - vc);
+        double sz = minmod(vc - u[ppz], u[pmz] - vc);
+        nu[v] = vc + sx * ox + sy * oy + sz * oz;
+    }
+    for (int v = 0; v < 8; v++) Ua(P, b, v, cc) = nu[v];
+    int nf = 0;
+    regrid_c2p(P, b, cc, nf);
+    add_faults(P, nf);
+}
+// amrtree::ccrstr (grid.cpp:657-718): parent zone U = volume-weighted mean
+// of the children's zones it covers (old layout, slots ch[8 n + h], -1 =
+// none), then floor_or_keep.  One workgroup row per flattened parent.
+__global__ __launch_bounds__(TPB) void k_restrict_zones(KP P, const double* __restrict__ Uold,
+                                                        const double* __restrict__ dxyz_old,
+                                                        const int* __restrict__ par, const int* __restrict__ ch) {
+    const int n = blockIdx.y;
+    const int a = blockIdx.x * TPB + threadIdx.x;
+    const int pi = a / 256, pj = (a / 16) % 16, pk = a % 16;
+    const int b = par[n];
+    const int pc = lindx(pi, pj, pk);
+    double acc[8];
+    for (int v = 0; v < 8; v++) acc[v] = 0.0;
+    double volt = 0.0;
+    for (int h = 0; h < 8; h++) {
+        const int cs = ch[8 * n + h];
+        if (cs < 0) continue;
+        const int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
+        const double vol = dxyz_old[3 * cs] * dxyz_old[3 * cs + 1] * dxyz_old[3 * cs + 2];
+        const double* uc = Uold + (size_t)cs * 8 * NCELL;
+        for (int di = 0; di < 2; di++)
+            for (int dj = 0; dj < 2; dj++)
+                for (int dk = 0; dk < 2; dk++) {
+                    const int fi = 2 * (pi - ix * (BLK / 2)) + di;
+                    const int fj = 2 * (pj - iy * (BLK / 2)) + dj;
+                    const int fk = 2 * (pk - iz * (BLK / 2)) + dk;
+                    if (fi < 0 || fi >= BLK || fj < 0 || fj >= BLK || fk < 0 || fk >= BLK) continue;
+                    const int fc = lindx(fi, fj, fk);
+                    volt += vol;
+                    for (int v = 0; v < 8; v++) acc[v] += vol * uc[(size_t)v * NCELL + fc];
+                }
+    }
+    if (volt > 0.0) {
+        const double inv = 1.0 / volt;
+        for (int v = 0; v < 8; v++) acc[v] *= inv;
+    }
+    for (int v = 0; v < 8; v++) Ua(P, b, v, pc) = acc[v];
+    int nf = 0;
+    regrid_c2p(P, b, pc, nf);
+    add_faults(P, nf);
+}
+// patch::B_init (grid.cpp:208-251, quirk Q28: the z sweep tests i, not k)
+// of the listed blocks: face fields from the stored cells' W.B.
+__global__ __launch_bounds__(TPB) void k_binit(KP P, const int* __restrict__ blocks) {
+    const int b = blocks[blockIdx.y];
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= 3 * NFACE) return;
+    const int dir = t / NFACE, f = t % NFACE;
+    auto WB = [&](int d, int i, int j, int k) { return Wa(P, b, WB0 + d, lindx(i, j, k)); };
+    double v;
+    int i, j, k, idx;
+    if (dir == 0) {  // i in [-2, 18], j, k in [-2, 18)
+        i = f / 400 - GH; j = (f / 20) % 20 - GH; k = f % 20 - GH;
+        if (i > -GH && i < BLK + GH) v = (WB(0, i - 1, j, k) + WB(0, i, j, k)) / 2;
+        else if (i == -GH) v = WB(0, i, j, k);
+        else v = WB(0, i - 1, j, k);
+        idx = ixf(i, j, k);
+    } else if (dir == 1) {  // j in [-2, 18]
+        i = f / 420 - GH; j = (f / 20) % 21 - GH; k = f % 20 - GH;
+        if (j > -GH && j < BLK + GH) v = (WB(1, i, j - 1, k) + WB(1, i, j, k)) / 2;
+        else if (j == -GH) v = WB(1, i, j, k);
+        else v = WB(1, i, j - 1, k);
+        idx = iyf(i, j, k);
+    } else {  // k in [-2, 18]
+        i = f / 420 - GH; j = (f / 21) % 20 - GH; k = f % 21 - GH;
+        if (k > -GH && k < BLK + GH) v = (WB(2, i, j, k - 1) + WB(2, i, j, k)) / 2;
+        else if (i == -GH) v = WB(2, i, j, k);
+        else v = WB(2, i, j, k - 1);
+        idx = izf(i, j, k);
+    }
+    P.Bf[((size_t)b * 3 + dir) * NFACE + idx] = v;
+}
+// Exact-copy flags of the ghost plan (see mark_exact_copies) from the
+// device's tree-metric tables: one workgroup per (leaf, side).
+__global__ __launch_bounds__(TPB) void k_mark_same(KP P, const int* __restrict__ leaves,
+                                                   unsigned char* __restrict__ same) {
+    __shared__ int differ;
+    if (threadIdx.x == 0) differ = 0;
+    __syncthreads();
+    const int b = leaves[blockIdx.x / 6], side = blockIdx.x % 6;
+    const int kind = P.ghtab[b * 12 + side * 2], q = P.ghtab[b * 12 + side * 2 + 1];
+    if (kind != GH_COPY) {
+        if (threadIdx.x == 0) same[b * 6 + side] = 0;
+        return;
+    }
+    const int dim = side / 2, s = side % 2;
+    int d = 0;
+    for (int t = threadIdx.x; t < GH * BLK * BLK; t += TPB) {
+        const int gl = t / (BLK * BLK), t1 = (t / BLK) % BLK, t2 = t % BLK;
+        const int c = cell_of(dim, s ? BLK + gl : gl - GH, t1, t2);
+        const int sc = cell_of(dim, (s ? 0 : BLK - GH) + gl, t1, t2);
+        for (int v = 0; v < NGCT; v++) {
+            const double x = P.gct[((size_t)b * NGCT + v) * NCELL + c];
+            const double y = P.gct[((size_t)q * NGCT + v) * NCELL + sc];
+            if (__double_as_longlong(x) != __double_as_longlong(y)) d = 1;
+        }
+    }
+    if (d) atomicOr(&differ, 1);
+    __syncthreads();
+    if (threadIdx.x == 0) same[b * 6 + side] = differ ? 0 : 1;
+}
+
 // ----------------------------------------------------------- state dump
 // W of the active zones of leaves l0 .. l0 + n - 1, packed [n][8][4096]
 // (k fastest) for the dump file.
@@ -1116,7 +1332,7 @@ public:
     GhostPlan plan;
     DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz, org;
     DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
-    DBuf<int> ghtab, leaves, waveblk;
+    DBuf<int> ghtab, leaves, waveblk, lvl;
     DBuf<unsigned char> ghsame;
     std::vector<int> wave_off;  // offsets into waveblk
     DBuf<short> cell_order;
@@ -1197,23 +1413,72 @@ public:
         return P;
     }
 
-    // (Re)build for the host tree's current block structure.
-    void build(const HostTree& t) {
-        gamma = t.prm.gamma;
-        nb = t.nblocks();
-        GeomHost g;
-        build_geometry(t, g);
-        build_ghost_plan(t, plan);
-        mark_exact_copies(g, plan);
-        U.alloc((size_t)nb * 8 * NCELL);
-        W.alloc((size_t)nb * 8 * NCELL);
-        Bf.alloc((size_t)nb * 3 * NFACE);
+    // Per-block scratch arrays (sized by nb), cleared.
+    void alloc_scratch() {
         Us.alloc((size_t)nb * 5 * NINT);
         F.alloc((size_t)nb * 3 * 8 * NFC);
         E.alloc((size_t)nb * 3 * NEDGE);
         HIPCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
         HIPCHK(hipMemsetAsync(E.p, 0, E.n * sizeof(double), stream));
         HIPCHK(hipMemsetAsync(Us.p, 0, Us.n * sizeof(double), stream));
+        cst.alloc((size_t)nb * NCELL);
+        HIPCHK(hipMemsetAsync(cst.p, 0, cst.n, stream));
+        lg.alloc((size_t)nb * 2 * NCELL);
+    }
+    // Everything derived from the block structure: ghost plan, leaf list,
+    // ordering waves, queues and reduction scratch.  The exact-copy flags
+    // come from the host geometry when given, else from the device tables.
+    void tables(const HostTree& t, const GeomHost* g) {
+        build_ghost_plan(t, plan);
+        ghtab.alloc(plan.tab.size()); ghtab.upload(plan.tab.data(), plan.tab.size(), stream);
+        leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
+        if (!plan.leaves.empty()) leaves.upload(plan.leaves.data(), plan.leaves.size(), stream);
+        ghsame.alloc(plan.same.size());
+        if (g) {
+            mark_exact_copies(*g, plan);
+            ghsame.upload(plan.same.data(), plan.same.size(), stream);
+        } else {
+            HIPCHK(hipMemsetAsync(ghsame.p, 0, ghsame.n, stream));
+            if (!plan.leaves.empty())
+                hipLaunchKernelGGL(k_mark_same, dim3(plan.leaves.size() * 6), dim3(TPB), 0, stream, kp(),
+                                   (const int*)leaves.p, ghsame.p);
+            HIPCHK(hipGetLastError());
+        }
+        std::vector<int> wb;
+        wave_off.assign(1, 0);
+        for (auto& w : plan.waves) {
+            wb.insert(wb.end(), w.begin(), w.end());
+            wave_off.push_back((int)wb.size());
+        }
+        waveblk.alloc(std::max<size_t>(1, wb.size()));
+        if (!wb.empty()) waveblk.upload(wb.data(), wb.size(), stream);
+        std::vector<int> lv(nb);
+        for (int b = 0; b < nb; b++) lv[b] = t.quilt[b]->lvl;
+        lvl.alloc(std::max<size_t>(1, lv.size()));
+        if (!lv.empty()) lvl.upload(lv.data(), lv.size(), stream);
+        diagpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT) * NDIAG + NDIAG);
+        dtpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT));
+        {
+            // the largest pushing grid, in workgroups: c2p over leaves, floors over all blocks
+            const size_t wg = std::max<size_t>({(size_t)1, plan.leaves.size() * WG_CELL, (size_t)nb * WG_INT});
+            qcap = (unsigned int)(((wg + NSUBQ - 1) / NSUBQ) * TPB);
+            const size_t total = (size_t)NSUBQ * qcap;
+            qn.alloc((size_t)NSUBQ * QN_STRIDE);
+            HIPCHK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(unsigned int), stream));
+            qb.alloc(total); qc.alloc(total); qx.alloc(total); qs.alloc(total * QS_WORDS);
+        }
+    }
+
+    // (Re)build for the host tree's current block structure.
+    void build(const HostTree& t) {
+        gamma = t.prm.gamma;
+        nb = t.nblocks();
+        GeomHost g;
+        build_geometry(t, g);
+        U.alloc((size_t)nb * 8 * NCELL);
+        W.alloc((size_t)nb * 8 * NCELL);
+        Bf.alloc((size_t)nb * 3 * NFACE);
+        alloc_scratch();
         gct.alloc(g.cell_tree.size()); gct.upload(g.cell_tree.data(), g.cell_tree.size(), stream);
         gcc.alloc(g.cell_cell.size()); gcc.upload(g.cell_cell.data(), g.cell_cell.size(), stream);
         gfl.alloc(g.floors.size()); gfl.upload(g.floors.data(), g.floors.size(), stream);
@@ -1228,19 +1493,7 @@ public:
             }
             org.alloc(o.size()); org.upload(o.data(), o.size(), stream);
         }
-        diagpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT) * NDIAG + NDIAG);
-        ghtab.alloc(plan.tab.size()); ghtab.upload(plan.tab.data(), plan.tab.size(), stream);
-        ghsame.alloc(plan.same.size()); ghsame.upload(plan.same.data(), plan.same.size(), stream);
-        leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
-        if (!plan.leaves.empty()) leaves.upload(plan.leaves.data(), plan.leaves.size(), stream);
-        std::vector<int> wb;
-        wave_off.assign(1, 0);
-        for (auto& w : plan.waves) {
-            wb.insert(wb.end(), w.begin(), w.end());
-            wave_off.push_back((int)wb.size());
-        }
-        waveblk.alloc(std::max<size_t>(1, wb.size()));
-        if (!wb.empty()) waveblk.upload(wb.data(), wb.size(), stream);
+        tables(t, &g);
         // storage-order ghost list, then the con2prim order
         std::vector<short> order;
         for (int c = 0; c < NCELL; c++) {
@@ -1254,21 +1507,147 @@ public:
         cell_order.alloc(order.size()); cell_order.upload(order.data(), order.size(), stream);
         scal.alloc(2);
         HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
-        cst.alloc((size_t)nb * NCELL);
-        dtpart.alloc(std::max<size_t>(1, plan.leaves.size() * WG_INT));
-        lg.alloc((size_t)nb * 2 * NCELL);
-        {
-            // the largest pushing grid, in workgroups: c2p over leaves, floors over all blocks
-            const size_t wg = std::max<size_t>({(size_t)1, plan.leaves.size() * WG_CELL, (size_t)nb * WG_INT});
-            qcap = (unsigned int)(((wg + NSUBQ - 1) / NSUBQ) * TPB);
-            const size_t total = (size_t)NSUBQ * qcap;
-            qn.alloc((size_t)NSUBQ * QN_STRIDE);
-            HIPCHK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(unsigned int), stream));
-            qb.alloc(total); qc.alloc(total); qx.alloc(total); qs.alloc(total * QS_WORDS);
-        }
         qstat.alloc(QS_N);
         HIPCHK(hipMemsetAsync(qstat.p, 0, QS_N * sizeof(unsigned long long), stream));
         HIPCHK(hipStreamSynchronize(stream));
+    }
+
+    // Re-lay a per-block array for the new block order: new slot b takes old
+    // slot map_old[b], or (for a block that did not exist) slot map_new[b]
+    // of `fresh` (packed host data of the new blocks), else zeros.
+    void relayout(DBuf<double>& X, size_t words, int nb_new, const int* map_old_d, const int* map_new_d,
+                  const std::vector<double>* fresh) {
+        DBuf<double> Y;
+        Y.alloc((size_t)nb_new * words);
+        HIPCHK(hipMemsetAsync(Y.p, 0, Y.n * sizeof(double), stream));
+        const int chunks = (int)std::min<size_t>(64, (words + TPB - 1) / TPB);
+        if (nb_new > 0) {
+            hipLaunchKernelGGL(k_gather, dim3(chunks, nb_new), dim3(TPB), 0, stream, Y.p, (const double*)X.p,
+                               map_old_d, (long)words);
+            if (fresh && !fresh->empty()) {
+                DBuf<double> tmp;
+                tmp.alloc(fresh->size());
+                tmp.upload(fresh->data(), fresh->size(), stream);
+                hipLaunchKernelGGL(k_gather, dim3(chunks, nb_new), dim3(TPB), 0, stream, Y.p, (const double*)tmp.p,
+                                   map_new_d, (long)words);
+                HIPCHK(hipStreamSynchronize(stream));
+            }
+        }
+        HIPCHK(hipGetLastError());
+        std::swap(X.p, Y.p);
+        std::swap(X.n, Y.n);
+    }
+
+    // amrtree::regrid with the state on the device: flags from the device
+    // state, the tree changes on the host, the data work on the device.
+    // Returns the number of blocks refined plus flattened.
+    int regrid(HostTree& t) {
+        const int nl = (int)plan.leaves.size();
+        for (int b = 0; b < nb; b++) t.quilt[b]->dev = b;
+        std::vector<unsigned char> lf(nl, 0);
+        if (nl) {
+            DBuf<unsigned int> fl;
+            fl.alloc(nl);
+            HIPCHK(hipMemsetAsync(fl.p, 0, fl.n * sizeof(unsigned int), stream));
+            hipLaunchKernelGGL(k_regrid_flags, dim3(nl * WG_INT), dim3(TPB), 0, stream, kp(), (const int*)leaves.p,
+                               (const int*)lvl.p, fl.p);
+            HIPCHK(hipGetLastError());
+            std::vector<unsigned int> f(nl);
+            HIPCHK(hipMemcpyAsync(f.data(), fl.p, nl * sizeof(unsigned int), hipMemcpyDeviceToHost, stream));
+            HIPCHK(hipStreamSynchronize(stream));
+            for (int l = 0; l < nl; l++) {
+                const int lv = t.quilt[plan.leaves[l]]->lvl;
+                const bool refine = (f[l] & 1u) && lv < 4;
+                const bool erase = !(f[l] & 2u) && lv > 0;
+                lf[l] = (refine ? 1 : 0) | (erase ? 2 : 0);
+            }
+        }
+        RegridLog log;
+        t.regrid_structure(lf, log);
+        if (log.refined.empty() && log.flattened.empty()) return 0;
+        // new layout
+        const int nb_new = t.nblocks();
+        std::vector<int> map_old(nb_new), map_new(nb_new, -1), fresh_blocks;
+        std::unordered_map<const Patch*, int> slot;
+        for (int b = 0; b < nb_new; b++) {
+            const Patch* p = t.quilt[b].get();
+            slot[p] = b;
+            map_old[b] = p->dev;
+            if (p->dev < 0) { map_new[b] = (int)fresh_blocks.size(); fresh_blocks.push_back(b); }
+        }
+        GeomHost g;
+        build_geometry_blocks(t, fresh_blocks, g);
+        std::vector<double> o((size_t)fresh_blocks.size() * 3);
+        for (size_t n = 0; n < fresh_blocks.size(); n++) {
+            const Patch& p = *t.quilt[fresh_blocks[n]];
+            o[3 * n] = p.xe[0]; o[3 * n + 1] = p.ye[0]; o[3 * n + 2] = p.ze[0];
+        }
+        DBuf<int> mo, mn;
+        mo.alloc(nb_new); mo.upload(map_old.data(), nb_new, stream);
+        mn.alloc(nb_new); mn.upload(map_new.data(), nb_new, stream);
+        // the old state stays the source of the data work
+        DBuf<double> Uold, dxyz_old;
+        Uold.alloc(U.n);
+        HIPCHK(hipMemcpyAsync(Uold.p, U.p, U.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        dxyz_old.alloc(dxyz.n);
+        HIPCHK(hipMemcpyAsync(dxyz_old.p, dxyz.p, dxyz.n * sizeof(double), hipMemcpyDeviceToDevice, stream));
+        relayout(U, (size_t)8 * NCELL, nb_new, mo.p, mn.p, nullptr);
+        relayout(W, (size_t)8 * NCELL, nb_new, mo.p, mn.p, nullptr);
+        relayout(Bf, (size_t)3 * NFACE, nb_new, mo.p, mn.p, nullptr);
+        relayout(gct, (size_t)NGCT * NCELL, nb_new, mo.p, mn.p, &g.cell_tree);
+        relayout(gcc, (size_t)NGCC * NCELL, nb_new, mo.p, mn.p, &g.cell_cell);
+        relayout(gfl, (size_t)2 * NCELL, nb_new, mo.p, mn.p, &g.floors);
+        relayout(gfc, (size_t)3 * NGFC * NFC, nb_new, mo.p, mn.p, &g.face);
+        relayout(gchr, (size_t)NCHR * NINT, nb_new, mo.p, mn.p, &g.chr);
+        relayout(dxyz, 3, nb_new, mo.p, mn.p, &g.dxyz);
+        relayout(org, 3, nb_new, mo.p, mn.p, &o);
+        nb = nb_new;
+        alloc_scratch();
+        KP P = kp();
+        std::vector<int> binit;
+        if (!log.refined.empty()) {
+            std::vector<int> par, ch;
+            for (Patch* p : log.refined) {
+                par.push_back(p->dev);
+                for (int h = 0; h < 8; h++) {
+                    ch.push_back(slot.at(p->children[h]));
+                    binit.push_back(ch.back());
+                }
+            }
+            DBuf<int> dp, dc;
+            dp.alloc(par.size()); dp.upload(par.data(), par.size(), stream);
+            dc.alloc(ch.size()); dc.upload(ch.data(), ch.size(), stream);
+            hipLaunchKernelGGL(k_refine_zones, dim3(NINT / TPB, (unsigned)(8 * par.size())), dim3(TPB), 0, stream, P,
+                               (const double*)Uold.p, (const int*)dp.p, (const int*)dc.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        if (!log.flattened.empty()) {
+            std::vector<int> par, ch;
+            for (auto& fp : log.flattened) {
+                par.push_back(slot.at(fp.first));
+                binit.push_back(par.back());
+                for (int h = 0; h < 8; h++) ch.push_back(fp.second[h]);
+            }
+            DBuf<int> dp, dc;
+            dp.alloc(par.size()); dp.upload(par.data(), par.size(), stream);
+            dc.alloc(ch.size()); dc.upload(ch.data(), ch.size(), stream);
+            hipLaunchKernelGGL(k_restrict_zones, dim3(NINT / TPB, (unsigned)par.size()), dim3(TPB), 0, stream, P,
+                               (const double*)Uold.p, (const double*)dxyz_old.p, (const int*)dp.p, (const int*)dc.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        if (!binit.empty()) {
+            DBuf<int> db;
+            db.alloc(binit.size()); db.upload(binit.data(), binit.size(), stream);
+            hipLaunchKernelGGL(k_binit, dim3((3 * NFACE + TPB - 1) / TPB, (unsigned)binit.size()), dim3(TPB), 0, stream,
+                               P, (const int*)db.p);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        tables(t, nullptr);
+        HIPCHK(hipStreamSynchronize(stream));
+        return (int)(log.refined.size() + log.flattened.size());
     }
 
     void upload_state(const HostTree& t) {
@@ -1554,6 +1933,19 @@ int bh_tree_regrid(bh_tree* t) {
         host_changed(t);
         t->layout_dirty = true;
         return finish_faults(t, t->host->faults - f0);
+    });
+}
+int bh_tree_regrid_device(bh_tree* t, int* changed) {
+    return guarded(t, [&] {
+        need_device(t);
+        const int n = t->dev->regrid(*t->host);
+        if (changed) *changed = n;
+        // the device holds the state of the new layout; the host copy of
+        // every block is now stale
+        t->host_valid = false;
+        t->dev_valid = true;
+        t->layout_dirty = false;
+        return finish_faults(t, t->dev->take_faults());
     });
 }
 int bh_tree_nblocks(const bh_tree* t, int* nb) {

@@ -645,6 +645,73 @@ void HostTree::regrid() {
     erased_.clear();
 }
 
+// amrtree::regrid's tree changes (grid.cpp:739-835) with the flags of the
+// flagging loop given: the refine pass creates the children in the same
+// quilt positions as refine() (no data), the flatten pass removes the same
+// children as flatten() (no restriction) and logs what the data work needs.
+void HostTree::regrid_structure(const std::vector<unsigned char>& leaf_flags, RegridLog& log) {
+    size_t n = 0;
+    for (auto& pp : quilt) {
+        if (!pp->leaf) continue;
+        if (n >= leaf_flags.size()) throw std::runtime_error("regrid flags: fewer flags than leaves");
+        const unsigned char f = leaf_flags[n++];
+        if (f & 1) pp->refine = true;
+        if (f & 2) pp->erase = true;
+    }
+    if (n != leaf_flags.size()) throw std::runtime_error("regrid flags: more flags than leaves");
+    std::vector<Patch*> rfnQ, fltQ;
+    for (auto& pp : quilt) {
+        if (pp->leaf && pp->refine) rfnQ.push_back(pp.get());
+        if (pp->leaf && pp->erase) fltQ.push_back(pp.get());
+    }
+    for (Patch* p : rfnQ) {
+        double xm = (p->xe[0] + p->xe[BLK]) / 2, ym = (p->ye[0] + p->ye[BLK]) / 2, zm = (p->ze[0] + p->ze[BLK]) / 2;
+        double xl = p->xe[0], xh = p->xe[BLK], yl = p->ye[0], yh = p->ye[BLK], zl = p->ze[0], zh = p->ze[BLK];
+        for (int h = 0; h < 8; h++) {
+            int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
+            std::array<double, 3> c1 = {ix ? xm : xl, iy ? ym : yl, iz ? zm : zl};
+            std::array<double, 3> c2 = {ix ? xh : xm, iy ? yh : ym, iz ? zh : zm};
+            std::array<int, 3> cijk = {2 * p->ijk[0] + ix, 2 * p->ijk[1] + iy, 2 * p->ijk[2] + iz};
+            quilt.push_back(std::make_unique<Patch>(p->lvl + 1, cijk, c1, c2, p));
+            p->children[h] = quilt.back().get();
+        }
+        p->leaf = false;
+        p->refine = false;
+        log.refined.push_back(p);
+    }
+    for (Patch* p : fltQ) {
+        if (std::find(erased_.begin(), erased_.end(), p) != erased_.end()) continue;
+        Patch* par = p->parent;
+        if (par && par->leaf == false) {
+            bool dhl = true;
+            for (int h = 0; h < 8; h++) {
+                Patch* sib = par->children[h];
+                if (sib && (!sib->leaf || sib->refine)) { dhl = false; break; }
+            }
+            bool all_leaves = true;
+            for (int h = 0; h < 8; h++)
+                if (par->children[h] && !par->children[h]->leaf) all_leaves = false;
+            if (dhl && all_leaves) {
+                std::array<int, 8> slots;
+                for (int h = 0; h < 8; h++) {
+                    Patch* ch = par->children[h];
+                    slots[h] = ch ? ch->dev : -1;
+                    if (!ch) continue;
+                    erased_.push_back(ch);
+                    quilt.erase(std::remove_if(quilt.begin(), quilt.end(),
+                                               [&](const std::unique_ptr<Patch>& q) { return q.get() == ch; }),
+                                quilt.end());
+                    par->children[h] = nullptr;
+                }
+                par->leaf = true;
+                log.flattened.push_back({par, slots});
+            }
+        }
+        if (std::find(erased_.begin(), erased_.end(), p) == erased_.end()) p->erase = false;
+    }
+    erased_.clear();
+}
+
 // ------------------------------------------------------- state exchange
 void HostTree::export_blocks(int* meta, double* edges) const {
     for (size_t b = 0; b < quilt.size(); b++) {
@@ -680,8 +747,8 @@ void HostTree::import_state(const double* U, const double* W, const double* Bf) 
 // ------------------------------------------------------------ geometry
 namespace bh {
 
-void build_geometry(const HostTree& t, GeomHost& g) {
-    const int nb = t.nblocks();
+void build_geometry_blocks(const HostTree& t, const std::vector<int>& blocks, GeomHost& g) {
+    const int nb = (int)blocks.size();
     const TreeParams P = t.prm;
     g.cell_tree.assign((size_t)nb * NGCT * NCELL, 0.0);
     g.cell_cell.assign((size_t)nb * NGCC * NCELL, 0.0);
@@ -692,7 +759,7 @@ void build_geometry(const HostTree& t, GeomHost& g) {
     int bad_chr = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_chr)
     for (int b = 0; b < nb; b++) {
-        const Patch& p = *t.quilt[b];
+        const Patch& p = *t.quilt[blocks[b]];
         g.dxyz[3 * b] = p.dx(); g.dxyz[3 * b + 1] = p.dy(); g.dxyz[3 * b + 2] = p.dz();
         double* ct = &g.cell_tree[(size_t)b * NGCT * NCELL];
         double* cc = &g.cell_cell[(size_t)b * NGCC * NCELL];
@@ -752,6 +819,12 @@ void build_geometry(const HostTree& t, GeomHost& g) {
                 }
     }
     if (bad_chr) throw std::runtime_error("a lowered Christoffel of the cell metric expected to be +0.0 is not");
+}
+
+void build_geometry(const HostTree& t, GeomHost& g) {
+    std::vector<int> all(t.nblocks());
+    for (int b = 0; b < t.nblocks(); b++) all[b] = b;
+    build_geometry_blocks(t, all, g);
 }
 
 void build_ghost_plan(const HostTree& t, GhostPlan& g) {

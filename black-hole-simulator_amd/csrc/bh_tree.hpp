@@ -31,6 +31,7 @@ struct Patch {
     bool leaf = true;
     bool refine = false;
     bool erase = false;
+    int dev = -1;  // slot of this block in the device arrays before a device regrid
     double xe[BLK + 1], ye[BLK + 1], ze[BLK + 1];
     std::vector<double> U, W, Bf;  // [8][8000], [8][8000], [3][8400]
     std::vector<double> r, th, xc, yc, zc, vol;  // per stored cell
@@ -55,6 +56,14 @@ struct TreeParams {
     double M, a, Q, gamma;
 };
 
+// Structural record of a regrid whose data work runs elsewhere (on the
+// device): the refined leaves in refine order and, per flattened parent, the
+// slots its removed children held.
+struct RegridLog {
+    std::vector<Patch*> refined;
+    std::vector<std::pair<Patch*, std::array<int, 8>>> flattened;
+};
+
 class HostTree {
 public:
     TreeParams prm;
@@ -74,6 +83,10 @@ public:
     void refine(Patch* p);
     void ccrstr(Patch* p);
     void flatten(Patch* p);
+    // regrid's tree changes alone, from per-leaf flags computed elsewhere
+    // (leaf order = quilt order; bit 0 refine, bit 1 erase); the data work of
+    // refine and ccrstr is left to the caller, who reads it from `log`
+    void regrid_structure(const std::vector<unsigned char>& leaf_flags, RegridLog& log);
 
     int nblocks() const { return (int)quilt.size(); }
     int index_of(const Patch* p) const;
@@ -110,6 +123,8 @@ struct GeomHost {
     std::vector<double> dxyz;       // [nb][3]
 };
 void build_geometry(const HostTree& t, GeomHost& g);
+// the same tables for the listed blocks only, packed in list order
+void build_geometry_blocks(const HostTree& t, const std::vector<int>& blocks, GeomHost& g);
 BH_HD int sym4(int n, int l) { return n <= l ? n * (7 - n) / 2 + l : l * (7 - l) / 2 + n; }
 
 // Ghost-fill plan: for every leaf and face side the neighbour found by

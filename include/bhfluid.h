@@ -53,6 +53,14 @@ int bh_tree_B_pot_init(bh_tree* t, double rho_max);
 int bh_tree_ptoc_interior(bh_tree* t);
 /* grid::amrtree::regrid(), src/fluid/grid/grid.cpp:739-835. */
 int bh_tree_regrid(bh_tree* t);
+/* The same regrid (grid.cpp:739-835) with the state on the GPU: the
+ * refinement flags are computed on the device, the block table changes on
+ * the host, and prolongation (refine), restriction (ccrstr), their con2prim
+ * and floor fallback and patch::B_init run on the device, which keeps the
+ * state; only the geometry of new blocks is built on the host.  Results are
+ * identical to bh_tree_regrid.  *changed (may be NULL) = blocks refined plus
+ * parents flattened. */
+int bh_tree_regrid_device(bh_tree* t, int* changed);
 
 int bh_tree_nblocks(const bh_tree* t, int* nb);
 int bh_tree_get_blocks(const bh_tree* t, int* meta, double* edges);

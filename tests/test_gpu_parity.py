@@ -202,3 +202,40 @@ def test_dump_packed_on_device(tmp_path):
     _, W, _ = t.state()
     Wl = W[meta[:, 4] == 1].reshape(-1, 8, 20, 20, 20)[:, :, 2:18, 2:18, 2:18]
     assert np.array_equal(d["W"].view(np.int64), Wl.view(np.int64))
+
+
+def _same_state(a, b):
+    ma, ea = a.blocks()
+    mb, eb = b.blocks()
+    assert np.array_equal(ma, mb) and bits_equal(ea, eb)
+    for x, y, nm in zip(a.state(), b.state(), ("U", "W", "Bf")):
+        n, ex = first_diff(x, y)
+        assert n == 0, (nm, n, ex)
+
+
+@pytest.mark.parametrize("name", ["R2", "R3"])
+def test_regrid_on_device_matches_host(name):
+    """amrtree::regrid with the state on the GPU (device flags, host block
+    table, device prolongation / restriction / B_init) gives the host
+    regrid's blocks and state bit for bit, before and after a step and a
+    second regrid."""
+    lo, hi, nqlt, regrid, a, Q = getattr(configs, name)
+    assert regrid
+    host = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    dev = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    for t in (host, dev):
+        bhfluid.torus_setup(t, regrid=False)
+    host.regrid()
+    dev.prepare()
+    changed = dev.regrid_device()
+    assert changed > 0
+    _same_state(host, dev)
+    for t in (host, dev):
+        try:
+            bhfluid.RK2Integrator(configs.CFL).step(t)
+        except bhfluid.DomainError:
+            pass
+    _same_state(host, dev)
+    host.regrid()
+    dev.regrid_device()
+    _same_state(host, dev)
