@@ -119,8 +119,29 @@ def build_tree(dom=DOM, nqlt=NQLT, regrid=True):
     import bhfluid
 
     t = bhfluid.AmrTree((-dom,) * 3, (dom,) * 3, nqlt, 1.0, A_SPIN, Q_CHARGE, GAMMA)
+    t0 = time.perf_counter()
     bhfluid.torus_setup(t, regrid=regrid)
+    t.host_setup_s = time.perf_counter() - t0
     return t
+
+
+def device_setup(nqlt):
+    """main.cpp:94-168 with the state on the GPU throughout (initial data and
+    regrid on the device) on a fresh tree of the bench grid; seconds of each
+    part (the first includes building the roots' geometry tables)."""
+    import bhfluid
+
+    t = bhfluid.AmrTree((-DOM,) * 3, (DOM,) * 3, nqlt, 1.0, A_SPIN, Q_CHARGE, GAMMA)
+    t0 = time.perf_counter()
+    t.init_torus_device()
+    bhfluid.device_sync()
+    t1 = time.perf_counter()
+    t.regrid_device()
+    bhfluid.device_sync()
+    t2 = time.perf_counter()
+    nb = t.nblocks
+    del t
+    return {"init_torus_device_s": round(t1 - t0, 3), "regrid_device_s": round(t2 - t1, 3), "patches": nb}
 
 
 def cpu_baseline():
@@ -175,6 +196,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-device-setup", action="store_true",
+                    help="skip timing the initial data and regrid on the device")
     ap.add_argument("--nqlt", type=int, default=NQLT)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="before the timed region, run the traffic-counter calibration copy of MIB MiB")
@@ -239,6 +262,9 @@ def main():
     Ui = U.reshape(nb, 8, 20, 20, 20)[leaf][:, :, 2:18, 2:18, 2:18]
     nonfinite = int((~np.isfinite(Ui).all(axis=1)).sum())
     del U, Ui
+    host_setup_s = tree.host_setup_s
+    del tree
+    dev_setup = device_setup(args.nqlt) if not args.no_device_setup else None
 
     value = leaf_zones * args.steps / el
     ms_per_step = 1e3 * el / args.steps
@@ -290,6 +316,8 @@ def main():
             "kernels": kernels,
             "brent_queued_cells": queued,
             "diagnostics_ms": round(diag_ms, 3),
+            "setup_host_torus_and_regrid_s": round(host_setup_s, 2),
+            "setup_on_device": dev_setup,
             "diagnostics_after": {k: float(v) for k, v in diag.items()},
         },
         "roofline": {

@@ -114,6 +114,8 @@ def lib():
         _sig(L, "bh_c2p_sign_batch", _i, [_d, _i, _dp, _ip, _dp, _i])
         _sig(L, "bh_tree_diagnostics", _i, [_v, _dp])
         _sig(L, "bh_tree_regrid_device", _i, [_v, _ip])
+        _sig(L, "bh_tree_init_torus_device", _i, [_v, _dp])
+        _sig(L, "bh_math2_batch", _i, [_i, _i, _dp, _dp, _dp, _i])
         _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
@@ -257,6 +259,20 @@ class AmrTree:
         self._check(self._L.bh_tree_regrid_device(self._h, ctypes.byref(n)), "regrid_device")
         return n.value
 
+    def init_torus_device(self) -> float:
+        """The whole initial data of src/fluid/main.cpp:94-159 computed on the
+        GPU (constructor floor state, ``fm_init``, global rho_max,
+        ``B_pot_init``, ``ptoc`` of the active zones), with correctly rounded
+        transcendental functions; the state stays on the device.  Returns the
+        global rho_max.  A torus that cannot be placed leaves the atmosphere,
+        as in the reference (``self.torus_placed`` is then False)."""
+        out = ctypes.c_double(0.0)
+        rc = self._L.bh_tree_init_torus_device(self._h, ctypes.byref(out))
+        self.torus_placed = rc != 2
+        if rc != 2:
+            self._check(rc, "init_torus_device")
+        return out.value
+
     def regrid(self):
         """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""
         self._check(self._L.bh_tree_regrid(self._h), "regrid")
@@ -327,10 +343,16 @@ class init:
         tree._check(tree._L.bh_tree_B_pot_init(tree._h, float(rho_max)), "B_pot_init")
 
 
-def torus_setup(tree: AmrTree, regrid: bool = False) -> float:
+def torus_setup(tree: AmrTree, regrid: bool = False, device: bool = False) -> float:
     """The initialisation sequence of src/fluid/main.cpp:98-168: torus, global
     rho_max, magnetic field, conserved variables, optional regrid.  Returns
-    rho_max."""
+    rho_max.  ``device=True`` runs all of it on the GPU
+    (``AmrTree.init_torus_device`` and ``regrid_device``) for a fresh tree."""
+    if device:
+        rho_max = tree.init_torus_device()
+        if regrid:
+            tree.regrid_device()
+        return rho_max
     init.fm_init(tree)  # main.cpp:98 ignores a torus that could not be placed
     rho_max = tree.rho_max()
     if rho_max < 1e-14:

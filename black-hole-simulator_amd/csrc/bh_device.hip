@@ -22,6 +22,7 @@
 #include "bh_common.hpp"
 #include "bh_crmath.hpp"
 #include "bh_physics.hpp"
+#include "bh_init.hpp"
 #include "bh_tree.hpp"
 #include "../../include/bhfluid.h"
 
@@ -86,6 +87,7 @@ struct KP {
     const double* gchr;  // [nb][40][4096]
     const double* dxyz;  // [nb][3]
     const double* org;   // [nb][3] first x, y, z cell edge of each block
+    const double* gth;   // [nb][8000] polar angle of each cell centre
     const int* ghtab;    // [nb][12]
     const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
     // [3904] ghost cells in storage order (the fill copies whole rows and
@@ -979,6 +981,187 @@ __global__ __launch_bounds__(TPB) void k_diag_reduce(const double* __restrict__ 
     if (threadIdx.x < NDIAG) out[threadIdx.x] = lds[threadIdx.x * TPB];
 }
 
+// ---------------------------------------------------------- initial data
+// The set-up of src/fluid/main.cpp:94-159 on the device: the constructor's
+// floor state, fm_init, the global rho_max, B_pot_init and ptoc of the active
+// zones.  Cell r and theta, the floor state and the metric come from the
+// geometry tables; the transcendental functions of the initial data are the
+// correctly rounded ones of bh_crtrig.hpp.
+struct FmPar { double M, a, Q, Gamma, K, l0, W_in, r_H, rho_scal; };
+
+__device__ __forceinline__ void cell_centre(const KP& P, int b, int i, int j, int k, double& xc, double& yc,
+                                            double& zc) {
+    xc = P.org[3 * b] + (i + 0.5) * P.dxyz[3 * b];
+    yc = P.org[3 * b + 1] + (j + 0.5) * P.dxyz[3 * b + 1];
+    zc = P.org[3 * b + 2] + (k + 0.5) * P.dxyz[3 * b + 2];
+}
+// maximum of non-negative doubles by their bit patterns, one atomic per workgroup
+__device__ __forceinline__ void wg_max_bits(unsigned long long v, unsigned long long* out) {
+    __shared__ unsigned long long red[TPB];
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = red[threadIdx.x] > red[threadIdx.x + h] ? red[threadIdx.x] : red[threadIdx.x + h];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && red[0] != 0ull) atomicMax(out, red[0]);
+}
+
+// amrtree constructor, grid.cpp:177-206: floor state and its conserved form
+// in every stored cell
+__global__ __launch_bounds__(TPB) void k_ic_atmosphere(KP P) {
+    const int b = blockIdx.x / WG_CELL;
+    const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
+    if (c >= NCELL) return;
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    const Prim w = floor_prim(fl[c], fl[NCELL + c]);
+    int nf = 0;
+    const Cons u = ptoc(cell_tree(P, b, c), P.gamma, w, nf);
+    storeW(P, b, c, w);
+    storeU(P, b, c, u);
+    add_faults(P, nf);
+}
+// One active zone of fm_init (initconds.cpp:446-470 and 476-589): false where
+// the zone keeps the atmosphere (near the axis or the horizon, or ln h <= 0),
+// else the unnormalised torus density and the metric parts at the zone.
+__device__ __forceinline__ bool fm_zone(const KP& P, const FmPar& f, int b, int i, int j, int k, double& xc,
+                                        double& yc, KsParts& mc, double& rho) {
+    double zc;
+    cell_centre(P, b, i, j, k, xc, yc, zc);
+    double r = sqrt(xc * xc + yc * yc + zc * zc);
+    if (r < 1e-10) r = 1e-10;
+    const double th = P.gth[(size_t)b * NCELL + lindx(i, j, k)];
+    if (!isfinite(r) || !isfinite(th) || r <= 1.05 * f.r_H || fabs(cr_sin(th)) < 0.01) return false;
+    mc = ks_parts(f.M, f.a, f.Q, r, th);
+    const double lnH = fm_lnh(mc, f.l0, f.W_in);
+    if (lnH <= 0.0 || !isfinite(lnH)) return false;
+    rho = fm_rho(lnH, f.Gamma, f.K);
+    return true;
+}
+// the first pass of fm_init: maximum finite torus density (initconds.cpp:446-470)
+__global__ __launch_bounds__(TPB) void k_fm_rho_max(KP P, FmPar f, unsigned long long* __restrict__ out) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    double xc, yc, rho = 0.0;
+    KsParts mc;
+    unsigned long long v = 0ull;
+    if (fm_zone(P, f, b, a / 256, (a / 16) % 16, a % 16, xc, yc, mc, rho) && isfinite(rho) && rho > 0.0)
+        v = (unsigned long long)__double_as_longlong(rho);
+    wg_max_bits(v, out);
+}
+// the second pass of fm_init: torus state, normalised to rho_max = 1, or the
+// atmosphere (initconds.cpp:476-589)
+__global__ __launch_bounds__(TPB) void k_fm_write(KP P, FmPar f) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+    const int c = lindx(i, j, k);
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    Prim W = floor_prim(fl[c], fl[NCELL + c]);
+    double xc, yc, rho;
+    KsParts mc;
+    if (fm_zone(P, f, b, i, j, k, xc, yc, mc, rho)) {
+        W.rho = f.rho_scal * rho;
+        const double pp = f.K * cr_pow(W.rho, f.Gamma);
+        W.eps = pp / ((f.Gamma - 1.0) * W.rho);
+        fm_velocity(mc, f.l0, xc, yc, W.v);
+    }
+    storeW(P, b, c, W);
+}
+// global maximum rho over active zones, main.cpp:101-111 (std::max from 0)
+__global__ __launch_bounds__(TPB) void k_ic_rho_max(KP P, unsigned long long* __restrict__ out) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const double rho = Wa(P, b, WRHO, lindx(a / 256, (a / 16) % 16, a % 16));
+    wg_max_bits(rho > 0.0 ? (unsigned long long)__double_as_longlong(rho) : 0ull, out);
+}
+// B_pot_init's patch rho_max: the global value raised to the patch's active
+// maximum (initconds.cpp:597-606); one workgroup per block
+__global__ __launch_bounds__(TPB) void k_bpot_max(KP P, double glmx, double* __restrict__ pmax) {
+    __shared__ double red[TPB];
+    const int b = blockIdx.x;
+    double m = glmx;
+    for (int a = threadIdx.x; a < NINT; a += TPB) m = smax(m, Wa(P, b, WRHO, lindx(a / 256, (a / 16) % 16, a % 16)));
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int h = TPB / 2; h > 0; h >>= 1) {
+        if ((int)threadIdx.x < h) red[threadIdx.x] = smax(red[threadIdx.x], red[threadIdx.x + h]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) pmax[b] = red[0];
+}
+// Face fields from the curl of A at the edges (initconds.cpp:608-655, quirk:
+// the z-face takes A_y on x-edges and A_x on y-edges) for i, j, k in [-1, 17]
+__global__ __launch_bounds__(TPB) void k_bpot_faces(KP P, const double* __restrict__ pmax) {
+    const int b = blockIdx.x / WG_EDGE;
+    const int n = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
+    if (n >= 19 * 19 * 19) return;
+    const int i = n / 361 - 1, j = (n / 19) % 19 - 1, k = n % 19 - 1;
+    const double rho_max = pmax[b];
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const int top = BLK + GH - 1;
+    auto rho = [&](int ii, int jj, int kk) { return Wa(P, b, WRHO, lindx(ii, jj, kk)); };
+    // edge densities, initconds.cpp:25-39
+    auto rhox = [&](int ii, int jj, int kk) {
+        const int j1 = min(jj + 1, top), k1 = min(kk + 1, top);
+        return (rho(ii, jj, kk) + rho(ii, j1, kk) + rho(ii, jj, k1) + rho(ii, j1, k1)) / 4;
+    };
+    auto rhoy = [&](int ii, int jj, int kk) {
+        const int i1 = min(ii + 1, top), k1 = min(kk + 1, top);
+        return (rho(ii, jj, kk) + rho(i1, jj, kk) + rho(ii, jj, k1) + rho(i1, jj, k1)) / 4;
+    };
+    // A at the x-edge (xc, yc + dy/2, zc + dz/2) and y-edge (xc + dx/2, yc, zc + dz/2)
+    // of cell (ii, jj, kk), initconds.cpp:41-48
+    auto Axe = [&](int ii, int jj, int kk, double& Ax, double& Ay) {
+        double xc, yc, zc;
+        cell_centre(P, b, ii, jj, kk, xc, yc, zc);
+        fm_A_xy(xc, yc + dy / 2.0, zc + dz / 2.0, rhox(ii, jj, kk), rho_max, Ax, Ay);
+    };
+    auto Aye = [&](int ii, int jj, int kk, double& Ax, double& Ay) {
+        double xc, yc, zc;
+        cell_centre(P, b, ii, jj, kk, xc, yc, zc);
+        fm_A_xy(xc + dx / 2.0, yc, zc + dz / 2.0, rhoy(ii, jj, kk), rho_max, Ax, Ay);
+    };
+    double xe_x, xe_y, xem_x, xem_y, xei_x, xei_y;  // x-edges at (i,j,k), (i,j,k-1), (i-1,j,k)
+    double ye_x, ye_y, yem_x, yem_y, yej_x, yej_y;  // y-edges at (i,j,k), (i,j,k-1), (i,j-1,k)
+    Axe(i, j, k, xe_x, xe_y);
+    Axe(i, j, k - 1, xem_x, xem_y);
+    Axe(i - 1, j, k, xei_x, xei_y);
+    Aye(i, j, k, ye_x, ye_y);
+    Aye(i, j, k - 1, yem_x, yem_y);
+    Aye(i, j - 1, k, yej_x, yej_y);
+    const double Az = 0.0;  // A_z on the z-edges
+    double* Bf = P.Bf + (size_t)b * 3 * NFACE;
+    Bf[ixf(i, j, k)] = (Az - Az) / dy - (ye_y - yem_y) / dz;
+    Bf[NFACE + iyf(i, j, k)] = (xe_x - xem_x) / dz - (Az - Az) / dx;
+    Bf[2 * NFACE + izf(i, j, k)] = (xe_y - xei_y) / dx - (ye_x - yej_x) / dy;
+}
+// constrans::f2cB with the cell metric (transport.cpp:60-75) over
+// i, j, k in [-1, 17]: W.B the face average, U.B = sqrt(-g) B
+__global__ __launch_bounds__(TPB) void k_bpot_f2cb(KP P) {
+    const int b = blockIdx.x / WG_EDGE;
+    const int n = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
+    if (n >= 19 * 19 * 19) return;
+    const int i = n / 361 - 1, j = (n / 19) % 19 - 1, k = n % 19 - 1;
+    const int c = lindx(i, j, k);
+    const double* Bf = P.Bf + (size_t)b * 3 * NFACE;
+    const double b0 = (Bf[ixf(i, j, k)] + Bf[ixf(i - 1, j, k)]) / 2;
+    const double b1 = (Bf[NFACE + iyf(i, j, k)] + Bf[NFACE + iyf(i, j - 1, k)]) / 2;
+    const double b2 = (Bf[2 * NFACE + izf(i, j, k)] + Bf[2 * NFACE + izf(i, j, k - 1)]) / 2;
+    const double sg = cell_sqrtdetg(P, b, c);
+    Wa(P, b, WB0, c) = b0; Wa(P, b, WB1, c) = b1; Wa(P, b, WB2, c) = b2;
+    Ua(P, b, UB0, c) = sg * b0; Ua(P, b, UB1, c) = sg * b1; Ua(P, b, UB2, c) = sg * b2;
+}
+// U = ptoc(W) on active zones, main.cpp:148-159
+__global__ __launch_bounds__(TPB) void k_ptoc_interior(KP P) {
+    const int b = blockIdx.x / WG_INT;
+    const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
+    const int c = lindx(a / 256, (a / 16) % 16, a % 16);
+    int nf = 0;
+    storeU(P, b, c, ptoc(cell_tree(P, b, c), P.gamma, loadW(P, b, c), nf));
+    add_faults(P, nf);
+}
+
 // ---------------------------------------------------------------- regrid
 // The flagging loop of amrtree::regrid (grid.cpp:744-795) over the active
 // zones of every leaf: bit 0 of flags[leaf] = some zone asks for refinement
@@ -1293,10 +1476,25 @@ __global__ void k_c2p_sign_batch(int n, double gm, const double* in, int* sign, 
     sign[t] = c2p_f_sign(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
     f[t] = c2p_f(gm, a[0], a[1], a[2], a[3], a[4], a[5]);
 }
+BH_HD double math1(int op, double x) {
+    switch (op) {
+        case 0: return cr_log(x);
+        case 1: return cr_exp(x);
+        case 2: return cr_sin(x);
+        case 3: return cr_cos(x);
+        default: return cr_acos(x);
+    }
+}
+BH_HD double math2(int op, double x, double y) { return op == 0 ? cr_pow(x, y) : cr_atan2(x, y); }
 __global__ void k_math_batch(int op, int n, const double* x, double* y) {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    y[t] = op == 0 ? cr_log(x[t]) : cr_exp(x[t]);
+    y[t] = math1(op, x[t]);
+}
+__global__ void k_math2_batch(int op, int n, const double* x, const double* y, double* out) {
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    out[t] = math2(op, x[t], y[t]);
 }
 
 // ------------------------------------------------------------ device tree
@@ -1330,7 +1528,7 @@ public:
     int limiter = LIM_VANLEER;
     hipStream_t stream = nullptr;
     GhostPlan plan;
-    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz, org;
+    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz, org, gth;
     DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
     DBuf<int> ghtab, leaves, waveblk, lvl;
     DBuf<unsigned char> ghsame;
@@ -1403,6 +1601,7 @@ public:
         P.gamma = gamma; P.limiter = limiter;
         P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p; P.E = E.p;
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
+        P.gth = gth.p;
         P.ghtab = ghtab.p; P.ghsame = ghsame.p;
         P.ghost_order = cell_order.p; P.cell_order = cell_order.p + NGHOST;
         P.dtbits = scal.p; P.faults = scal.p + 1;
@@ -1485,6 +1684,7 @@ public:
         gfc.alloc(g.face.size()); gfc.upload(g.face.data(), g.face.size(), stream);
         gchr.alloc(g.chr.size()); gchr.upload(g.chr.data(), g.chr.size(), stream);
         dxyz.alloc(g.dxyz.size()); dxyz.upload(g.dxyz.data(), g.dxyz.size(), stream);
+        gth.alloc(g.th.size()); gth.upload(g.th.data(), g.th.size(), stream);
         {
             std::vector<double> o((size_t)nb * 3);
             for (int b = 0; b < nb; b++) {
@@ -1600,6 +1800,7 @@ public:
         relayout(gfc, (size_t)3 * NGFC * NFC, nb_new, mo.p, mn.p, &g.face);
         relayout(gchr, (size_t)NCHR * NINT, nb_new, mo.p, mn.p, &g.chr);
         relayout(dxyz, 3, nb_new, mo.p, mn.p, &g.dxyz);
+        relayout(gth, (size_t)NCELL, nb_new, mo.p, mn.p, &g.th);
         relayout(org, 3, nb_new, mo.p, mn.p, &o);
         nb = nb_new;
         alloc_scratch();
@@ -1765,6 +1966,64 @@ public:
         double dt;
         std::memcpy(&dt, &bits, sizeof dt);
         return dt;
+    }
+    // The initial data of main.cpp:94-159 written on the device over every
+    // block: floor state, fm_init, global rho_max, B_pot_init, ptoc of the
+    // active zones.  Returns 0, 1 where fm_init's L0_comp has a vanishing
+    // denominator (nothing written), 2 where fm_init places no torus (the
+    // rest is done), 3 where the global rho_max is below 1e-14 (main.cpp
+    // stops there; floor state and torus written).
+    int init_torus(const TreeParams& prm, double& rho_max, std::string& msg) {
+        FmScalars s;
+        int fm = fm_scalars(prm.M, prm.a, prm.Q, prm.gamma, s);
+        if (fm == 1) {
+            msg = "LNRF angular momentum: denominator is zero (r_max inside the ergosphere)";
+            return 1;
+        }
+        KP P = kp();
+        HIPCHK(hipMemsetAsync(Bf.p, 0, Bf.n * sizeof(double), stream));
+        launch("k_ic_atmosphere", k_ic_atmosphere, dim3(nb * WG_CELL), P);
+        DBuf<unsigned long long> red;
+        red.alloc(2);
+        HIPCHK(hipMemsetAsync(red.p, 0, 2 * sizeof(unsigned long long), stream));
+        unsigned long long bits[2] = {0ull, 0ull};
+        FmPar f{prm.M, prm.a, prm.Q, prm.gamma, s.K, s.l0, s.W_in, s.r_H, 0.0};
+        if (fm == 0) {
+            launch("k_fm_rho_max", k_fm_rho_max, dim3(nb * WG_INT), P, f, red.p);
+            red.download(bits, 1, stream);
+            HIPCHK(hipStreamSynchronize(stream));
+            double fm_max;
+            std::memcpy(&fm_max, &bits[0], sizeof fm_max);
+            if (fm_max < 1e-14) {
+                fm = 2;
+                msg = "no torus cells found";
+            } else {
+                f.rho_scal = 1.0 / fm_max;
+                launch("k_fm_write", k_fm_write, dim3(nb * WG_INT), P, f);
+            }
+        } else {
+            msg = "ln(h) <= 0 at r = r_max: the torus cannot be initialised";
+        }
+        launch("k_ic_rho_max", k_ic_rho_max, dim3(nb * WG_INT), P, red.p + 1);
+        red.download(bits, 2, stream);
+        HIPCHK(hipStreamSynchronize(stream));
+        std::memcpy(&rho_max, &bits[1], sizeof rho_max);
+        if (rho_max < 1e-14) {
+            msg = "rho_max is zero after the torus initialisation";
+            return 3;
+        }
+        // B_pot_init returns at once for rho_max <= 1e-10 (initconds.cpp:597-599)
+        if (rho_max > 1e-10) {
+            DBuf<double> pmax;
+            pmax.alloc(nb);
+            hipLaunchKernelGGL(k_bpot_max, dim3(nb), dim3(TPB), 0, stream, P, rho_max, pmax.p);
+            launch("k_bpot_faces", k_bpot_faces, dim3(nb * WG_EDGE), P, (const double*)pmax.p);
+            launch("k_bpot_f2cb", k_bpot_f2cb, dim3(nb * WG_EDGE), P);
+            HIPCHK(hipStreamSynchronize(stream));
+        }
+        launch("k_ptoc_interior", k_ptoc_interior, dim3(nb * WG_INT), P);
+        sync();
+        return fm;
     }
     long long take_faults() {
         unsigned long long f = 0;
@@ -1946,6 +2205,27 @@ int bh_tree_regrid_device(bh_tree* t, int* changed) {
         t->dev_valid = true;
         t->layout_dirty = false;
         return finish_faults(t, t->dev->take_faults());
+    });
+}
+int bh_tree_init_torus_device(bh_tree* t, double* rho_max) {
+    return guarded(t, [&] {
+        if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+        if (t->layout_dirty) {
+            t->dev->build(*t->host);
+            t->layout_dirty = false;
+        }
+        double rm = 0.0;
+        std::string msg;
+        const int rc = t->dev->init_torus(t->host->prm, rm, msg);
+        if (rc == 1) return fail(t, BH_ESTATE, msg);
+        // the device now holds the whole state
+        device_changed(t);
+        if (rho_max) *rho_max = rm;
+        if (rc == 3) return fail(t, BH_ESTATE, msg);
+        const long long nf = t->dev->take_faults();
+        if (nf > 0) return finish_faults(t, nf);
+        if (rc == 2) { t->err = msg; return 2; }
+        return BH_OK;
     });
 }
 int bh_tree_nblocks(const bh_tree* t, int* nb) {
@@ -2263,9 +2543,9 @@ int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* 
     }
 }
 int bh_math_batch(int op, int n, const double* x, double* y, int on_device) {
-    if (n < 0 || (op != 0 && op != 1)) return BH_EINVAL;
+    if (n < 0 || op < 0 || op > 4) return BH_EINVAL;
     if (!on_device) {
-        for (int i = 0; i < n; i++) y[i] = op == 0 ? cr_log(x[i]) : cr_exp(x[i]);
+        for (int i = 0; i < n; i++) y[i] = math1(op, x[i]);
         return BH_OK;
     }
     try {
@@ -2275,6 +2555,25 @@ int bh_math_batch(int op, int n, const double* x, double* y, int on_device) {
         if (n) hipLaunchKernelGGL(k_math_batch, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpy(y, dy, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+        return BH_OK;
+    } catch (const DevError& e) {
+        return fail(nullptr, BH_EDEVICE, e.msg);
+    }
+}
+int bh_math2_batch(int op, int n, const double* x, const double* y, double* out, int on_device) {
+    if (n < 0 || op < 0 || op > 1) return BH_EINVAL;
+    if (!on_device) {
+        for (int i = 0; i < n; i++) out[i] = math2(op, x[i], y[i]);
+        return BH_OK;
+    }
+    try {
+        Scratch s;
+        double* dx = s.put(x, n);
+        double* dy = s.put(y, n);
+        double* dout = s.put<double>(nullptr, n);
+        if (n) hipLaunchKernelGGL(k_math2_batch, dim3((n + 255) / 256), dim3(256), 0, 0, op, n, dx, dy, dout);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpy(out, dout, (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
         return BH_OK;
     } catch (const DevError& e) {
         return fail(nullptr, BH_EDEVICE, e.msg);

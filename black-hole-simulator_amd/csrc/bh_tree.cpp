@@ -756,6 +756,7 @@ void build_geometry_blocks(const HostTree& t, const std::vector<int>& blocks, Ge
     g.face.assign((size_t)nb * 3 * NGFC * NFC, 0.0);
     g.chr.assign((size_t)nb * NCHR * NINT, 0.0);
     g.dxyz.assign((size_t)nb * 3, 0.0);
+    g.th.assign((size_t)nb * NCELL, 0.0);
     int bad_chr = 0;
 #pragma omp parallel for schedule(dynamic) reduction(| : bad_chr)
     for (int b = 0; b < nb; b++) {
@@ -775,6 +776,7 @@ void build_geometry_blocks(const HostTree& t, const std::vector<int>& blocks, Ge
             Prim f = t.pvfs(p.r[c]);
             fl[c] = f.rho;
             fl[NCELL + c] = f.eps;
+            g.th[(size_t)b * NCELL + c] = p.th[c];
         }
         // faces: (r, theta) of a face is the mean of the two cells' (grid.cpp:266-269)
         for (int dir = 0; dir < 3; dir++) {

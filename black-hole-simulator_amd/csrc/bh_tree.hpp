@@ -121,6 +121,7 @@ struct GeomHost {
     std::vector<double> face;       // [nb][3][NGFC][4352]
     std::vector<double> chr;        // [nb][NCHR][4096] the non-zero lowered Christoffels
     std::vector<double> dxyz;       // [nb][3]
+    std::vector<double> th;         // [nb][8000] polar angle of each cell centre (patch::th)
 };
 void build_geometry(const HostTree& t, GeomHost& g);
 // the same tables for the listed blocks only, packed in list order

@@ -61,6 +61,20 @@ int bh_tree_regrid(bh_tree* t);
  * identical to bh_tree_regrid.  *changed (may be NULL) = blocks refined plus
  * parents flattened. */
 int bh_tree_regrid_device(bh_tree* t, int* changed);
+/* The whole initial data of src/fluid/main.cpp:94-159 computed on the GPU, the
+ * state left on the device: the floor state of grid::amrtree's constructor
+ * (grid.cpp:177-206) in every stored cell of every block, fm_init
+ * (solver/initconds.cpp:228-591), the global rho_max over active zones
+ * (main.cpp:101-111), B_pot_init on every patch (initconds.cpp:594-668) and
+ * U = ptoc(W) on active zones (main.cpp:148-159).  For a tree fresh from
+ * bh_tree_create this is the sequence bh_tree_fm_init, bh_tree_rho_max,
+ * bh_tree_B_pot_init, bh_tree_ptoc_interior, with the initial data's
+ * transcendental functions (sin, cos, pow, exp, log, atan2, acos) correctly
+ * rounded instead of the C library's.  *rho_max (may be NULL) = the global
+ * rho_max.  Returns 0; 2 where fm_init places no torus (the rest is done, as
+ * main.cpp continues); BH_ESTATE where L0_comp's denominator vanishes (nothing
+ * written) or rho_max < 1e-14 (main.cpp stops there). */
+int bh_tree_init_torus_device(bh_tree* t, double* rho_max);
 
 int bh_tree_nblocks(const bh_tree* t, int* nb);
 int bh_tree_get_blocks(const bh_tree* t, int* meta, double* edges);
@@ -128,8 +142,12 @@ int bh_ctop_batch(double M, double a, double Q, double gm, int n, const double* 
 int bh_hlld_batch(double M, double a, double Q, double gm, int n, const double* r, const double* th, int dim,
                   const double* WL, const double* WR, const double* UL, const double* UR, double* F,
                   double* ctspeed);
-/* op 0: log, 1: exp, correctly rounded; on_device 0 evaluates the same code on the host. */
+/* Correctly rounded functions of the step and the initial data, for parity tests (no
+ * reference counterpart): op 0 log, 1 exp, 2 sin, 3 cos, 4 acos of x[i]; on_device 0
+ * evaluates the same code on the host. */
 int bh_math_batch(int op, int n, const double* x, double* y, int on_device);
+/* op 0: out[i] = pow(x[i], y[i]); op 1: out[i] = atan2(x[i], y[i]); correctly rounded. */
+int bh_math2_batch(int op, int n, const double* x, const double* y, double* out, int on_device);
 /* The con2prim energy residual (conserved::fres, src/fluid/physics/conserved.cpp:80-123)
  * at in[i] = (xi, Ssq, Bsq, SB, D, tau): f[i] = the residual, sign[i] = its sign as
  * proved by the division-free evaluation the Brent and bracket loops use (+1, -1, or 0

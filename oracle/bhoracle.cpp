@@ -1164,6 +1164,298 @@ double step(Tree& t, double cfl) {
     return dt;
 }
 
+// ---------------------------------------------------------- initial data
+// The set-up of main.cpp:94-159: amrtree's constructor state, fm_init, the
+// global rho_max, B_pot_init on every patch, ptoc of the active zones.  The
+// functions the initial data calls (initconds.cpp, and metric::comp inside it)
+// come from IcLibm: mode 0 = the C library (the reference), mode 1 = correctly
+// rounded (binary128 rounded once; pow's exact cases y = 2 and y = -1 are the
+// IEEE operations x*x and 1/x, which are correctly rounded, as in mode 0).  The geometry the
+// tree already holds (cell r, theta, the floor state, the metric inside ptoc
+// and f2cB) is the C library's in both modes.
+extern "C" __float128 sinq(__float128);
+extern "C" __float128 cosq(__float128);
+extern "C" __float128 powq(__float128, __float128);
+extern "C" __float128 atan2q(__float128, __float128);
+extern "C" __float128 acosq(__float128);
+double crq_sin(double x) { return (double)sinq((__float128)x); }
+double crq_cos(double x) { return (double)cosq((__float128)x); }
+double crq_pow(double x, double y) {
+    if (y == 2.0) return x * x;
+    if (y == -1.0) return 1.0 / x;
+    return (double)powq((__float128)x, (__float128)y);
+}
+// the C library's pow as the reference's compiled calls see it: GCC and
+// clang fold pow(x, 2.0) into x*x and pow(x, -1.0) into 1.0/x (both exact
+// rewrites) wherever the exponent is a literal, which covers every such
+// exponent of the initial data
+double lib_pow(double x, double y) {
+    if (y == 2.0) return x * x;
+    if (y == -1.0) return 1.0 / x;
+    return ::pow(x, y);
+}
+double crq_atan2(double y, double x) { return (double)atan2q((__float128)y, (__float128)x); }
+double crq_acos(double x) { return (double)acosq((__float128)x); }
+struct IcLibm {
+    double (*sin)(double);
+    double (*cos)(double);
+    double (*pow)(double, double);
+    double (*atan2)(double, double);
+    double (*acos)(double);
+    double (*exp)(double);
+    double (*log)(double);
+};
+IcLibm ic_libm_for(int mode) {
+    typedef double (*F1)(double);
+    typedef double (*F2)(double, double);
+    if (mode == 1) return {crq_sin, crq_cos, crq_pow, crq_atan2, crq_acos, cr_exp, cr_log};
+    return {static_cast<F1>(::sin), static_cast<F1>(::cos), lib_pow, static_cast<F2>(::atan2),
+            static_cast<F1>(::acos), static_cast<F1>(::exp), static_cast<F1>(::log)};
+}
+
+// the parts of metric::comp the torus reads: g_tt, g_tphi, g_phiphi
+// (gcova, spacetime.cpp:20-34), alpha (lapse, 93-95) and beta^phi (shift,
+// 96-100, with g^{t phi} = 0 from gcntr, 35-49)
+struct KsIc { double g00, g03, g33, alpha, beta2; };
+KsIc ks_ic(const IcLibm& L, const KS& k, double r, double th) {
+    const double M = k.M, a = k.a, Q = k.Q;
+    auto Sigma = [&]() { return r * r + a * a * L.pow(L.cos(th), 2); };
+    auto H = [&]() { return (2 * M * r - Q * Q) * L.pow(Sigma(), -1); };
+    KsIc m;
+    m.g00 = -1 + H();
+    m.g03 = -H() * a * L.pow(L.sin(th), 2);
+    m.g33 = (r * r + a * a + H() * a * a * L.pow(L.sin(th), 2)) * L.pow(L.sin(th), 2);
+    m.alpha = L.pow(1 + H(), -1.0 / 2.0);
+    const double gi00 = -1 - H(), gi03 = 0.0;
+    m.beta2 = -gi03 / gi00;
+    return m;
+}
+// L0_comp, initconds.cpp:121-141; false where the reference throws
+bool ic_L0(const IcLibm& L, const KS& ks, double r_max, double M, double& l0) {
+    const double dr = 1e-4 * M;
+    const KsIc mc0 = ks_ic(L, ks, r_max, M_PI / 2.0);
+    const KsIc mcp = ks_ic(L, ks, r_max + dr, M_PI / 2.0);
+    const KsIc mcm = ks_ic(L, ks, r_max - dr, M_PI / 2.0);
+    double gtt = mc0.g00, gtp = mc0.g03, gpp = mc0.g33;
+    const double dgtt = (mcp.g00 - mcm.g00) / (2.0 * dr);
+    const double dgtp = (mcp.g03 - mcm.g03) / (2.0 * dr);
+    const double dgpp = (mcp.g33 - mcm.g33) / (2.0 * dr);
+    double A = -gtp * (gpp * (dgtt * dgpp + 2 * dgtp * dgtp) + gtt * dgpp * dgpp) +
+               std::sqrt((dgtp * dgtp - dgtt * dgpp) *
+                         L.pow((gpp * (gpp * dgtt - gtt * dgpp) + 2 * gtp * gtp * dgpp - 2 * gtp * gpp * dgtp), 2)) +
+               gpp * gpp * dgtt * dgtp + gtt * gpp * dgtp * dgpp + 2 * gtp * gtp * dgtp * dgpp;
+    double B = dgtt * (gpp * gpp * dgtt + 4 * gtp * gtp * dgpp - 4 * gtp * gpp * dgtp) -
+               2 * gtt * (gpp * (dgtt * dgpp - 2 * dgtp * dgtp) + 2 * gtp * dgtp * dgpp) + gtt * gtt * dgpp * dgpp;
+    if (std::abs(B) < 1e-14) return false;
+    l0 = A / B;
+    return true;
+}
+// lnrf_, lnh_pot, W_in_, initconds.cpp:151-183
+struct IcLnrf { double e2nu, e2psi, enu, epsi, omega; };
+IcLnrf ic_lnrf(const KsIc& mc) {
+    IcLnrf lp;
+    lp.e2nu = mc.g03 * mc.g03 / mc.g33 - mc.g00;
+    lp.e2psi = mc.g33;
+    lp.enu = std::sqrt(lp.e2nu);
+    lp.epsi = std::sqrt(lp.e2psi);
+    lp.omega = -mc.g03 / mc.g33;
+    return lp;
+}
+double ic_lnh(const IcLibm& L, const KsIc& mc, double l, double W_in) {
+    const IcLnrf lp = ic_lnrf(mc);
+    if (lp.e2nu <= 0.0) return -1e30;
+    return -0.5 * std::sqrt((4 * l * l * lp.e2nu) / lp.e2psi + 1) +
+           0.5 * L.log((std::sqrt((4 * l * l * lp.e2nu) / lp.e2psi + 1) + 1) / lp.e2nu) - l * lp.omega + W_in;
+}
+double ic_W_in(const IcLibm& L, const KsIc& mc, double l) {
+    const IcLnrf lp = ic_lnrf(mc);
+    if (lp.e2nu <= 0.0) return -1e30;
+    return 0.5 * std::sqrt((4 * l * l * lp.e2nu) / lp.e2psi + 1) -
+           0.5 * L.log((std::sqrt((4 * l * l * lp.e2nu) / lp.e2psi + 1) + 1) / lp.e2nu) + l * lp.omega;
+}
+// disctherm, initconds.cpp:186-203
+struct IcTherm { double rho, p, eps; };
+IcTherm ic_therm(const IcLibm& L, double lnH, double Gamma, double K) {
+    IcTherm th{0.0, 0.0, 0.0};
+    if (lnH <= 0.0) return th;
+    const double H = L.exp(lnH);
+    const double arg = (Gamma - 1) * (H - 1) / (Gamma * K);
+    if (arg <= 0.0) return th;
+    th.rho = L.pow(arg, 1.0 / (Gamma - 1.0));
+    th.p = K * L.pow(arg, Gamma / (Gamma - 1.0));
+    th.eps = th.p / ((Gamma - 1.0) * th.rho);
+    return th;
+}
+// velcomp, initconds.cpp:206-225
+void ic_vel(const KsIc& mc, const IcLnrf& lp, double l, double xc, double yc, double v[3]) {
+    v[0] = v[1] = v[2] = 0.0;
+    const double u_phi = std::sqrt(0.5 * (std::sqrt(1.0 + 4.0 * l * l * lp.e2nu / lp.e2psi) - 1.0));
+    const double uphi = u_phi / lp.epsi + (lp.omega / lp.enu) * std::sqrt(1.0 + u_phi * u_phi);
+    const double gtt = mc.g00, gtp = mc.g03, gpp = mc.g33;
+    const double B = 2.0 * gtp * uphi;
+    const double C = gpp * uphi * uphi + 1.0;
+    const double D = B * B - 4.0 * gtt * C;
+    if (D < 0.0) return;
+    const double ut = (-B - std::sqrt(D)) / (2.0 * gtt);
+    const double Omega = uphi / ut;
+    const double vphi = Omega - mc.beta2 / mc.alpha;
+    const double rcyl = std::sqrt(xc * xc + yc * yc);
+    if (rcyl < 1e-10) return;
+    v[0] = -vphi * yc;
+    v[1] = vphi * xc;
+}
+// fm_init, initconds.cpp:228-591 (live code).  0 = placed, 1 = L0_comp
+// throws, 2 = the reference prints an error and returns.
+int ic_fm_init(Tree& t, const IcLibm& L) {
+    const double M = t.ks.M, a = t.ks.a, Q = t.ks.Q, Gamma = t.gamma;
+    const double r_max = 12.0 * M;
+    const double r_in = 4.5 * M;
+    double r_H = M + std::sqrt(M * M - a * a - Q * Q);
+    double K = 0.001 * L.pow(M, Gamma - 1.0);
+    double rho_tgt = 1.0;
+    double l0;
+    if (!ic_L0(L, t.ks, r_max, M, l0)) return 1;
+    const double W_in = ic_W_in(L, ks_ic(L, t.ks, r_in, M_PI / 2.0), l0);
+    const double lnh_max = ic_lnh(L, ks_ic(L, t.ks, r_max, M_PI / 2.0), l0, W_in);
+    if (lnh_max <= 0.0) return 2;
+    EOS st = t.eos();
+    auto skip = [&](double r, double th) {
+        return !std::isfinite(r) || !std::isfinite(th) || r <= 1.05 * r_H || std::abs(L.sin(th)) < 0.01;
+    };
+    double rho_max = 0.0;
+    for (int b = 0; b < t.nb; b++)
+        for (int i = 0; i < BLK; i++)
+            for (int j = 0; j < BLK; j++)
+                for (int k = 0; k < BLK; k++) {
+                    const Geo& g = t.gc(b, lindx(i, j, k));
+                    if (skip(g.r, g.th)) continue;
+                    const KsIc mc = ks_ic(L, t.ks, g.r, g.th);
+                    const double lnH = ic_lnh(L, mc, l0, W_in);
+                    if (lnH <= 0.0 || !std::isfinite(lnH)) continue;
+                    const IcTherm thm = ic_therm(L, lnH, Gamma, K);
+                    if (std::isfinite(thm.rho)) rho_max = smax(rho_max, thm.rho);
+                }
+    if (rho_max < 1e-14) return 2;
+    double rho_scal = rho_tgt / rho_max;
+    for (int b = 0; b < t.nb; b++)
+        for (int i = 0; i < BLK; i++)
+            for (int j = 0; j < BLK; j++)
+                for (int k = 0; k < BLK; k++) {
+                    const int c = lindx(i, j, k);
+                    const Geo& g = t.gc(b, c);
+                    Prim W_p = pvfs(st, g.r);
+                    if (skip(g.r, g.th)) { t.setW(b, c, W_p); continue; }
+                    const KsIc mc = ks_ic(L, t.ks, g.r, g.th);
+                    const double lnH = ic_lnh(L, mc, l0, W_in);
+                    if (lnH <= 0.0 || !std::isfinite(lnH)) { t.setW(b, c, W_p); continue; }
+                    const IcTherm thm = ic_therm(L, lnH, Gamma, K);
+                    W_p.rho = rho_scal * thm.rho;
+                    W_p.p = K * L.pow(W_p.rho, Gamma);
+                    W_p.eps = W_p.p / ((Gamma - 1.0) * W_p.rho);
+                    ic_vel(mc, ic_lnrf(mc), l0, g.xc, g.yc, W_p.v);
+                    t.setW(b, c, W_p);
+                }
+    return 0;
+}
+// A_vpot, initconds.cpp:17-23
+void ic_A(const IcLibm& L, double x, double y, double z, double rho, double rho_max, double A[3]) {
+    double Ap = smax(rho / rho_max - 0.2, 0.0);
+    double phi = L.atan2(y, x);
+    double theta = L.acos(z / std::sqrt(x * x + y * y + z * z));
+    A[0] = -Ap * L.sin(phi) * L.sin(theta);
+    A[1] = Ap * L.cos(phi) * L.sin(theta);
+    A[2] = 0.0;
+}
+// B_pot_init, initconds.cpp:594-668 (edge densities 25-39, edge positions
+// 41-52; quirk: Bfz takes A_y on x-edges and A_x on y-edges), then f2cB
+void ic_B_pot(Tree& t, const IcLibm& L, int p, double glmx_rho) {
+    const Block& B = t.blk[p];
+    const double dx = B.dx(), dy = B.dy(), dz = B.dz();
+    double rho_max = glmx_rho;
+    if (rho_max <= 1e-10) return;
+    for (int i = 0; i < BLK; i++)
+        for (int j = 0; j < BLK; j++)
+            for (int k = 0; k < BLK; k++) rho_max = smax(rho_max, t.w(p, 0, lindx(i, j, k)));
+    const int top = BLK + GH - 1;
+    auto rho = [&](int i, int j, int k) { return t.w(p, 0, lindx(i, j, k)); };
+    auto erx = [&](int i, int j, int k) {
+        int j1 = std::min(j + 1, top), k1 = std::min(k + 1, top);
+        return (rho(i, j, k) + rho(i, j1, k) + rho(i, j, k1) + rho(i, j1, k1)) / 4;
+    };
+    auto ery = [&](int i, int j, int k) {
+        int i1 = std::min(i + 1, top), k1 = std::min(k + 1, top);
+        return (rho(i, j, k) + rho(i1, j, k) + rho(i, j, k1) + rho(i1, j, k1)) / 4;
+    };
+    auto erz = [&](int i, int j, int k) {
+        int i1 = std::min(i + 1, top), j1 = std::min(j + 1, top);
+        return (rho(i, j, k) + rho(i1, j, k) + rho(i, j1, k) + rho(i1, j1, k)) / 4;
+    };
+    // A at an x-, y- or z-edge of cell (i, j, k), component comp
+    auto Ax = [&](int comp, int i, int j, int k) {
+        const Geo& g = t.gc(p, lindx(i, j, k));
+        double A[3];
+        ic_A(L, g.xc, g.yc + dy / 2.0, g.zc + dz / 2.0, erx(i, j, k), rho_max, A);
+        return A[comp];
+    };
+    auto Ay = [&](int comp, int i, int j, int k) {
+        const Geo& g = t.gc(p, lindx(i, j, k));
+        double A[3];
+        ic_A(L, g.xc + dx / 2.0, g.yc, g.zc + dz / 2.0, ery(i, j, k), rho_max, A);
+        return A[comp];
+    };
+    auto Az = [&](int comp, int i, int j, int k) {
+        const Geo& g = t.gc(p, lindx(i, j, k));
+        double A[3];
+        ic_A(L, g.xc + dx / 2.0, g.yc + dy / 2.0, g.zc, erz(i, j, k), rho_max, A);
+        return A[comp];
+    };
+    double *Bx = t.bfx(p), *By = t.bfy(p), *Bz = t.bfz(p);
+    for (int i = -GH + 1; i < BLK + GH; i++)
+        for (int j = -GH + 1; j < BLK + GH; j++)
+            for (int k = -GH + 1; k < BLK + GH; k++) {
+                Bx[ixf(i, j, k)] = (Az(2, i, j, k) - Az(2, i, j - 1, k)) / dy - (Ay(1, i, j, k) - Ay(1, i, j, k - 1)) / dz;
+                By[iyf(i, j, k)] = (Ax(0, i, j, k) - Ax(0, i, j, k - 1)) / dz - (Az(2, i, j, k) - Az(2, i - 1, j, k)) / dx;
+                Bz[izf(i, j, k)] = (Ax(1, i, j, k) - Ax(1, i - 1, j, k)) / dx - (Ay(0, i, j, k) - Ay(0, i, j - 1, k)) / dy;
+            }
+    f2cB(t, p);
+}
+// main.cpp:94-159 after amrtree's constructor (grid.cpp:177-206).  Returns
+// 0, 1 (L0_comp throws; nothing after the constructor), 2 (no torus placed),
+// 3 (global rho_max below 1e-14: main returns).
+int init_torus(Tree& t, int mode, double& rho_max) {
+    const IcLibm L = ic_libm_for(mode);
+    EOS st = t.eos();
+    std::fill(t.U.begin(), t.U.end(), 0.0);
+    std::fill(t.W.begin(), t.W.end(), 0.0);
+    std::fill(t.Bf.begin(), t.Bf.end(), 0.0);
+    for (int b = 0; b < t.nb; b++)
+        for (int c = 0; c < NCELL; c++) {
+            const Geo& g = t.gc(b, c);
+            Prim w = pvfs(st, g.r);
+            t.setW(b, c, w);
+            t.setU(b, c, ptoc(t.ks, st, w, g.r, g.th));
+        }
+    rho_max = 0.0;
+    const int fm = ic_fm_init(t, L);
+    if (fm == 1) return 1;
+    for (int b = 0; b < t.nb; b++)
+        for (int i = 0; i < BLK; i++)
+            for (int j = 0; j < BLK; j++)
+                for (int k = 0; k < BLK; k++) rho_max = std::max(rho_max, t.w(b, 0, lindx(i, j, k)));
+    if (rho_max < 1e-14) return 3;
+    for (int b = 0; b < t.nb; b++) ic_B_pot(t, L, b, rho_max);
+    for (int b = 0; b < t.nb; b++)
+        for (int i = 0; i < BLK; i++)
+            for (int j = 0; j < BLK; j++)
+                for (int k = 0; k < BLK; k++) {
+                    const int c = lindx(i, j, k);
+                    const Geo& g = t.gc(b, c);
+                    Prim w = t.getW(b, c);
+                    t.setU(b, c, ptoc(t.ks, st, w, g.r, g.th));
+                }
+    return fm;
+}
+
 void put_prim(const Prim& w, double* o) {
     o[0] = w.rho; o[1] = w.eps;
     for (int d = 0; d < 3; d++) { o[2 + d] = w.v[d]; o[5 + d] = w.B[d]; }
@@ -1222,6 +1514,12 @@ void* or_tree_new(int nb, const int* meta, const double* edges, double M, double
     return t;
 }
 void or_tree_free(void* h) { delete static_cast<Tree*>(h); }
+int or_init_torus(void* h, int libm_mode, double* rho_max) {
+    double rm = 0.0;
+    int rc = init_torus(*static_cast<Tree*>(h), libm_mode, rm);
+    if (rho_max) *rho_max = rm;
+    return rc;
+}
 void or_set_state(void* h, const double* U, const double* W, const double* Bf, const double* dU) {
     Tree* t = static_cast<Tree*>(h);
     if (U) std::memcpy(t->U.data(), U, t->U.size() * sizeof(double));
@@ -1411,4 +1709,16 @@ void or_reconfp(void* h, int b, int i, int j, int k, int dim, int lim, double* L
 }
 double or_log(double x, int mode) { return libm_for(mode).log(x); }
 double or_exp(double x, int mode) { return libm_for(mode).exp(x); }
+double or_ic_fn(int op, double x, double y, int mode) {
+    const IcLibm L = ic_libm_for(mode);
+    switch (op) {
+        case 0: return L.log(x);
+        case 1: return L.exp(x);
+        case 2: return L.sin(x);
+        case 3: return L.cos(x);
+        case 4: return L.acos(x);
+        case 5: return L.pow(x, y);
+        default: return L.atan2(x, y);
+    }
+}
 }

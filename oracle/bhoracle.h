@@ -25,6 +25,13 @@ extern "C" {
 void* or_tree_new(int nb, const int* meta, const double* edges, double M, double a, double Q, double gm,
                   int libm_mode);
 void or_tree_free(void* t);
+/* The initial data of main.cpp:94-159 on a fresh tree: amrtree's constructor
+ * state (grid.cpp:177-206), fm_init, global rho_max, B_pot_init, ptoc of the
+ * active zones.  libm_mode chooses the functions the initial data calls
+ * (0 = C library, 1 = correctly rounded); the geometry is the C library's.
+ * Returns 0, 1 (L0_comp throws), 2 (no torus placed), 3 (rho_max < 1e-14);
+ * faults counted as in or_step. */
+int or_init_torus(void* t, int libm_mode, double* rho_max);
 void or_set_state(void* t, const double* U, const double* W, const double* Bf, const double* dU);
 void or_get_state(void* t, double* U, double* W, double* Bf, double* dU);
 /* Each returns the number of equation-of-state calls that saw rho <= 0 (the
@@ -55,6 +62,9 @@ int or_hlld(double M, double a, double Q, double gm, const double* WL, const dou
 void or_reconfp(void* t, int b, int i, int j, int k, int dim, int lim, double* L, double* R);
 double or_log(double x, int libm_mode);
 double or_exp(double x, int libm_mode);
+/* the initial data's functions: op 0 log, 1 exp, 2 sin, 3 cos, 4 acos of x,
+ * 5 pow(x, y), 6 atan2(x, y); mode as or_init_torus */
+double or_ic_fn(int op, double x, double y, int libm_mode);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,7 @@ def oracle_lib():
         L = ctypes.CDLL(str(ORACLE_SO))
         _sig(L, "or_tree_new", _v, [_i, _ip, _dp, _d, _d, _d, _d, _i])
         _sig(L, "or_tree_free", None, [_v])
+        _sig(L, "or_init_torus", _i, [_v, _i, _dp])
         _sig(L, "or_set_state", None, [_v, _dp, _dp, _dp, _dp])
         _sig(L, "or_get_state", None, [_v, _dp, _dp, _dp, _dp])
         _sig(L, "or_step", _ll, [_v, _d, _dp])
@@ -83,6 +84,7 @@ def oracle_lib():
         _sig(L, "or_reconfp", None, [_v, _i, _i, _i, _i, _i, _i, _dp, _dp])
         _sig(L, "or_log", _d, [_d, _i])
         _sig(L, "or_exp", _d, [_d, _i])
+        _sig(L, "or_ic_fn", _d, [_i, _d, _d, _i])
         _oracle = L
     return _oracle
 
@@ -263,6 +265,14 @@ class OracleTree:
         dU = np.zeros((nb, 8, NCELL)) if with_dU else None
         self.L.or_get_state(self.h, _p(U), _p(W), _p(Bf), _p(dU))
         return State(self.meta, self.edges, U, W, Bf, dU)
+
+    def init_torus(self, libm=0):
+        """The initial data of main.cpp:94-159 on this (fresh) block table;
+        libm 0 = C library, 1 = correctly rounded.  Returns (code, rho_max)
+        with code 0, 1 (L0_comp throws), 2 (no torus), 3 (rho_max < 1e-14)."""
+        rm = _d(0.0)
+        rc = self.L.or_init_torus(self.h, int(libm), ctypes.byref(rm))
+        return rc, rm.value
 
     def step(self, cfl=0.4):
         dt = _d(0.0)

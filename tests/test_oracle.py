@@ -19,6 +19,7 @@ import numpy as np
 import pytest
 
 import configs
+from ic_cases import ic_math_inputs
 from oracle_libs import (OracleTree, RefTree, State, bits_equal, first_diff, oracle_lib, ref_available)
 
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -368,3 +369,98 @@ def test_correctly_rounded_math_on_host():
         g = np.array([O.or_log(v, 0) if op == 0 else O.or_exp(v, 0) for v in x])
         # the C library is within half an ulp except on rare near-ties
         assert (g != q).mean() < 5e-3
+
+
+# ------------------------------------------------------------ initial data
+def test_initial_data_math_oracle_is_correctly_rounded():
+    """The oracle's correctly rounded sin, cos, acos, pow, atan2 (binary128
+    rounded once) against 300-bit mpmath, independent of both the C library
+    and the product; the C library agrees except on rare near-ties."""
+    mpmath = pytest.importorskip("mpmath")
+    mpmath.mp.prec = 300
+    O = oracle_lib()
+    th, c, px, py, ay, ax = (v[::6] for v in ic_math_inputs(np.random.default_rng(5)))
+    cases = [(2, th, th, mpmath.sin), (3, th, th, mpmath.cos), (4, c, c, mpmath.acos),
+             (5, px, py, mpmath.power), (6, ay, ax, mpmath.atan2)]
+    for op, x, y, f in cases:
+        for a, b in zip(x, y):
+            want = float(f(mpmath.mpf(float(a))) if op < 5 else f(mpmath.mpf(float(a)), mpmath.mpf(float(b))))
+            if op in (2, 6) and a == 0.0:
+                want = float(np.sin(a)) if op == 2 else float(np.arctan2(a, b))  # signed zeros
+            got = O.or_ic_fn(op, a, b, 1)
+            assert got == want and np.signbit(got) == np.signbit(want), (op, a.hex(), b.hex(), got, want)
+
+
+def test_initial_data_math_product_matches_oracle_on_host():
+    """The product's correctly rounded functions (the device's source,
+    evaluated on the host) equal the oracle's correctly rounded mode bit for
+    bit, signed zeros included."""
+    import bhfluid
+
+    L, O = bhfluid.lib(), oracle_lib()
+    th, c, px, py, ay, ax = ic_math_inputs(np.random.default_rng(6))
+    for op, x in ((2, th), (3, th), (4, c)):
+        y = np.zeros_like(x)
+        assert L.bh_math_batch(op, len(x), P(x), P(y), 0) == 0
+        q = np.array([O.or_ic_fn(op, v, 0.0, 1) for v in x])
+        assert bits_equal(y, q) and np.array_equal(np.signbit(y), np.signbit(q)), (op, first_diff(y, q))
+    for op, x, yy in ((0, px, py), (1, ay, ax)):
+        out = np.zeros_like(x)
+        assert L.bh_math2_batch(op, len(x), P(x), P(yy), P(out), 0) == 0
+        q = np.array([O.or_ic_fn(5 + op, a, b, 1) for a, b in zip(x, yy)])
+        assert bits_equal(out, q) and np.array_equal(np.signbit(out), np.signbit(q)), (op, first_diff(out, q))
+
+
+def _fresh_blocks(cfg):
+    import bhfluid
+
+    lo, hi, nqlt, regrid, a, q = cfg
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    meta, edges = t.blocks()
+    return meta, edges, (1.0, a, q, configs.GAMMA)
+
+
+@pytest.mark.parametrize("name", ["R0", "R1", "R4"])
+def test_oracle_initial_data_matches_reference_digests(trees, name):
+    """The oracle's restatement of main.cpp:94-159 with the C library's
+    functions gives the reference's initial state, bit for bit."""
+    meta, edges, params = _fresh_blocks(getattr(configs, name))
+    o = OracleTree(meta, edges, *params)
+    rc, rho_max = o.init_torus(0)
+    assert rc == (2 if name in ("R0", "R1") else 0)
+    s = o.state()
+    assert digest(s.U, s.W, s.Bf) == trees[name]["initial"]
+
+
+@pytest.mark.parametrize("name", ["R2", "R3"])
+@needs_ref
+def test_oracle_initial_data_matches_reference_before_regrid(name):
+    lo, hi, nqlt, regrid, a, q = getattr(configs, name)
+    meta, edges, params = _fresh_blocks(getattr(configs, name))
+    o = OracleTree(meta, edges, *params)
+    rc, rho_max = o.init_torus(0)
+    r = RefTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    assert r.torus_init(regrid=False) == rho_max
+    s, rs = o.state(), r.state()
+    for x, y in ((s.U, rs.U), (s.W, rs.W), (s.Bf, rs.Bf)):
+        assert bits_equal(x, y), first_diff(x, y)
+
+
+def test_oracle_initial_data_modes_differ_only_by_rounding():
+    """Correctly rounded functions move the torus only at the rounding level:
+    same NaN masks, same zero pattern of the density normalisation, and the
+    primitives (other than the face-differenced fields) within 1e-12."""
+    meta, edges, params = _fresh_blocks(configs.R4)
+    s = {}
+    for mode in (0, 1):
+        o = OracleTree(meta, edges, *params)
+        assert o.init_torus(mode) == (0, 1.0)
+        s[mode] = o.state()
+    for nm in ("U", "W", "Bf"):
+        a, b = getattr(s[0], nm), getattr(s[1], nm)
+        assert np.array_equal(np.isnan(a), np.isnan(b))
+    W0, W1 = s[0].W[:, :5], s[1].W[:, :5]
+    assert np.array_equal(W0 == 0, W1 == 0)
+    assert np.max(np.abs(W0 - W1) / np.maximum(np.abs(W0), 1e-300)) < 1e-12
+    assert np.count_nonzero(W0 != W1) > 0  # the two modes are distinguishable here
+    assert np.count_nonzero(s[0].Bf != s[1].Bf) < 0.01 * s[0].Bf.size
