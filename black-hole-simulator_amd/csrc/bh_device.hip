@@ -276,15 +276,21 @@ __device__ __forceinline__ void ghost_floor(const KP& P, int b, int c, const GPt
     Wa(P, b, WEPS, c) = w.eps;
     storeU(P, b, c, ptoc(m, P.gamma, w, nf));
 }
-// First con2prim pass: 1 = solved (W written when ok), 0 = failed, -1 = queued
+// First con2prim pass: 1 = solved (pv set), 0 = failed, -1 = queued.  A
+// Brent fallback whose outcome is known without its loop (the atmosphere
+// form) is finished here; the others go to the queue.
 __device__ __forceinline__ int c2p_first(const KP& P, int b, int c, const GPt& m, int kind, Prim& pv, int& nf) {
     C2P s;
     const Cons u = loadU(P, b, c);
     ctop_setup(m, u, s);
     double xi, xi_h;
     if (!ctop_newton(P.gamma, s, xi, xi_h)) {
-        push_brent(P, b, c, kind, xi_h, u, m);
-        return -1;
+        const int r = ctop_brent_short(P.gamma, s, xi_h, xi);
+        if (r < 0) {
+            push_brent(P, b, c, kind, xi_h, u, m);
+            return -1;
+        }
+        if (r == 0) return 0;
     }
     return ctop_finish(P.gamma, s, xi, pv, nf) ? 1 : 0;
 }

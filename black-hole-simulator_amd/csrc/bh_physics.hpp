@@ -235,6 +235,52 @@ BH_HD double c2p_f_signed(double gm, double xi, double Ssq, double Bsq, double S
 
 BH_HD void dswap(double& a, double& b) { double t = a; a = b; b = t; }
 
+// The Brent loop below when Ssq = Bsq = SB = 0 (no momentum or field in the
+// components the inversion reads, quirk Q6), the usual state of the
+// atmosphere.  Then vsq = +0, ltz = 1, rho = D and the residual evaluates
+// exactly as f(xi) = ((xi - tau) - D) - c (xi - D), c = (gm - 1)/gm, whose
+// rounding error is below 7u (xi + |tau| + D), u = 2^-53 (five roundings,
+// each at most u times a term bounded by xi + |tau| + D).  So for xi above
+// the last midpoint s_n of the loop the residual is positive once
+// (s_n - D)(1 - c) - tau > 7u (s_n + |tau| + D); the condition tested
+// below, with 32u, also covers the rounding of its own evaluation.  With
+// fa > 0 and xb < xa every midpoint then moves xa (fa * fs < 0 is false),
+// xb never moves, and the loop ends on root = xb after the same number of
+// halvings, or fails after 100; the midpoints themselves are replayed
+// exactly, and so is the test that keeps the step a bisection.  Returns 1
+// (root = xb), 0 (the 100 passes end without convergence), or -1 when the
+// argument does not apply and the loop must run.
+BH_HD int c2p_brent_atmosphere(double gm, double xa, double xb, double D, double tau, double tol, double& root) {
+    const double c = (gm - 1.0) / gm;
+    // Short form.  For 0 < xb < 2^-16 every pass halves the width w = xa - xb
+    // up to a relative 2^-30 (the midpoint's rounding is below 2^-53 (xa + xb)
+    // and 2^-53 xb << tol <= w), so from w <= 2^80 tol the loop converges
+    // within 82 passes; each midpoint exceeds xb by at least ~tol/2 (the last
+    // one halves a width >= tol), so the residual test at xb + 0.49 tol covers
+    // all of them; and (3 xa + 4)/4 >= 1 > xb keeps every pass a bisection.
+    if (xb > 0.0 && xb < 0x1p-16 && xa - xb <= 0x1p80 * tol) {
+        const double s = xb + 0.49 * tol;
+        if (!((s - D) * (1.0 - c) - tau > 0x1p-48 * (fabs(s) + fabs(tau) + D))) return -1;
+        root = xb;
+        return 1;
+    }
+    double a = xa, s = xa;
+    int n = 0;
+    bool converged = false;
+    for (int i = 0; i < 100; i++) {
+        if (fabs(xb - a) < tol) { converged = true; break; }
+        if ((3 * a + 4) / 4 < xb) return -1;  // the interpolant could be taken
+        s = (a + xb) / 2;
+        a = s;
+        n = i + 1;
+    }
+    // the midpoints decrease from xa towards xb, so s is the smallest
+    if (n > 0 && !((s - D) * (1.0 - c) - tau > 0x1p-48 * (fabs(s) + fabs(tau) + D))) return -1;
+    if (!converged) return 0;
+    root = xb;
+    return 1;
+}
+
 // Brent-Dekker fallback, conserved.cpp:232-285 (quirk Q13: bisection test
 // (3a+4)/4; fa and fb are never refreshed inside the loop).
 BH_HD bool c2p_brent(double gm, double xa, double xb, double Ssq, double Bsq, double SB, double D, double tau,
@@ -243,6 +289,11 @@ BH_HD bool c2p_brent(double gm, double xa, double xb, double Ssq, double Bsq, do
     double fb = c2p_f(gm, xb, Ssq, Bsq, SB, D, tau);
     if (fa * fb > 0.0) return false;
     if (fabs(fa) < fabs(fb)) { dswap(xa, xb); dswap(fa, fb); }
+    if (Ssq == 0.0 && Bsq == 0.0 && SB == 0.0 && fa > 0.0 && D > 0.0 && xb < xa && isfinite(xa) &&
+        isfinite(tau)) {
+        const int r = c2p_brent_atmosphere(gm, xa, xb, D, tau, tol, root);
+        if (r >= 0) return r == 1;
+    }
     double xc = xa, fc = fa;
     bool mflag = true;
     double s = 0.0, d = 0.0;
@@ -349,6 +400,20 @@ BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h) {
 }
 BH_HD bool ctop_brent(double gm, const C2P& s, double xi_h, double& xi) {
     return c2p_brent(gm, s.D, xi_h, s.Ssq, s.Bsq, s.SB, s.D, s.tau, xi, 1e-10);
+}
+// ctop_brent where its outcome is known without the loop: the end-point
+// test, then the atmosphere form (c2p_brent_atmosphere).  1 = root xi,
+// 0 = Brent fails, -1 = the loop must run.
+BH_HD int ctop_brent_short(double gm, const C2P& s, double xi_h, double& xi) {
+    if (!(s.Ssq == 0.0 && s.Bsq == 0.0 && s.SB == 0.0)) return -1;
+    double xa = s.D, xb = xi_h;
+    double fa = c2p_f(gm, xa, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
+    double fb = c2p_f(gm, xb, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
+    if (fa * fb > 0.0) return 0;
+    if (fabs(fa) < fabs(fb)) { dswap(xa, xb); dswap(fa, fb); }
+    if (fa > 0.0 && s.D > 0.0 && xb < xa && isfinite(xa) && isfinite(s.tau))
+        return c2p_brent_atmosphere(gm, xa, xb, s.D, s.tau, 1e-10, xi);
+    return -1;
 }
 // Primitive recovery from the root xi (conserved.cpp:196-229)
 BH_HD bool ctop_finish(double gm, const C2P& s, double xi, Prim& pv, int& nf) {
