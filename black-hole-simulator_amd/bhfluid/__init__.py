@@ -116,6 +116,8 @@ def lib():
         _sig(L, "bh_tree_regrid_device", _i, [_v, _ip])
         _sig(L, "bh_tree_init_torus_device", _i, [_v, _dp])
         _sig(L, "bh_math2_batch", _i, [_i, _i, _dp, _dp, _dp, _i])
+        _sig(L, "bh_tree_use_graph", _i, [_v, _i])
+        _sig(L, "bh_profile_only", _i, [_v, ctypes.c_char_p])
         _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
         _sig(L, "bh_copy_probe", _i, [ctypes.c_longlong, _i, _dp])
@@ -273,6 +275,11 @@ class AmrTree:
             self._check(rc, "init_torus_device")
         return out.value
 
+    def use_graph(self, enable: bool = True):
+        """Replay the step from a captured HIP graph (default) or launch its
+        kernels one by one; identical results."""
+        self._check(self._L.bh_tree_use_graph(self._h, int(bool(enable))), "use_graph")
+
     def regrid(self):
         """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""
         self._check(self._L.bh_tree_regrid(self._h), "regrid")
@@ -285,8 +292,10 @@ class AmrTree:
         """Build the device copy (geometry tables, ghost plan, state upload)."""
         self._check(self._L.bh_tree_prepare(self._h), "prepare")
 
-    def profile(self, enable: bool = True):
-        """Per-kernel HIP-event timing of subsequent launches."""
+    def profile(self, enable: bool = True, only: str = ""):
+        """Per-kernel HIP-event timing of subsequent launches (of the kernels
+        named ``only``, or of all)."""
+        self._check(self._L.bh_profile_only(self._h, only.encode()), "profile")
         self._check(self._L.bh_profile(self._h, 1 if enable else 0), "profile")
 
     def profile_report(self):

@@ -590,7 +590,9 @@ __global__ __launch_bounds__(TPB) void k_logs(KP P) {
 // linear v and transverse B; reconstruction.cpp:40-97), the normal field
 // taken from the face, ptoc of both states at the face and the HLLD flux.
 // quirk Q1: the Riemann solver is called with dim 0 for every direction.
-__global__ __launch_bounds__(TPB) void k_flux(KP P) {
+// Four wavefronts per SIMD (128 VGPRs, a few spilled) measured 3 % faster
+// than the three the register count would otherwise allow.
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(KP P) {
     const int b = blockIdx.x / WG_FACE;
     const int f = (blockIdx.x % WG_FACE) * TPB + threadIdx.x;
     if (f >= 3 * NFC) return;
@@ -1555,6 +1557,16 @@ public:
 
     // Per-kernel timing with HIP events on this tree's stream.
     bool profile = false;
+    std::string profile_only;  // time only the kernels of this name ("" = all)
+    // The step as a HIP graph: captured on first use for the current block
+    // layout and CFL number, replayed per step (no host work between
+    // kernels).  The step issues kernels only (queue counters are cleared
+    // by k_qreset, not by memset nodes).
+    bool use_graph = true;
+    int layout_gen = 0;
+    hipGraphExec_t step_exec = nullptr;
+    int step_gen = -1;
+    double step_cfl = 0.0;
     struct Rec { const char* name; hipEvent_t a, b; };
     std::vector<Rec> recs;
     std::vector<hipEvent_t> pool;
@@ -1569,7 +1581,7 @@ public:
     }
     template <class Kern, class... Args>
     void launch(const char* name, Kern k, dim3 g, Args... args) {
-        if (profile) {
+        if (profile && (profile_only.empty() || profile_only == name)) {
             Rec r{name, ev(), ev()};
             HIPCHK(hipEventRecord(r.a, stream));
             hipLaunchKernelGGL(k, g, dim3(TPB), 0, stream, args...);
@@ -1596,7 +1608,13 @@ public:
     }
 
     DeviceTree() { HIPCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking)); }
+    void graph_release() {
+        if (step_exec) (void)hipGraphExecDestroy(step_exec);
+        step_exec = nullptr;
+        step_gen = -1;
+    }
     ~DeviceTree() {
+        graph_release();
         for (auto& r : recs) { pool.push_back(r.a); pool.push_back(r.b); }
         for (auto e : pool) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
@@ -1676,6 +1694,7 @@ public:
 
     // (Re)build for the host tree's current block structure.
     void build(const HostTree& t) {
+        layout_gen++;
         gamma = t.prm.gamma;
         nb = t.nblocks();
         GeomHost g;
@@ -1748,6 +1767,7 @@ public:
     // state, the tree changes on the host, the data work on the device.
     // Returns the number of blocks refined plus flattened.
     int regrid(HostTree& t) {
+        layout_gen++;
         const int nl = (int)plan.leaves.size();
         for (int b = 0; b < nb; b++) t.quilt[b]->dev = b;
         std::vector<unsigned char> lf(nl, 0);
@@ -1930,10 +1950,30 @@ public:
         launch_rkstg(1);
         launch_floors();
     }
+    // one step, from the captured graph where allowed (not while kernels are
+    // timed one by one)
+    void step(double cfl) {
+        if (profile || !use_graph) {
+            launch_step(cfl);
+            return;
+        }
+        if (!step_exec || step_gen != layout_gen || step_cfl != cfl) {
+            graph_release();
+            hipGraph_t g = nullptr;
+            HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            launch_step(cfl);
+            HIPCHK(hipStreamEndCapture(stream, &g));
+            HIPCHK(hipGraphInstantiate(&step_exec, g, nullptr, nullptr, 0));
+            HIPCHK(hipGraphDestroy(g));
+            step_gen = layout_gen;
+            step_cfl = cfl;
+        }
+        HIPCHK(hipGraphLaunch(step_exec, stream));
+    }
     void run_steps(double cfl, int nsteps) {
         if ((int)dts.n < nsteps) dts.alloc(nsteps);
         for (int s = 0; s < nsteps; s++) {
-            launch_step(cfl);
+            step(cfl);
             HIPCHK(hipMemcpyAsync(dts.p + s, scal.p, sizeof(double), hipMemcpyDeviceToDevice, stream));
         }
     }
@@ -2292,7 +2332,7 @@ int bh_tree_floors(bh_tree* t) {
 int bh_rk2_step(bh_tree* t, double cfl, double* dt) {
     return guarded(t, [&] {
         need_device(t);
-        t->dev->launch_step(cfl);
+        t->dev->step(cfl);
         t->dev->sync();
         device_changed(t);
         if (dt) *dt = t->dev->read_dt();
@@ -2306,7 +2346,7 @@ int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_i
         int n = 0;
         long long nf = 0;
         while (tt < t_end && n < nsteps) {
-            t->dev->launch_step(cfl);
+            t->dev->step(cfl);
             t->dev->sync();
             tt += t->dev->read_dt();
             n++;
@@ -2316,6 +2356,21 @@ int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_i
         if (t_io) *t_io = tt;
         if (steps_out) *steps_out = n;
         return finish_faults(t, nf);
+    });
+}
+int bh_profile_only(bh_tree* t, const char* name) {
+    return guarded(t, [&] {
+        if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+        t->dev->profile_only = name ? name : "";
+        return BH_OK;
+    });
+}
+int bh_tree_use_graph(bh_tree* t, int enable) {
+    return guarded(t, [&] {
+        if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+        t->dev->use_graph = enable != 0;
+        if (!enable) t->dev->graph_release();
+        return BH_OK;
     });
 }
 int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out) {

@@ -99,6 +99,10 @@ int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_i
  * t_end test); dt_out[nsteps] receives each step's dt.  No counterpart in
  * the reference; used by the benchmark. */
 int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out);
+/* Run steps from a captured HIP graph of the whole step (default 1) or launch the kernels
+ * one by one (0).  Results are identical; no reference counterpart.  Timing with
+ * bh_profile always launches one by one. */
+int bh_tree_use_graph(bh_tree* t, int enable);
 /* Build the device copy (geometry, ghost plan, state upload) ahead of the
  * first step, so that set-up is not timed with it. */
 int bh_tree_prepare(bh_tree* t);
@@ -107,6 +111,8 @@ int bh_tree_prepare(bh_tree* t);
  * counting the cells each con2prim pass handed to the Brent fallback.
  * Measurement only. */
 int bh_profile(bh_tree* t, int enable);
+/* Restrict the timing to the kernels of one name ("" or NULL: all kernels). */
+int bh_profile_only(bh_tree* t, const char* name);
 int bh_profile_report(bh_tree* t, char* buf, int len);
 int bh_device_sync(void);
 /* Equation-of-state domain faults (rho <= 0) since the last call. */

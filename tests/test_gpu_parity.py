@@ -329,3 +329,22 @@ def test_device_initial_data_close_to_reference():
     rel = np.abs(Wa[fin] - Wb[fin]) / scale
     # velocities at stagnation are compared on the magnitude of the velocity field
     assert np.max(np.where(np.abs(Wb[fin]) > 1e-8, rel, 0.0)) < 1e-10
+
+
+@pytest.mark.parametrize("name", ["R3", "R4"])
+def test_step_graph_matches_eager(name):
+    """The step replayed from its captured HIP graph equals the kernels
+    launched one by one, bit for bit, on every replay (three replays, with
+    host reads and synchronisation between them; R3 has two ghost waves)."""
+    g, _ = build(getattr(configs, name))
+    e, _ = build(getattr(configs, name))
+    e.use_graph(False)
+    for k in range(3):
+        for t in (g, e):
+            try:
+                bhfluid.RK2Integrator(configs.CFL).step(t)
+            except bhfluid.DomainError:
+                pass
+        for x, y, nm in zip(g.state(), e.state(), ("U", "W", "Bf")):
+            n, ex = first_diff(x, y)
+            assert n == 0, (name, k, nm, n, ex)
