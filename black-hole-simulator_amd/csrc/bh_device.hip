@@ -90,12 +90,10 @@ struct KP {
     const double* gth;   // [nb][8000] polar angle of each cell centre
     const int* ghtab;    // [nb][12]
     const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
-    // [3904] ghost cells in storage order (the fill copies whole rows and
-    // planes: neighbouring lanes touch neighbouring addresses)
-    const short* ghost_order;
-    // [8000] con2prim visiting order: the ghost cells grouped by face side,
-    // then edges and corners, then the 4096 active zones (lanes of a
-    // wavefront share the side, hence the fill kind and the typical cost)
+    // [8000] visiting order of the ghost fill (its first 3904 entries) and
+    // of the con2prim phase: the ghost cells in storage order, then the
+    // active zones (neighbouring lanes touch neighbouring addresses; the
+    // z-side ghost rows, two cells per side and row, meet across rows)
     const short* cell_order;
     unsigned long long* dtbits;  // dt as an ordered bit pattern
     unsigned long long* faults;
@@ -308,7 +306,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
     const int b = blocks[blockIdx.x / WG_GHOST];
     const int t = (blockIdx.x % WG_GHOST) * TPB + threadIdx.x;
     if (t >= NGHOST) return;
-    const int c = P.ghost_order[t];
+    const int c = P.cell_order[t];
     const int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
     int nf = 0;
     const int oi = (i < 0 || i >= BLK), oj = (j < 0 || j >= BLK), ok = (k < 0 || k >= BLK);
@@ -408,8 +406,8 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
 }
 
 // con2prim on every stored cell of every leaf, integrator.cpp:201-222.
-// Ghost cells first, then active zones (mostly settled by the early pass),
-// so that wavefronts hold cells of like cost.
+// Ghost cells first, in storage order, then active zones (mostly settled by
+// the early pass).
 __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leaves) {
     const int b = leaves[blockIdx.x / WG_CELL];
     const int t = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
@@ -1627,7 +1625,7 @@ public:
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
         P.gth = gth.p;
         P.ghtab = ghtab.p; P.ghsame = ghsame.p;
-        P.ghost_order = cell_order.p; P.cell_order = cell_order.p + NGHOST;
+        P.cell_order = cell_order.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
         P.lg = lg.p;
@@ -1719,15 +1717,13 @@ public:
             org.alloc(o.size()); org.upload(o.data(), o.size(), stream);
         }
         tables(t, &g);
-        // storage-order ghost list, then the con2prim order
+        // ghost cells in storage order, then the active zones
         std::vector<short> order;
         for (int c = 0; c < NCELL; c++) {
             int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
             if (i < 0 || i >= BLK || j < 0 || j >= BLK || k < 0 || k >= BLK) order.push_back((short)c);
         }
-        const std::vector<short> by_side = ghost_cell_order();
-        if ((int)order.size() != NGHOST || (int)by_side.size() != NGHOST) throw DevError("ghost cell table size");
-        order.insert(order.end(), by_side.begin(), by_side.end());
+        if ((int)order.size() != NGHOST) throw DevError("ghost cell table size");
         for (int a = 0; a < NINT; a++) order.push_back((short)lindx(a / 256, (a / 16) % 16, a % 16));
         cell_order.alloc(order.size()); cell_order.upload(order.data(), order.size(), stream);
         scal.alloc(2);

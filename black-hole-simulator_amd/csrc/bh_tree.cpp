@@ -929,20 +929,4 @@ void mark_exact_copies(const GeomHost& geo, GhostPlan& g) {
         }
 }
 
-std::vector<short> ghost_cell_order() {
-    std::vector<short> out;
-    for (int side = 0; side < 6; side++) {
-        const int dim = side / 2, s = side % 2;
-        for (int gl = 0; gl < GH; gl++)
-            for (int t1 = 0; t1 < BLK; t1++)
-                for (int t2 = 0; t2 < BLK; t2++) out.push_back((short)cell_on_axis(dim, s ? BLK + gl : gl - GH, t1, t2));
-    }
-    for (int c = 0; c < NCELL; c++) {
-        const int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
-        const int o = (i < 0 || i >= BLK) + (j < 0 || j >= BLK) + (k < 0 || k >= BLK);
-        if (o >= 2) out.push_back((short)c);
-    }
-    return out;
-}
-
 }  // namespace bh

@@ -144,8 +144,5 @@ struct GhostPlan {
 };
 void build_ghost_plan(const HostTree& t, GhostPlan& g);
 void mark_exact_copies(const GeomHost& geo, GhostPlan& g);
-// Ghost cells of a block: the 6 x 512 face-ghost cells grouped by side
-// (dim 0 low, dim 0 high, dim 1 low, ...), then the 832 edge and corner cells.
-std::vector<short> ghost_cell_order();
 
 }  // namespace bh
