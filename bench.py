@@ -52,8 +52,8 @@ A_SPIN, Q_CHARGE, GAMMA, CFL = 0.9, 0.1, 5.0 / 3.0, 0.4
 #   k_ghost  : U, W of 3072 face-ghost sources in, U, W of 3904 ghost cells out
 #   k_update : U, W, RK store in (4096 x 21 words), U and store out (4096 x 13),
 #              fluxes 5 words on 3 x 4352 faces in
-#   k_emf    : 6 flux-B words on 3 x 4352 faces in, 3 x 6859 EMFs out
-#   k_bfupd  : 3 x 6859 EMFs in, 3 x 5832 face B in/out
+#   k_bfupd  : 6 flux-B words on 3 x 4352 faces in (EMFs formed in place),
+#              3 x 5832 face B in/out
 #   k_f2cb   : 3 x 8400 face B in, 6859 cells x 6 words out
 #   k_floors : U, W in and out on 4096 cells
 #   k_dtcomp : 5 W words on 4096 cells in
@@ -64,8 +64,7 @@ ALGO_BYTES_PER_BLOCK = {
     "k_c2p_pre": (4096 * 16) * W8,
     "k_ghost": (3072 * 16 + 3904 * 16) * W8,
     "k_update": (4096 * (21 + 13) + 3 * 4352 * 5) * W8,
-    "k_emf": (3 * 4352 * 6 + 3 * 6859) * W8,
-    "k_bfupd": (3 * 6859 + 2 * 3 * 5832) * W8,
+    "k_bfupd": (3 * 4352 * 6 + 2 * 3 * 5832) * W8,
     "k_f2cb": (3 * 8400 + 6859 * 6) * W8,
     "k_floors": (4096 * 32) * W8,
     "k_dtcomp": (4096 * 5) * W8,
@@ -75,7 +74,7 @@ ALGO_BYTES_PER_BLOCK = {
     "k_qreset": 0,
 }
 # launches per step and the blocks each launch covers ("leaf" or "all")
-KERNEL_SCOPE = {"k_flux": "all", "k_emf": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
+KERNEL_SCOPE = {"k_flux": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
                 "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf", "k_logs": "all",
                 "k_dtreduce": "leaf", "k_c2p_pre": "leaf", "k_brent": "leaf", "k_qreset": "leaf"}
 # HBM bytes per dispatch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE,

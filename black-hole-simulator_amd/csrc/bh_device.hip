@@ -79,7 +79,6 @@ struct KP {
     double* Bf;          // [nb][3][8400]
     double* Us;          // [nb][5][4096] RK store of D, S, tau
     double* F;           // [nb][3][8][4352] fluxes on the computed faces
-    double* E;           // [nb][3][8820] EMFs
     const double* gct;   // [nb][8][8000]
     const double* gcc;   // [nb][11][8000]
     const double* gfl;   // [nb][2][8000]
@@ -654,29 +653,30 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 }
 
 // ---------------------------------------------------- constrained transport
-// constrans::emfcomp, transport.cpp:11-36: edge EMFs as 4-point averages of
-// the transverse face fluxes of B (zero outside computed faces, quirk Q19).
-__global__ __launch_bounds__(TPB) void k_emf(KP P) {
-    const int b = blockIdx.x / WG_EDGE;
-    const int t = (blockIdx.x % WG_EDGE) * TPB + threadIdx.x;
-    if (t >= 19 * 19 * 19) return;
-    const int i = t / 361 - GH, j = (t / 19) % 19 - GH, k = t % 19 - GH;
-    const double* F = P.F + (size_t)b * 3 * 8 * NFC;
-    auto fB = [&](int dir, int comp, int ii, int jj, int kk) -> double {
+// constrans::emfcomp, transport.cpp:11-36: the edge EMFs as 4-point
+// averages of the transverse face fluxes of B (zero outside computed faces,
+// quirk Q19).  Ex sits at iex(i, j, k), Ey at izf(i, j, k), Ez at iyf(i, j, k)
+// of the reference's EMF arrays.
+struct EmfView {
+    const double* F;
+    __device__ double fB(int dir, int comp, int ii, int jj, int kk) const {
         return face_in(dir, ii, jj, kk) ? F[(dir * 8 + 5 + comp) * NFC + face_lf(dir, ii, jj, kk)] : 0.0;
-    };
-    // all loads ahead of the stores (E and F are both double arrays, so a
-    // store could otherwise hold back the loads after it)
-    const double a0 = fB(1, 2, i, j, k), a1 = fB(1, 2, i, j, k + 1), a2 = fB(2, 1, i, j, k), a3 = fB(2, 1, i, j + 1, k);
-    const double b0 = fB(2, 0, i, j, k), b1 = fB(2, 0, i + 1, j, k), b2 = fB(0, 2, i, j, k), b3 = fB(0, 2, i, j, k + 1);
-    const double c0 = fB(0, 1, i, j, k), c1 = fB(0, 1, i, j + 1, k), c2 = fB(1, 0, i, j, k), c3 = fB(1, 0, i + 1, j, k);
-    double* E = P.E + (size_t)b * 3 * NEDGE;
-    E[iex(i, j, k)] = (a0 + a1 - a2 - a3) / 4;
-    E[NEDGE + izf(i, j, k)] = (b0 + b1 - b2 - b3) / 4;
-    E[2 * NEDGE + iyf(i, j, k)] = (c0 + c1 - c2 - c3) / 4;
-}
+    }
+    __device__ double ex(int i, int j, int k) const {
+        return (fB(1, 2, i, j, k) + fB(1, 2, i, j, k + 1) - fB(2, 1, i, j, k) - fB(2, 1, i, j + 1, k)) / 4;
+    }
+    __device__ double ey(int i, int j, int k) const {
+        return (fB(2, 0, i, j, k) + fB(2, 0, i + 1, j, k) - fB(0, 2, i, j, k) - fB(0, 2, i, j, k + 1)) / 4;
+    }
+    __device__ double ez(int i, int j, int k) const {
+        return (fB(0, 1, i, j, k) + fB(0, 1, i, j + 1, k) - fB(1, 0, i, j, k) - fB(1, 0, i + 1, j, k)) / 4;
+    }
+};
 
-// constrans::Bfupdate, transport.cpp:38-58 (not RK-averaged, quirk Q21)
+// constrans::Bfupdate, transport.cpp:38-58 (not RK-averaged, quirk Q21), with
+// the nine EMFs each face triple needs formed in place from the fluxes
+// (emfcomp's EMF arrays are not stored: every EMF read here lies in the
+// range emfcomp computes, [-2, 16]^3)
 __global__ __launch_bounds__(TPB) void k_bfupd(KP P) {
     const int b = blockIdx.x / WG_BF;
     const int t = (blockIdx.x % WG_BF) * TPB + threadIdx.x;
@@ -684,16 +684,14 @@ __global__ __launch_bounds__(TPB) void k_bfupd(KP P) {
     const int i = t / 324 - 1, j = (t / 18) % 18 - 1, k = t % 18 - 1;
     const double dt = __longlong_as_double((long long)*P.dtbits);
     const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
-    const double* Ex = P.E + (size_t)b * 3 * NEDGE;
-    const double* Ey = Ex + NEDGE;
-    const double* Ez = Ex + 2 * NEDGE;
+    const EmfView E{P.F + (size_t)b * 3 * 8 * NFC};
     double* Bx = P.Bf + (size_t)b * 3 * NFACE;
     double* By = Bx + NFACE;
     double* Bz = Bx + 2 * NFACE;
     // all loads ahead of the stores
-    const double ez = Ez[iyf(i, j, k)], ez_jm = Ez[iyf(i, j - 1, k)], ez_im = Ez[iyf(i - 1, j, k)];
-    const double ey = Ey[izf(i, j, k)], ey_km = Ey[izf(i, j, k - 1)], ey_im = Ey[izf(i - 1, j, k)];
-    const double ex = Ex[iex(i, j, k)], ex_km = Ex[iex(i, j, k - 1)], ex_jm = Ex[iex(i, j - 1, k)];
+    const double ez = E.ez(i, j, k), ez_jm = E.ez(i, j - 1, k), ez_im = E.ez(i - 1, j, k);
+    const double ey = E.ey(i, j, k), ey_km = E.ey(i, j, k - 1), ey_im = E.ey(i - 1, j, k);
+    const double ex = E.ex(i, j, k), ex_km = E.ex(i, j, k - 1), ex_jm = E.ex(i, j - 1, k);
     const double bx = Bx[ixf(i, j, k)], by = By[iyf(i, j, k)], bz = Bz[izf(i, j, k)];
     double dEz_dy = (ez - ez_jm) / dy;
     double dEy_dz = (ey - ey_km) / dz;
@@ -833,7 +831,8 @@ __global__ __launch_bounds__(TPB) void k_update(KP P, int stage) {
         if (stage == 0) usp[(size_t)v * NINT] = u[v];
         Ua(P, b, v, c) = nu[v];
     }
-    for (int d = 0; d < 3; d++) Ua(P, b, UB0 + d, c) = sg * W.B[d];
+    // U.B = sqrt(-g) W.B (integrator.cpp:252-255) is already in place: k_f2cb
+    // stored exactly these products over [-1, 17]^3 earlier in the stage.
     cst_at(P, b, c) = 0;
     add_faults(P, nf);
 }
@@ -1534,7 +1533,7 @@ public:
     int limiter = LIM_VANLEER;
     hipStream_t stream = nullptr;
     GhostPlan plan;
-    DBuf<double> U, W, Bf, Us, F, E, gct, gcc, gfl, gfc, gchr, dxyz, org, gth;
+    DBuf<double> U, W, Bf, Us, F, gct, gcc, gfl, gfc, gchr, dxyz, org, gth;
     DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
     DBuf<int> ghtab, leaves, waveblk, lvl;
     DBuf<unsigned char> ghsame;
@@ -1621,7 +1620,7 @@ public:
     KP kp() {
         KP P;
         P.gamma = gamma; P.limiter = limiter;
-        P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p; P.E = E.p;
+        P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p;
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
         P.gth = gth.p;
         P.ghtab = ghtab.p; P.ghsame = ghsame.p;
@@ -1638,9 +1637,7 @@ public:
     void alloc_scratch() {
         Us.alloc((size_t)nb * 5 * NINT);
         F.alloc((size_t)nb * 3 * 8 * NFC);
-        E.alloc((size_t)nb * 3 * NEDGE);
         HIPCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
-        HIPCHK(hipMemsetAsync(E.p, 0, E.n * sizeof(double), stream));
         HIPCHK(hipMemsetAsync(Us.p, 0, Us.n * sizeof(double), stream));
         cst.alloc((size_t)nb * NCELL);
         HIPCHK(hipMemsetAsync(cst.p, 0, cst.n, stream));
@@ -1929,7 +1926,6 @@ public:
         }
         launch("k_logs", k_logs, dim3(nb * WG_CELL), P);
         launch("k_flux", k_flux, dim3(nb * WG_FACE), P);
-        launch("k_emf", k_emf, dim3(nb * WG_EDGE), P);
         launch("k_bfupd", k_bfupd, dim3(nb * WG_BF), P);
         launch("k_f2cb", k_f2cb, dim3(nb * WG_EDGE), P);
         launch("k_update", k_update, dim3(nb * WG_INT), P, stage);
