@@ -360,8 +360,11 @@ def test_correctly_rounded_math_on_host():
 
     L, O = bhfluid.lib(), oracle_lib()
     rng = np.random.default_rng(1)
-    for op, x in ((0, np.exp(rng.uniform(-700, 700, 30000))), (1, rng.uniform(-740, 709, 30000)),
-                  (1, rng.uniform(-30, 3, 30000))):
+    logx = np.concatenate([np.exp(rng.uniform(-700, 700, 30000)), 1 + rng.uniform(-2 ** -6, 2 ** -6, 10000),
+                           1 + rng.integers(-2 ** 20, 2 ** 20, 5000) * 2.0 ** -52, rng.uniform(0.74, 1.51, 10000),
+                           np.ldexp(1.0, rng.integers(-1074, 1023, 2000)),
+                           np.ldexp(rng.uniform(1, 2, 2000), rng.integers(-1074, -1000, 2000))])
+    for op, x in ((0, logx), (1, rng.uniform(-740, 709, 30000)), (1, rng.uniform(-30, 3, 30000))):
         y = np.zeros_like(x)
         assert L.bh_math_batch(op, len(x), P(x), P(y), 0) == 0
         q = np.array([O.or_log(v, 1) if op == 0 else O.or_exp(v, 1) for v in x])
