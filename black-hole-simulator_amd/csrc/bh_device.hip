@@ -89,6 +89,9 @@ struct KP {
     const double* gth;   // [nb][8000] polar angle of each cell centre
     const int* ghtab;    // [nb][12]
     const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
+    // [nb][2] the leaf whose high (0) / low (1) z-side ghosts copy this
+    // leaf's low (0) / high (1) z layers and are filled by push_zghost, or -1
+    const int* push_to;
     // [8000] visiting order of the ghost fill (its first 3904 entries) and
     // of the con2prim phase: the ghost cells in storage order, then the
     // active zones (neighbouring lanes touch neighbouring addresses; the
@@ -296,6 +299,19 @@ __device__ __forceinline__ int cell_of(int dim, int a, int t1, int t2) {
     return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
 }
 
+// Active zones whose U a ghost fill mirrors (domain-boundary sides): their
+// pre-con2prim W is read by the fill, so the early pass leaves them alone.
+__device__ __forceinline__ bool mirror_source(const KP& P, int b, int i, int j, int k) {
+    const int* gt = P.ghtab + b * 12;
+    const int x[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (gt[(2 * d) * 2] == GH_BNDY && x[d] < GH) return true;
+        if (gt[(2 * d + 1) * 2] == GH_BNDY && x[d] >= BLK - GH) return true;
+    }
+    return false;
+}
+
 // ---------------------------------------------------------------- ghosts
 // amrtree::ghosts for the leaves of one wave, grid.cpp:390-571: each thread
 // fills one ghost cell from its face neighbour (copy, coarse prolongation or
@@ -320,7 +336,14 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
         const int nb = P.ghtab[b * 12 + (2 * dim + s) * 2 + 1];
         if (kind == GH_COPY || kind == GH_BNDY) {
             int src_b, src_c;
-            if (kind == GH_COPY) { src_b = nb; src_c = cell_of(dim, (s ? 0 : BLK - GH) + g, t1, t2); }
+            if (kind == GH_COPY) {
+                src_b = nb;
+                src_c = cell_of(dim, (s ? 0 : BLK - GH) + g, t1, t2);
+                // already written by the source's early con2prim (push_zghost)
+                if (dim == 2 && P.push_to[nb * 2 + (s ? 0 : 1)] == b && P.ghsame[b * 6 + 4 + s] &&
+                    !mirror_source(P, nb, t1, t2, (s ? 0 : BLK - GH) + g))
+                    return;
+            }
             else { src_b = b; src_c = cell_of(dim, s ? BLK - 1 - g : g, t1, t2); }
             // all sixteen loads (and the source's con2prim status) ahead of
             // the stores into the same arrays
@@ -435,18 +458,37 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     add_faults(P, nf);
 }
 
-// Active zones whose U a ghost fill mirrors (domain-boundary sides): their
-// pre-con2prim W is read by the fill, so the early pass leaves them alone.
-__device__ __forceinline__ bool mirror_source(const KP& P, int b, int i, int j, int k) {
-    const int* gt = P.ghtab + b * 12;
-    const int x[3] = {i, j, k};
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-        if (gt[(2 * d) * 2] == GH_BNDY && x[d] < GH) return true;
-        if (gt[(2 * d + 1) * 2] == GH_BNDY && x[d] >= BLK - GH) return true;
+// Same-level z-side copies done by the source's early con2prim.  A z-layer
+// zone (k < 2 or k >= 14) of leaf b whose neighbour across that side copies
+// it into a ghost row (push_to: set where that neighbour is a leaf that no
+// prolongation reads; used where the metric bits agree) writes the copy and
+// the ghost's con2prim outcome there directly, the values the fill and the
+// ghost con2prim would write: U and W copied; on success the memo set; on
+// failure rho, eps floored and U recomputed with the shared metric.  This
+// replaces the fill's reads of two-cell rows, which waste most of each line.
+__device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, int k, const GPt& m, const Cons& u,
+                                            const Prim& w, unsigned char st) {
+    const int lo = k < GH;
+    const int B = P.push_to[b * 2 + (lo ? 0 : 1)];
+    if (B < 0 || !P.ghsame[B * 6 + (lo ? 5 : 4)]) return;
+    const int g = lindx(i, j, lo ? k + BLK : k - BLK);
+    if (st == 1) {
+        storeU(P, B, g, u);
+        storeW(P, B, g, w);
+        cst_at(P, B, g) = 1;
+        return;
     }
-    return false;
+    int nf = fail_faults(st);
+    const double* fl = P.gfl + (size_t)B * 2 * NCELL;
+    Prim wg = w;
+    wg.rho = fl[g];
+    wg.eps = fl[NCELL + g];
+    storeW(P, B, g, wg);
+    storeU(P, B, g, ptoc(m, P.gamma, wg, nf));
+    cst_at(P, B, g) = 0;
+    add_faults(P, nf);
 }
+__device__ __forceinline__ bool z_layer(int k) { return k < GH || k >= BLK - GH; }
 
 // The con2prim of the active zones of every leaf, run before the ghost fill.
 // The fill reads active zones' U (copies, prolongation) and W (copies, whose
@@ -461,17 +503,27 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
     const int a = (blockIdx.x % WG_INT) * TPB + threadIdx.x;
     const int i = a / 256, j = (a / 16) % 16, k = a % 16;
     const int c = lindx(i, j, k);
-    if (cst_at(P, b, c) || mirror_source(P, b, i, j, k)) return;
+    if (mirror_source(P, b, i, j, k)) return;
+    const unsigned char st0 = cst_at(P, b, c);
+    const GPt m = cell_tree(P, b, c);
+    if (st0) {
+        // outcome already known for this U
+        if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st0);
+        return;
+    }
     int nf = 0;
-    GPt m = cell_tree(P, b, c);
     Prim pv;
     const int r = c2p_first(P, b, c, m, Q_PRE, pv, nf);
     if (r > 0) {
         storeW(P, b, c, pv);
         cst_at(P, b, c) = 1;
+        if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), pv, 1);
     } else if (r == 0) {
-        cst_at(P, b, c) = fail_status(nf);
+        const unsigned char st = fail_status(nf);
+        cst_at(P, b, c) = st;
+        if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
     }
+    // a queued zone is pushed by k_brent
 }
 
 // floors semantics after con2prim of an active zone (grid.cpp:342-354)
@@ -529,12 +581,17 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
         }
         if (kind == Q_PRE) {
             // early con2prim pass: store a success, record a failure
+            unsigned char st = 1;
             if (ok) {
                 storeW(P, b, c, pv);
-                cst_at(P, b, c) = 1;
             } else {
-                cst_at(P, b, c) = fail_status(nf);
+                st = fail_status(nf);
             }
+            cst_at(P, b, c) = st;
+            const int ck = c % 20 - GH;
+            if (z_layer(ck))
+                push_zghost(P, b, c / 400 - GH, (c / 20) % 20 - GH, ck, cell_tree(P, b, c), loadU(P, b, c),
+                            ok ? pv : loadW(P, b, c), st);
             continue;
         }
         const GPt m = cell_tree(P, b, c);
@@ -1535,7 +1592,7 @@ public:
     GhostPlan plan;
     DBuf<double> U, W, Bf, Us, F, gct, gcc, gfl, gfc, gchr, dxyz, org, gth;
     DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
-    DBuf<int> ghtab, leaves, waveblk, lvl;
+    DBuf<int> ghtab, leaves, waveblk, lvl, pushto;
     DBuf<unsigned char> ghsame;
     std::vector<int> wave_off;  // offsets into waveblk
     DBuf<short> cell_order;
@@ -1623,7 +1680,7 @@ public:
         P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p;
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
         P.gth = gth.p;
-        P.ghtab = ghtab.p; P.ghsame = ghsame.p;
+        P.ghtab = ghtab.p; P.ghsame = ghsame.p; P.push_to = pushto.p;
         P.cell_order = cell_order.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
@@ -1661,6 +1718,22 @@ public:
                 hipLaunchKernelGGL(k_mark_same, dim3(plan.leaves.size() * 6), dim3(TPB), 0, stream, kp(),
                                    (const int*)leaves.p, ghsame.p);
             HIPCHK(hipGetLastError());
+        }
+        {
+            // z-side copy pushes: destination leaves that no prolongation reads
+            std::vector<char> isleaf(nb, 0), psrc(nb, 0);
+            for (int L : plan.leaves) isleaf[L] = 1;
+            for (int L : plan.leaves)
+                for (int sd = 0; sd < 6; sd++)
+                    if (plan.tab[(size_t)L * 12 + sd * 2] == GH_PROLONG) psrc[plan.tab[(size_t)L * 12 + sd * 2 + 1]] = 1;
+            std::vector<int> pt((size_t)nb * 2, -1);
+            for (int B : plan.leaves)
+                for (int s = 0; s < 2; s++) {
+                    const int kind = plan.tab[(size_t)B * 12 + (4 + s) * 2], N = plan.tab[(size_t)B * 12 + (4 + s) * 2 + 1];
+                    if (kind == GH_COPY && N >= 0 && N != B && isleaf[N] && !psrc[B]) pt[(size_t)N * 2 + (s ? 0 : 1)] = B;
+                }
+            pushto.alloc(std::max<size_t>(2, pt.size()));
+            if (!pt.empty()) pushto.upload(pt.data(), pt.size(), stream);
         }
         std::vector<int> wb;
         wave_off.assign(1, 0);
