@@ -104,7 +104,9 @@ struct KP {
     // the same W).  Every kernel that writes U keeps it current.  Between the
     // early con2prim pass and the con2prim phase, an active zone may also hold
     // CST_FAIL (+1 when the failing call counted equation-of-state faults,
-    // always 3 then): con2prim of the current U fails; W and U are untouched.
+    // always 3 then): con2prim of the current U fails; W and U are untouched;
+    // or CST_DONE: the con2prim phase's work on the zone is already done (a
+    // failure applied early where no ghost fill reads the zone).
     unsigned char* cst;
     double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
     // Cells whose con2prim needs the Brent fallback are queued by the first
@@ -123,6 +125,7 @@ struct KP {
 };
 enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
 constexpr unsigned char CST_FAIL = 2;
+constexpr unsigned char CST_DONE = 4;
 // status of a failed con2prim and the faults it counted (0 or 3: the three
 // equation-of-state calls of the primitive recovery all see the same rho)
 __device__ __forceinline__ unsigned char fail_status(int nf) { return CST_FAIL + (nf > 0 ? 1 : 0); }
@@ -365,7 +368,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                     cst_at(P, b, c) = 1;
                     return;
                 }
-                if (st >= CST_FAIL) {
+                if (st == CST_FAIL || st == CST_FAIL + 1) {
                     nf = fail_faults(st);
                     cst_at(P, b, c) = 0;
                     ghost_floor(P, b, c, cell_tree(P, b, c), nf);
@@ -436,10 +439,10 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     if (t >= NCELL) return;
     const int c = P.cell_order[t];
     const unsigned char st = cst_at(P, b, c);
-    if (st == 1) return;
+    if (st == 1 || st == CST_DONE) return;
     int nf = 0;
     GPt m = cell_tree(P, b, c);
-    if (st >= CST_FAIL) {
+    if (st == CST_FAIL || st == CST_FAIL + 1) {
         // failure found by the early pass on this same U
         nf = fail_faults(st);
         c2p_floor(P, b, c, m, nf);
@@ -489,6 +492,11 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     add_faults(P, nf);
 }
 __device__ __forceinline__ bool z_layer(int k) { return k < GH || k >= BLK - GH; }
+// Active zones no ghost fill reads: copies and mirrors read the two layers
+// next to a face, prolongation the coarse layers -1, 0, 1 (quirk Q25).
+__device__ __forceinline__ bool deep_zone(int i, int j, int k) {
+    return i >= GH && i < BLK - GH && j >= GH && j < BLK - GH && k >= GH && k < BLK - GH;
+}
 
 // The con2prim of the active zones of every leaf, run before the ghost fill.
 // The fill reads active zones' U (copies, prolongation) and W (copies, whose
@@ -519,6 +527,14 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
         cst_at(P, b, c) = 1;
         if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), pv, 1);
     } else if (r == 0) {
+        if (deep_zone(i, j, k)) {
+            // the con2prim phase's failure path (floor state with the zone's
+            // B), applied now: nothing reads the zone before that phase
+            c2p_floor(P, b, c, m, nf);
+            cst_at(P, b, c) = CST_DONE;
+            add_faults(P, nf);
+            return;
+        }
         const unsigned char st = fail_status(nf);
         cst_at(P, b, c) = st;
         if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
@@ -582,15 +598,20 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
         if (kind == Q_PRE) {
             // early con2prim pass: store a success, record a failure
             unsigned char st = 1;
+            const int ci = c / 400 - GH, cj = (c / 20) % 20 - GH, ck = c % 20 - GH;
             if (ok) {
                 storeW(P, b, c, pv);
+            } else if (deep_zone(ci, cj, ck)) {
+                c2p_floor(P, b, c, cell_tree(P, b, c), nf);
+                cst_at(P, b, c) = CST_DONE;
+                add_faults(P, nf);
+                continue;
             } else {
                 st = fail_status(nf);
             }
             cst_at(P, b, c) = st;
-            const int ck = c % 20 - GH;
             if (z_layer(ck))
-                push_zghost(P, b, c / 400 - GH, (c / 20) % 20 - GH, ck, cell_tree(P, b, c), loadU(P, b, c),
+                push_zghost(P, b, ci, cj, ck, cell_tree(P, b, c), loadU(P, b, c),
                             ok ? pv : loadW(P, b, c), st);
             continue;
         }
