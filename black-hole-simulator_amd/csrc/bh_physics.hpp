@@ -9,6 +9,8 @@
 // marked "quirk" with the reference file:line.
 #pragma once
 
+#include <cfloat>
+
 #include "bh_common.hpp"
 
 namespace bh {
@@ -64,6 +66,18 @@ BH_HD double eos_cs2(double gm, double rho, double eps, int& nf) {
     return (gm * p) / (rho * h);
 }
 
+// q / l for a numerator that is often exactly zero (the magnetic terms in
+// field-free regions).  When q is +-0 and l finite and non-zero in every
+// lane of the wavefront, the quotient is the signed zero IEEE division gives
+// (the sign of q flipped for negative l) and the division sequence is
+// skipped; otherwise every lane divides.  Same bits as q / l in all cases.
+BH_HD double div_z(double q, double l) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if (__all(q == 0.0 && l != 0.0 && fabs(l) <= DBL_MAX)) return l > 0.0 ? q : -q;
+#endif
+    return q / l;
+}
+
 // ------------------------------------------------------------- ptoc
 // conserved::ptoc, conserved.cpp:17-73 (quirk Q7: b^2 divided by W).
 BH_HD Cons ptoc(const GPt& m, double gm, const Prim& pv, int& nf) {
@@ -116,7 +130,7 @@ BH_HD double c2p_fdf(double gm, double xi, double Ssq, double Bsq, double SB, do
     double rho = Dn / ltz;
     double G = gm - 1.0;
     double p = (G / gm) * (xi / ltz2 - Dn / ltz);
-    double pstar = p + K / (2 * ltz2) + Yi / (2 * X);
+    double pstar = p + div_z(K, 2 * ltz2) + div_z(Yi, 2 * X);
     if (p < 0.0 || rho < 0.0) { df = 0.0; return 1e32; }
     double f = xi - taun - Dn - pstar;
     double N = Ssq * X + Yi * (2.0 * xi + K);
@@ -127,7 +141,7 @@ BH_HD double c2p_fdf(double gm, double xi, double Ssq, double Bsq, double SB, do
     double dltz2 = ltz2 * ltz2 * dvv;
     double dltz = dltz2 / (2.0 * ltz);
     double dp = (G / gm) * ((1.0 / ltz2) - (xi * dltz2) / (ltz2 * ltz2) + Dn * dltz / ltz2);
-    double dpstar = dp - (K * dltz2) / (2.0 * ltz2 * ltz2) - Yi / (xi * X);
+    double dpstar = dp - div_z(K * dltz2, 2.0 * ltz2 * ltz2) - div_z(Yi, xi * X);
     df = 1 - dpstar;
     return f;
 }
@@ -145,7 +159,7 @@ BH_HD double c2p_f(double gm, double xi, double Ssq, double Bsq, double SB, doub
     double rho = Dn / ltz;
     double G = gm - 1.0;
     double p = (G / gm) * (xi / ltz2 - Dn / ltz);
-    double pstar = p + K / (2 * ltz2) + Yi / (2 * X);
+    double pstar = p + div_z(K, 2 * ltz2) + div_z(Yi, 2 * X);
     if (p < 0.0 || rho < 0.0) return 1e32;
     return xi - taun - Dn - pstar;
 }
@@ -348,7 +362,7 @@ BH_HD void ctop_setup(const GPt& m, const Cons& cv, C2P& s) {
     double S[3];
     s.Bsq = 0.0; s.Ssq = 0.0; s.SB = 0.0;
     for (int i = 0; i < 3; i++) {
-        s.B[i] = cv.B[i] / sg;
+        s.B[i] = div_z(cv.B[i], sg);
         S[i] = cv.S[i] / sg;
         double B2 = (i == 2) ? s.B[2] : 0.0;
         double S2 = (i == 2) ? S[2] : 0.0;
@@ -431,7 +445,7 @@ BH_HD bool ctop_finish(double gm, const C2P& s, double xi, Prim& pv, int& nf) {
     eos_enth(gm, pv.rho, pv.eps, nf);
     eos_temp(pv.rho, p, nf);
     for (int i = 0; i < 3; i++) {
-        pv.v[i] = (s.Su[i] + (SB / xi) * s.Bu[i]) / xpQ;
+        pv.v[i] = (s.Su[i] + div_z(SB, xi) * s.Bu[i]) / xpQ;
         pv.B[i] = s.B[i];
     }
     if (pv.rho <= 0.0 || pv.eps <= 0.0 || p <= 0.0) return false;
