@@ -106,8 +106,15 @@ struct KP {
     // CST_FAIL (+1 when the failing call counted equation-of-state faults,
     // always 3 then): con2prim of the current U fails; W and U are untouched;
     // or CST_DONE: the con2prim phase's work on the zone is already done (a
-    // failure applied early where no ghost fill reads the zone).
+    // failure applied early where no ghost fill reads the zone).  A ghost
+    // cell may hold CST_FLOOR: W is the floor state of the cell (floored rho
+    // and eps, v = B = +0) and U its ptoc, as the ghost failure path leaves
+    // them; flc then tells the con2prim phase its outcome.
     unsigned char* cst;
+    // [nb][8000] con2prim of the cell's floor state (ptoc of rho, eps
+    // floored, v = B = +0, the cell's tree metric): 0 = solved, else 1 + the
+    // equation-of-state faults of the failed call.  Static per cell.
+    const unsigned char* flc;
     double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
     // Cells whose con2prim needs the Brent fallback are queued by the first
     // pass (Newton for every cell) and finished by k_brent over a compact
@@ -126,6 +133,7 @@ struct KP {
 enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
 constexpr unsigned char CST_FAIL = 2;
 constexpr unsigned char CST_DONE = 4;
+constexpr unsigned char CST_FLOOR = 5;
 // status of a failed con2prim and the faults it counted (0 or 3: the three
 // equation-of-state calls of the primitive recovery all see the same rho)
 __device__ __forceinline__ unsigned char fail_status(int nf) { return CST_FAIL + (nf > 0 ? 1 : 0); }
@@ -275,8 +283,18 @@ __device__ __forceinline__ void c2p_floor(const KP& P, int b, int c, const GPt& 
     storeW(P, b, c, f);
     storeU(P, b, c, ptoc(m, P.gamma, f, nf));
 }
-// ghost con2prim failure path (grid.cpp:559-566): rho, eps floored, v and B kept
-__device__ __forceinline__ void ghost_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
+// true when v and B are all +0.0 (bit patterns zero): W with floored rho
+// and eps is then exactly the cell's floor state
+__device__ __forceinline__ bool vb_zero(const Prim& w) {
+    unsigned long long o = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        o |= (unsigned long long)__double_as_longlong(w.v[d]) | (unsigned long long)__double_as_longlong(w.B[d]);
+    return o == 0;
+}
+// ghost con2prim failure path (grid.cpp:559-566): rho, eps floored, v and B
+// kept.  Returns the memo state of the result (CST_FLOOR or 0).
+__device__ __forceinline__ unsigned char ghost_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
     const double* fl = P.gfl + (size_t)b * 2 * NCELL;
     Prim w = loadW(P, b, c);
     w.rho = fl[c];
@@ -284,6 +302,7 @@ __device__ __forceinline__ void ghost_floor(const KP& P, int b, int c, const GPt
     Wa(P, b, WRHO, c) = w.rho;
     Wa(P, b, WEPS, c) = w.eps;
     storeU(P, b, c, ptoc(m, P.gamma, w, nf));
+    return vb_zero(w) ? CST_FLOOR : 0;
 }
 // First con2prim pass: 1 = solved (pv set), 0 = failed, -1 = queued.  A
 // Brent fallback whose outcome is known without its loop (the atmosphere
@@ -376,8 +395,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                 }
                 if (st == CST_FAIL || st == CST_FAIL + 1) {
                     nf = fail_faults(st);
-                    cst_at(P, b, c) = 0;
-                    ghost_floor(P, b, c, cell_tree(P, b, c), nf);
+                    cst_at(P, b, c) = ghost_floor(P, b, c, cell_tree(P, b, c), nf);
                     add_faults(P, nf);
                     return;
                 }
@@ -430,8 +448,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
         storeW(P, b, c, pv);
         cst_at(P, b, c) = 1;
     } else if (r == 0) {
-        cst_at(P, b, c) = 0;
-        ghost_floor(P, b, c, m, nf);
+        cst_at(P, b, c) = ghost_floor(P, b, c, m, nf);
     }
     add_faults(P, nf);
 }
@@ -446,6 +463,15 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     const int c = P.cell_order[t];
     const unsigned char st = cst_at(P, b, c);
     if (st == 1 || st == CST_DONE) return;
+    if (st == CST_FLOOR) {
+        // con2prim of the floor state: where it fails, the failure path
+        // (c2p_floor) would store this same W and U again
+        const unsigned char code = P.flc[(size_t)b * NCELL + c];
+        if (code) {
+            add_faults(P, code - 1);
+            return;
+        }
+    }
     int nf = 0;
     GPt m = cell_tree(P, b, c);
     if (st == CST_FAIL || st == CST_FAIL + 1) {
@@ -494,7 +520,7 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     wg.eps = fl[NCELL + g];
     storeW(P, B, g, wg);
     storeU(P, B, g, ptoc(m, P.gamma, wg, nf));
-    cst_at(P, B, g) = 0;
+    cst_at(P, B, g) = vb_zero(wg) ? CST_FLOOR : 0;
     add_faults(P, nf);
 }
 __device__ __forceinline__ bool z_layer(int k) { return k < GH || k >= BLK - GH; }
@@ -631,8 +657,7 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
             c2p_floor(P, b, c, m, nf);
             cst_at(P, b, c) = 0;
         } else if (kind == Q_GHOST) {
-            ghost_floor(P, b, c, m, nf);
-            cst_at(P, b, c) = 0;
+            cst_at(P, b, c) = ghost_floor(P, b, c, m, nf);
         } else {
             // prolonged ghost: c2p floor, then the ghost con2prim of the new U
             c2p_floor(P, b, c, m, nf);
@@ -640,8 +665,7 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
                 storeW(P, b, c, pv);
                 cst_at(P, b, c) = 1;
             } else {
-                ghost_floor(P, b, c, m, nf);
-                cst_at(P, b, c) = 0;
+                cst_at(P, b, c) = ghost_floor(P, b, c, m, nf);
             }
         }
         add_faults(P, nf);
@@ -650,6 +674,22 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
 #undef FOR_QUEUED
 __global__ void k_qreset(KP P) {
     if (threadIdx.x < NSUBQ) P.qn[threadIdx.x * QN_STRIDE] = 0u;
+}
+
+// Outcome of the con2prim of every cell's floor state (the full con2prim,
+// conserved.cpp:125-229, of ptoc of the floor primitives), for CST_FLOOR.
+__global__ __launch_bounds__(TPB) void k_floor_c2p(KP P, unsigned char* __restrict__ out) {
+    const int b = blockIdx.x / WG_CELL;
+    const int c = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
+    if (c >= NCELL) return;
+    const double* fl = P.gfl + (size_t)b * 2 * NCELL;
+    const GPt m = cell_tree(P, b, c);
+    int nf = 0;
+    const Cons u = ptoc(m, P.gamma, floor_prim(fl[c], fl[NCELL + c]), nf);
+    nf = 0;
+    Prim pv;
+    const bool ok = ctop(m, P.gamma, u, pv, nf);
+    out[(size_t)b * NCELL + c] = ok ? 0 : (unsigned char)(1 + nf);
 }
 
 // ln(rho) and ln(eps) of every stored cell, for the log-space reconstruction
@@ -1637,6 +1677,7 @@ public:
     DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
     DBuf<double> dts;               // per-step dt of a multi-step run
     DBuf<unsigned char> cst;        // [nb][8000] con2prim-current flags
+    DBuf<unsigned char> flc;        // [nb][8000] con2prim outcome of each cell's floor state
     DBuf<unsigned long long> dtpart;  // per-workgroup dt candidates
     DBuf<double> lg;                // [nb][2][8000] log(rho), log(eps)
     DBuf<unsigned int> qn;          // Brent sub-queue lengths
@@ -1722,6 +1763,7 @@ public:
         P.cell_order = cell_order.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
+        P.flc = flc.p;
         P.lg = lg.p;
         P.qn = qn.p; P.qcap = qcap; P.qb = qb.p; P.qc = qc.p; P.qx = qx.p; P.qs = qs.p;
         P.qstat = qstat.p;
@@ -1795,6 +1837,12 @@ public:
             qn.alloc((size_t)NSUBQ * QN_STRIDE);
             HIPCHK(hipMemsetAsync(qn.p, 0, qn.n * sizeof(unsigned int), stream));
             qb.alloc(total); qc.alloc(total); qx.alloc(total); qs.alloc(total * QS_WORDS);
+        }
+        // floor-state con2prim outcomes (geometry and floors are in place)
+        flc.alloc((size_t)std::max(nb, 1) * NCELL);
+        if (nb > 0) {
+            hipLaunchKernelGGL(k_floor_c2p, dim3(nb * WG_CELL), dim3(TPB), 0, stream, kp(), flc.p);
+            HIPCHK(hipGetLastError());
         }
     }
 
