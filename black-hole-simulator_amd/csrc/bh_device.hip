@@ -42,6 +42,10 @@ struct DevError {
 constexpr int TPB = 256;
 constexpr int WG_GHOST = (NGHOST + TPB - 1) / TPB;   // 16
 constexpr int WG_CELL = (NCELL + TPB - 1) / TPB;     // 32
+// active zones within two cells of a block face (the rest, 12^3, are deep)
+constexpr int NBORDER = NINT - (BLK - 2 * GH) * (BLK - 2 * GH) * (BLK - 2 * GH);  // 2368
+constexpr int NC2P = NGHOST + NBORDER;                                             // 6272
+constexpr int WG_C2P = (NC2P + TPB - 1) / TPB;                                     // 25
 constexpr int WG_FACE = (3 * NFC + TPB - 1) / TPB;   // 51
 constexpr int WG_EDGE = (19 * 19 * 19 + TPB - 1) / TPB;  // 27
 constexpr int WG_BF = (18 * 18 * 18 + TPB - 1) / TPB;    // 23
@@ -93,8 +97,9 @@ struct KP {
     // leaf's low (0) / high (1) z layers and are filled by push_zghost, or -1
     const int* push_to;
     // [8000] visiting order of the ghost fill (its first 3904 entries) and
-    // of the con2prim phase: the ghost cells in storage order, then the
-    // active zones (neighbouring lanes touch neighbouring addresses; the
+    // of the con2prim phase (its first 6272): the ghost cells in storage
+    // order, then the border active zones, then the deep ones, each in
+    // storage order (neighbouring lanes touch neighbouring addresses; the
     // z-side ghost rows, two cells per side and row, meet across rows)
     const short* cell_order;
     unsigned long long* dtbits;  // dt as an ordered bit pattern
@@ -454,12 +459,13 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
 }
 
 // con2prim on every stored cell of every leaf, integrator.cpp:201-222.
-// Ghost cells first, in storage order, then active zones (mostly settled by
-// the early pass).
+// Ghost cells first, in storage order, then the border active zones (mostly
+// settled by the early pass).  The deep zones are not visited: the early
+// pass leaves each of them solved (memo 1) or floored (CST_DONE).
 __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leaves) {
-    const int b = leaves[blockIdx.x / WG_CELL];
-    const int t = (blockIdx.x % WG_CELL) * TPB + threadIdx.x;
-    if (t >= NCELL) return;
+    const int b = leaves[blockIdx.x / WG_C2P];
+    const int t = (blockIdx.x % WG_C2P) * TPB + threadIdx.x;
+    if (t >= NC2P) return;
     const int c = P.cell_order[t];
     const unsigned char st = cst_at(P, b, c);
     if (st == 1 || st == CST_DONE) return;
@@ -526,7 +532,7 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
 __device__ __forceinline__ bool z_layer(int k) { return k < GH || k >= BLK - GH; }
 // Active zones no ghost fill reads: copies and mirrors read the two layers
 // next to a face, prolongation the coarse layers -1, 0, 1 (quirk Q25).
-__device__ __forceinline__ bool deep_zone(int i, int j, int k) {
+__host__ __device__ __forceinline__ bool deep_zone(int i, int j, int k) {
     return i >= GH && i < BLK - GH && j >= GH && j < BLK - GH && k >= GH && k < BLK - GH;
 }
 
@@ -1873,14 +1879,19 @@ public:
             org.alloc(o.size()); org.upload(o.data(), o.size(), stream);
         }
         tables(t, &g);
-        // ghost cells in storage order, then the active zones
+        // ghost cells in storage order, then the border and the deep active zones
         std::vector<short> order;
         for (int c = 0; c < NCELL; c++) {
             int i = c / 400 - GH, j = (c / 20) % 20 - GH, k = c % 20 - GH;
             if (i < 0 || i >= BLK || j < 0 || j >= BLK || k < 0 || k >= BLK) order.push_back((short)c);
         }
         if ((int)order.size() != NGHOST) throw DevError("ghost cell table size");
-        for (int a = 0; a < NINT; a++) order.push_back((short)lindx(a / 256, (a / 16) % 16, a % 16));
+        for (int pass = 0; pass < 2; pass++)
+            for (int a = 0; a < NINT; a++) {
+                const int i = a / 256, j = (a / 16) % 16, k = a % 16;
+                if (deep_zone(i, j, k) == (pass == 1)) order.push_back((short)lindx(i, j, k));
+            }
+        if ((int)order.size() != NCELL) throw DevError("cell order table size");
         cell_order.alloc(order.size()); cell_order.upload(order.data(), order.size(), stream);
         scal.alloc(2);
         HIPCHK(hipMemsetAsync(scal.p, 0, 2 * sizeof(unsigned long long), stream));
@@ -2080,7 +2091,7 @@ public:
         }
         if (nl) {
             launch("k_qreset", k_qreset, dim3(1), P);
-            launch("k_c2p", k_c2p, dim3(nl * WG_CELL), P, (const int*)leaves.p);
+            launch("k_c2p", k_c2p, dim3(nl * WG_C2P), P, (const int*)leaves.p);
             launch("k_brent", k_brent, dim3(BRENT_WG), P, (int)QS_C2P);
         }
         launch("k_logs", k_logs, dim3(nb * WG_CELL), P);
