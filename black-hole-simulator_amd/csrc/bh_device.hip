@@ -317,8 +317,9 @@ __device__ __forceinline__ int c2p_first(const KP& P, int b, int c, const GPt& m
     const Cons u = loadU(P, b, c);
     ctop_setup(m, u, s);
     double xi, xi_h;
-    if (!ctop_newton(P.gamma, s, xi, xi_h)) {
-        const int r = ctop_brent_short(P.gamma, s, xi_h, xi);
+    C2PEnds ends;
+    if (!ctop_newton(P.gamma, s, xi, xi_h, &ends)) {
+        const int r = ctop_brent_short(P.gamma, s, xi_h, xi, &ends);
         if (r < 0) {
             push_brent(P, b, c, kind, xi_h, u, m);
             return -1;

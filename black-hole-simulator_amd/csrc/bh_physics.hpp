@@ -373,9 +373,15 @@ BH_HD void ctop_setup(const GPt& m, const Cons& cv, C2P& s) {
         s.SB += S[i] * s.Bu[i];
     }
 }
+// Residuals at the ends of the Brent bracket [D, xi_h], when the Newton
+// stage has already evaluated them (exact = the bracket was not widened).
+struct C2PEnds {
+    double fD, fh;
+    bool exact;
+};
 // Bracket [xi_l, xi_h] and Newton iteration.  Returns false when the
 // reference falls back to Brent (then xi_h is the bracket to use).
-BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h) {
+BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h, C2PEnds* ends = nullptr) {
     const int maxiter = 500;
     const double tol = 1e-10;
     const double D = s.D, tau = s.tau, Ssq = s.Ssq, Bsq = s.Bsq, SB = s.SB;
@@ -383,9 +389,11 @@ BH_HD bool ctop_newton(double gm, const C2P& s, double& xi, double& xi_h) {
     xi_h = tau + D + Bsq / 2.0 + 1e3;
     double fl = c2p_f(gm, xi_l, Ssq, Bsq, SB, D, tau);
     double fh = c2p_f(gm, xi_h, Ssq, Bsq, SB, D, tau);
+    if (ends) { ends->fD = fl; ends->fh = fh; ends->exact = true; }
     while (fl * fh > 0.0 && xi_h < 1e8 * D) {
         xi_h *= 2.0;
         fl = c2p_f_signed(gm, xi_h, Ssq, Bsq, SB, D, tau);  // fl only enters through fl * fh > 0
+        if (ends) ends->exact = false;
     }
     xi = (xi_l + xi_h) / 2.0;
     // The iterate is a pure function of the previous one, so once it is NaN,
@@ -418,11 +426,14 @@ BH_HD bool ctop_brent(double gm, const C2P& s, double xi_h, double& xi) {
 // ctop_brent where its outcome is known without the loop: the end-point
 // test, then the atmosphere form (c2p_brent_atmosphere).  1 = root xi,
 // 0 = Brent fails, -1 = the loop must run.
-BH_HD int ctop_brent_short(double gm, const C2P& s, double xi_h, double& xi) {
+// The residuals at D and xi_h are taken from the Newton stage where it
+// computed them with these same arguments.
+BH_HD int ctop_brent_short(double gm, const C2P& s, double xi_h, double& xi, const C2PEnds* ends = nullptr) {
     if (!(s.Ssq == 0.0 && s.Bsq == 0.0 && s.SB == 0.0)) return -1;
     double xa = s.D, xb = xi_h;
-    double fa = c2p_f(gm, xa, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
-    double fb = c2p_f(gm, xb, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
+    const bool known = ends && ends->exact;
+    double fa = known ? ends->fD : c2p_f(gm, xa, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
+    double fb = known ? ends->fh : c2p_f(gm, xb, s.Ssq, s.Bsq, s.SB, s.D, s.tau);
     if (fa * fb > 0.0) return 0;
     if (fabs(fa) < fabs(fb)) { dswap(xa, xb); dswap(fa, fb); }
     if (fa > 0.0 && s.D > 0.0 && xb < xa && isfinite(xa) && isfinite(s.tau))
