@@ -112,13 +112,15 @@ struct KP {
     // always 3 then): con2prim of the current U fails; W and U are untouched;
     // or CST_DONE: the con2prim phase's work on the zone is already done (a
     // failure applied early where no ghost fill reads the zone).  A ghost
-    // cell may hold CST_FLOOR: W is the floor state of the cell (floored rho
-    // and eps, v = B = +0) and U its ptoc, as the ghost failure path leaves
-    // them; flc then tells the con2prim phase its outcome.
+    // cell or an active zone after the floors may hold CST_FLOOR: W is the
+    // floor state of the cell (floored rho and eps, v = B = +0) and U its
+    // ptoc, as the failure paths leave them; flc then gives the outcome of
+    // the next con2prim of that U.
     unsigned char* cst;
     // [nb][8000] con2prim of the cell's floor state (ptoc of rho, eps
     // floored, v = B = +0, the cell's tree metric): 0 = solved, else 1 + the
-    // equation-of-state faults of the failed call.  Static per cell.
+    // equation-of-state faults of the failed con2prim + 8 x those of the
+    // floor state's ptoc (flc_faults).  Static per cell.
     const unsigned char* flc;
     double* lg;  // [nb][2][8000] log(rho), log(eps) of the current W
     // Cells whose con2prim needs the Brent fallback are queued by the first
@@ -139,6 +141,10 @@ enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
 constexpr unsigned char CST_FAIL = 2;
 constexpr unsigned char CST_DONE = 4;
 constexpr unsigned char CST_FLOOR = 5;
+// faults of a failed floor-state con2prim (flc code > 0): the con2prim alone,
+// or with the floor path's ptoc that follows it
+__device__ __forceinline__ int flc_c2p_faults(unsigned char code) { return (code - 1) & 7; }
+__device__ __forceinline__ int flc_faults(unsigned char code) { return ((code - 1) & 7) + ((code - 1) >> 3); }
 // status of a failed con2prim and the faults it counted (0 or 3: the three
 // equation-of-state calls of the primitive recovery all see the same rho)
 __device__ __forceinline__ unsigned char fail_status(int nf) { return CST_FAIL + (nf > 0 ? 1 : 0); }
@@ -475,7 +481,7 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
         // (c2p_floor) would store this same W and U again
         const unsigned char code = P.flc[(size_t)b * NCELL + c];
         if (code) {
-            add_faults(P, code - 1);
+            add_faults(P, flc_faults(code));
             return;
         }
     }
@@ -553,7 +559,22 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
     if (mirror_source(P, b, i, j, k)) return;
     const unsigned char st0 = cst_at(P, b, c);
     const GPt m = cell_tree(P, b, c);
-    if (st0) {
+    if (st0 == CST_FLOOR) {
+        // floor state left by the floors: its con2prim outcome is tabulated
+        const unsigned char code = P.flc[(size_t)b * NCELL + c];
+        if (code) {
+            if (deep_zone(i, j, k)) {
+                // the failure path would store this same W and U again
+                add_faults(P, flc_faults(code));
+                cst_at(P, b, c) = CST_DONE;
+                return;
+            }
+            const unsigned char st = fail_status(flc_c2p_faults(code));
+            cst_at(P, b, c) = st;
+            if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
+            return;
+        }
+    } else if (st0) {
         // outcome already known for this U
         if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st0);
         return;
@@ -595,7 +616,7 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
         f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
         storeW(P, b, c, f);
         storeU(P, b, c, ptoc(m, P.gamma, f, nf));
-        st = 0;
+        st = vb_zero(f) ? CST_FLOOR : 0;
     }
     cst_at(P, b, c) = st;
 }
@@ -691,12 +712,11 @@ __global__ __launch_bounds__(TPB) void k_floor_c2p(KP P, unsigned char* __restri
     if (c >= NCELL) return;
     const double* fl = P.gfl + (size_t)b * 2 * NCELL;
     const GPt m = cell_tree(P, b, c);
-    int nf = 0;
-    const Cons u = ptoc(m, P.gamma, floor_prim(fl[c], fl[NCELL + c]), nf);
-    nf = 0;
+    int nfp = 0, nf = 0;
+    const Cons u = ptoc(m, P.gamma, floor_prim(fl[c], fl[NCELL + c]), nfp);
     Prim pv;
     const bool ok = ctop(m, P.gamma, u, pv, nf);
-    out[(size_t)b * NCELL + c] = ok ? 0 : (unsigned char)(1 + nf);
+    out[(size_t)b * NCELL + c] = ok ? 0 : (unsigned char)(1 + nf + 8 * nfp);
 }
 
 // ln(rho) and ln(eps) of every stored cell, for the log-space reconstruction
