@@ -109,7 +109,8 @@ struct KP {
     // the same W).  Every kernel that writes U keeps it current.  Between the
     // early con2prim pass and the con2prim phase, an active zone may also hold
     // CST_FAIL (+1 when the failing call counted equation-of-state faults,
-    // always 3 then): con2prim of the current U fails; W and U are untouched;
+    // always 3 then): con2prim of the current U fails; W and U are untouched
+    // (CST_FLFAIL, +1 likewise, where they already hold the floor state);
     // or CST_DONE: the con2prim phase's work on the zone is already done (a
     // failure applied early where no ghost fill reads the zone).  A ghost
     // cell or an active zone after the floors may hold CST_FLOOR: W is the
@@ -141,6 +142,9 @@ enum { Q_C2P = 0, Q_GHOST = 1, Q_PROLONG = 2, Q_FLOOR = 3, Q_PRE = 4 };
 constexpr unsigned char CST_FAIL = 2;
 constexpr unsigned char CST_DONE = 4;
 constexpr unsigned char CST_FLOOR = 5;
+// CST_FAIL for a zone that already holds its floor state (CST_FLOOR before
+// the early pass), so that the failure path's floor would change nothing
+constexpr unsigned char CST_FLFAIL = 6;
 // faults of a failed floor-state con2prim (flc code > 0): the con2prim alone,
 // or with the floor path's ptoc that follows it
 __device__ __forceinline__ int flc_c2p_faults(unsigned char code) { return (code - 1) & 7; }
@@ -148,7 +152,11 @@ __device__ __forceinline__ int flc_faults(unsigned char code) { return ((code - 
 // status of a failed con2prim and the faults it counted (0 or 3: the three
 // equation-of-state calls of the primitive recovery all see the same rho)
 __device__ __forceinline__ unsigned char fail_status(int nf) { return CST_FAIL + (nf > 0 ? 1 : 0); }
-__device__ __forceinline__ int fail_faults(unsigned char st) { return st > CST_FAIL ? 3 : 0; }
+__device__ __forceinline__ int fail_faults(unsigned char st) { return (st & 1) ? 3 : 0; }
+// CST_FAIL, CST_FLFAIL, each + 1 with faults
+__device__ __forceinline__ bool is_fail(unsigned char st) {
+    return (st & ~1) == CST_FAIL || (st & ~1) == CST_FLFAIL;
+}
 
 constexpr int chr_slot(int m, int n, int l) {
     int lo = n < l ? n : l, hi = n < l ? l : n;
@@ -405,7 +413,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                     cst_at(P, b, c) = 1;
                     return;
                 }
-                if (st == CST_FAIL || st == CST_FAIL + 1) {
+                if (is_fail(st)) {
                     nf = fail_faults(st);
                     cst_at(P, b, c) = ghost_floor(P, b, c, cell_tree(P, b, c), nf);
                     add_faults(P, nf);
@@ -484,6 +492,13 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
             add_faults(P, flc_faults(code));
             return;
         }
+    }
+    if ((st & ~1) == CST_FLFAIL) {
+        // known failure of a zone already in its floor state: the floor
+        // path would store the same W and U again
+        add_faults(P, flc_faults(P.flc[(size_t)b * NCELL + c]));
+        cst_at(P, b, c) = 0;
+        return;
     }
     int nf = 0;
     GPt m = cell_tree(P, b, c);
@@ -569,7 +584,7 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
                 cst_at(P, b, c) = CST_DONE;
                 return;
             }
-            const unsigned char st = fail_status(flc_c2p_faults(code));
+            const unsigned char st = fail_status(flc_c2p_faults(code)) + (CST_FLFAIL - CST_FAIL);
             cst_at(P, b, c) = st;
             if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
             return;
