@@ -401,6 +401,15 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                 w[v] = Wa(P, src_b, v, src_c);
             }
             const unsigned char st = cst_at(P, src_b, src_c);
+            const bool reuse = kind == GH_COPY && P.ghsame[b * 6 + 2 * dim + s];
+            // A source that keeps its floor state (v = B = +0) with a known
+            // failure, copied into a ghost that holds its own floor state: the
+            // copy and the ghost floor path below would leave the ghost as it is.
+            if (reuse && (st & ~1) == CST_FLFAIL && cst_at(P, b, c) == CST_FLOOR) {
+                nf = fail_faults(st) + (P.gfl[(size_t)b * 2 * NCELL + c] <= 0.0 ? 2 : 0);
+                add_faults(P, nf);
+                return;
+            }
             for (int v = 0; v < 8; v++) {
                 Ua(P, b, v, c) = u[v];
                 Wa(P, b, v, c) = w[v];
@@ -408,7 +417,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             // The source holds this same U at the same metric bits, so its
             // known con2prim outcome is the ghost's: success returns the
             // source's W (copied above), failure takes the ghost floor path.
-            if (kind == GH_COPY && P.ghsame[b * 6 + 2 * dim + s]) {
+            if (reuse) {
                 if (st == 1) {
                     cst_at(P, b, c) = 1;
                     return;
