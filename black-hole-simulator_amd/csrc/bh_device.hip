@@ -561,6 +561,21 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     add_faults(P, nf);
 }
 __device__ __forceinline__ bool z_layer(int k) { return k < GH || k >= BLK - GH; }
+// push_zghost of a zone that keeps its floor state with a known failure
+// (CST_FLFAIL): a destination ghost that holds its own floor state would be
+// left as it is, so only the faults of the ghost floor path are counted.
+__device__ __forceinline__ void push_zfloor(const KP& P, int b, int i, int j, int k, const GPt& m, unsigned char st) {
+    const int lo = k < GH;
+    const int B = P.push_to[b * 2 + (lo ? 0 : 1)];
+    if (B < 0 || !P.ghsame[B * 6 + (lo ? 5 : 4)]) return;
+    const int g = lindx(i, j, lo ? k + BLK : k - BLK);
+    if (cst_at(P, B, g) == CST_FLOOR) {
+        add_faults(P, fail_faults(st) + (P.gfl[(size_t)B * 2 * NCELL + g] <= 0.0 ? 2 : 0));
+        return;
+    }
+    const int c = lindx(i, j, k);
+    push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
+}
 // Active zones no ghost fill reads: copies and mirrors read the two layers
 // next to a face, prolongation the coarse layers -1, 0, 1 (quirk Q25).
 __host__ __device__ __forceinline__ bool deep_zone(int i, int j, int k) {
@@ -595,7 +610,7 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
             }
             const unsigned char st = fail_status(flc_c2p_faults(code)) + (CST_FLFAIL - CST_FAIL);
             cst_at(P, b, c) = st;
-            if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), loadW(P, b, c), st);
+            if (z_layer(k)) push_zfloor(P, b, i, j, k, m, st);
             return;
         }
     } else if (st0) {
