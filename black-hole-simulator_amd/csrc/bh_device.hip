@@ -402,10 +402,11 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
             }
             const unsigned char st = cst_at(P, src_b, src_c);
             const bool reuse = kind == GH_COPY && P.ghsame[b * 6 + 2 * dim + s];
-            // A source that keeps its floor state (v = B = +0) with a known
-            // failure, copied into a ghost that holds its own floor state: the
-            // copy and the ghost floor path below would leave the ghost as it is.
-            if (reuse && (st & ~1) == CST_FLFAIL && cst_at(P, b, c) == CST_FLOOR) {
+            // A source with a known failure and v = B = +0, copied into a
+            // ghost that holds its own floor state: the copy and the ghost
+            // floor path below would leave the ghost as it is.
+            if (reuse && is_fail(st) && cst_at(P, b, c) == CST_FLOOR &&
+                ((st & ~1) == CST_FLFAIL || vb_zero(Prim{w[0], w[1], {w[2], w[3], w[4]}, {w[5], w[6], w[7]}}))) {
                 nf = fail_faults(st) + (P.gfl[(size_t)b * 2 * NCELL + c] <= 0.0 ? 2 : 0);
                 add_faults(P, nf);
                 return;
@@ -552,6 +553,11 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     }
     int nf = fail_faults(st);
     const double* fl = P.gfl + (size_t)B * 2 * NCELL;
+    if (vb_zero(w) && cst_at(P, B, g) == CST_FLOOR) {
+        // the ghost already holds the floor state this path would store
+        add_faults(P, nf + (fl[g] <= 0.0 ? 2 : 0));
+        return;
+    }
     Prim wg = w;
     wg.rho = fl[g];
     wg.eps = fl[NCELL + g];
