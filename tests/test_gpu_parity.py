@@ -119,7 +119,9 @@ def _run_parity(cfg, nsteps, name):
             dt_g = None
             fault_g = e.faults
         dt_o, fault_o = ot.step(configs.CFL)
-        assert (fault_g > 0) == (fault_o > 0), f"step {step}: faults gpu={fault_g} oracle={fault_o}"
+        # equation-of-state calls with rho <= 0, counted call for call (the
+        # device's memoised and tabulated con2prim outcomes carry their counts)
+        assert fault_g == fault_o, f"step {step}: faults gpu={fault_g} oracle={fault_o}"
         g = snapshot(t)
         o = ot.state()
         for nm, x, y in (("U", g.U, o.U), ("W", g.W, o.W), ("Bf", g.Bf, o.Bf)):
