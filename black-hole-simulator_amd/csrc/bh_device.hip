@@ -294,12 +294,25 @@ __device__ __forceinline__ void queued_setup(const KP& P, size_t i, C2P& s) {
     m.sqrtdetgam = q[8 * st]; m.gi13 = q[9 * st]; m.gi33 = q[10 * st];
     ctop_setup(m, u, s);
 }
+// storeW of a floor state whose B was read from this W: B is left in place,
+// and rho, eps and v are stored where they differ from what W holds (a zone
+// floored again keeps the same words)
+__device__ __forceinline__ void store_floorW(const KP& P, int b, int c, const Prim& f) {
+    double* w = P.W + (size_t)b * 8 * NCELL + c;
+    const double r = w[WRHO * NCELL], e = w[WEPS * NCELL];
+    const double v0 = w[WV0 * NCELL], v1 = w[WV1 * NCELL], v2 = w[WV2 * NCELL];
+    store_changed(w + WRHO * NCELL, r, f.rho);
+    store_changed(w + WEPS * NCELL, e, f.eps);
+    store_changed(w + WV0 * NCELL, v0, f.v[0]);
+    store_changed(w + WV1 * NCELL, v1, f.v[1]);
+    store_changed(w + WV2 * NCELL, v2, f.v[2]);
+}
 // c2p failure path (integrator.cpp:211-218): floor state with the cell's B
 __device__ __forceinline__ void c2p_floor(const KP& P, int b, int c, const GPt& m, int& nf) {
     const double* fl = P.gfl + (size_t)b * 2 * NCELL;
     Prim f = floor_prim(fl[c], fl[NCELL + c]);
     f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
-    storeW(P, b, c, f);
+    store_floorW(P, b, c, f);
     storeU(P, b, c, ptoc(m, P.gamma, f, nf));
 }
 // true when v and B are all +0.0 (bit patterns zero): W with floored rho
@@ -659,7 +672,7 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
     if (bhl || Wa(P, b, WRHO, c) <= fl[c]) {
         Prim f = floor_prim(fl[c], fl[NCELL + c]);
         f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
-        storeW(P, b, c, f);
+        store_floorW(P, b, c, f);
         storeU(P, b, c, ptoc(m, P.gamma, f, nf));
         st = vb_zero(f) ? CST_FLOOR : 0;
     }
