@@ -225,6 +225,17 @@ __device__ __forceinline__ double gcc_at(const KP& P, int b, int v, int c) {
 __device__ __forceinline__ void store_changed(double* p, double old, double x) {
     if (__double_as_longlong(x) != __double_as_longlong(old)) *p = x;
 }
+// storeW of a con2prim result: its B (U.B / sqrt(gamma)) mostly has the bits
+// W.B already holds (always where B = 0), so B is stored where it differs
+__device__ __forceinline__ void storeW_c2p(const KP& P, int b, int c, const Prim& w) {
+    double* p = P.W + (size_t)b * 8 * NCELL + c;
+    const double o0 = p[WB0 * NCELL], o1 = p[WB1 * NCELL], o2 = p[WB2 * NCELL];
+    p[WRHO * NCELL] = w.rho; p[WEPS * NCELL] = w.eps;
+    p[WV0 * NCELL] = w.v[0]; p[WV1 * NCELL] = w.v[1]; p[WV2 * NCELL] = w.v[2];
+    store_changed(p + WB0 * NCELL, o0, w.B[0]);
+    store_changed(p + WB1 * NCELL, o1, w.B[1]);
+    store_changed(p + WB2 * NCELL, o2, w.B[2]);
+}
 __device__ __forceinline__ void add_faults(const KP& P, int nf) {
     if (nf) atomicAdd(P.faults, (unsigned long long)nf);
 }
@@ -488,7 +499,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
     Prim pv;
     int r = c2p_first(P, b, c, m, Q_GHOST, pv, nf);
     if (r > 0) {
-        storeW(P, b, c, pv);
+        storeW_c2p(P, b, c, pv);
         cst_at(P, b, c) = 1;
     } else if (r == 0) {
         cst_at(P, b, c) = ghost_floor(P, b, c, m, nf);
@@ -536,7 +547,7 @@ __global__ __launch_bounds__(TPB) void k_c2p(KP P, const int* __restrict__ leave
     Prim pv;
     int r = c2p_first(P, b, c, m, Q_C2P, pv, nf);
     if (r > 0) {
-        storeW(P, b, c, pv);
+        storeW_c2p(P, b, c, pv);
         cst_at(P, b, c) = 1;
     } else if (r == 0) {
         c2p_floor(P, b, c, m, nf);
@@ -641,7 +652,7 @@ __global__ __launch_bounds__(TPB) void k_c2p_pre(KP P, const int* __restrict__ l
     Prim pv;
     const int r = c2p_first(P, b, c, m, Q_PRE, pv, nf);
     if (r > 0) {
-        storeW(P, b, c, pv);
+        storeW_c2p(P, b, c, pv);
         cst_at(P, b, c) = 1;
         if (z_layer(k)) push_zghost(P, b, i, j, k, m, loadU(P, b, c), pv, 1);
     } else if (r == 0) {
@@ -666,7 +677,7 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
     unsigned char st = 0;
     if (!bhl && have_pv) {
         bool hl = !(pv.rho <= 0.0 || pv.eps <= 0.0);
-        if (hl) { storeW(P, b, c, pv); st = 1; }
+        if (hl) { storeW_c2p(P, b, c, pv); st = 1; }
     }
     const double* fl = P.gfl + (size_t)b * 2 * NCELL;
     if (bhl || Wa(P, b, WRHO, c) <= fl[c]) {
@@ -718,7 +729,7 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
             unsigned char st = 1;
             const int ci = c / 400 - GH, cj = (c / 20) % 20 - GH, ck = c % 20 - GH;
             if (ok) {
-                storeW(P, b, c, pv);
+                storeW_c2p(P, b, c, pv);
             } else if (deep_zone(ci, cj, ck)) {
                 c2p_floor(P, b, c, cell_tree(P, b, c), nf);
                 cst_at(P, b, c) = CST_DONE;
@@ -737,7 +748,7 @@ __global__ __launch_bounds__(TPB) void k_brent(KP P, int pass) {
         if (kind == Q_FLOOR) {
             floors_finish(P, b, c, m, !ok, pv, ok, nf);
         } else if (ok) {
-            storeW(P, b, c, pv);
+            storeW_c2p(P, b, c, pv);
             cst_at(P, b, c) = 1;
         } else if (kind == Q_C2P) {
             c2p_floor(P, b, c, m, nf);
