@@ -225,6 +225,18 @@ __device__ __forceinline__ double gcc_at(const KP& P, int b, int v, int c) {
 __device__ __forceinline__ void store_changed(double* p, double old, double x) {
     if (__double_as_longlong(x) != __double_as_longlong(old)) *p = x;
 }
+// storeU of the ptoc of a floored state: its U.B (sqrt(gamma) B) keeps the
+// bits U.B holds wherever B = 0, so U.B is stored where it differs
+__device__ __forceinline__ void storeU_floor(const KP& P, int b, int c, const Cons& q) {
+    double* p = P.U + (size_t)b * 8 * NCELL + c;
+    const double o0 = p[UB0 * NCELL], o1 = p[UB1 * NCELL], o2 = p[UB2 * NCELL];
+    p[UD * NCELL] = q.D;
+    p[US0 * NCELL] = q.S[0]; p[US1 * NCELL] = q.S[1]; p[US2 * NCELL] = q.S[2];
+    p[UTAU * NCELL] = q.tau;
+    store_changed(p + UB0 * NCELL, o0, q.B[0]);
+    store_changed(p + UB1 * NCELL, o1, q.B[1]);
+    store_changed(p + UB2 * NCELL, o2, q.B[2]);
+}
 // storeW of a con2prim result: its B (U.B / sqrt(gamma)) mostly has the bits
 // W.B already holds (always where B = 0), so B is stored where it differs
 __device__ __forceinline__ void storeW_c2p(const KP& P, int b, int c, const Prim& w) {
@@ -324,7 +336,7 @@ __device__ __forceinline__ void c2p_floor(const KP& P, int b, int c, const GPt& 
     Prim f = floor_prim(fl[c], fl[NCELL + c]);
     f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
     store_floorW(P, b, c, f);
-    storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+    storeU_floor(P, b, c, ptoc(m, P.gamma, f, nf));
 }
 // true when v and B are all +0.0 (bit patterns zero): W with floored rho
 // and eps is then exactly the cell's floor state
@@ -344,7 +356,7 @@ __device__ __forceinline__ unsigned char ghost_floor(const KP& P, int b, int c, 
     w.eps = fl[NCELL + c];
     Wa(P, b, WRHO, c) = w.rho;
     Wa(P, b, WEPS, c) = w.eps;
-    storeU(P, b, c, ptoc(m, P.gamma, w, nf));
+    storeU_floor(P, b, c, ptoc(m, P.gamma, w, nf));
     return vb_zero(w) ? CST_FLOOR : 0;
 }
 // First con2prim pass: 1 = solved (pv set), 0 = failed, -1 = queued.  A
@@ -586,7 +598,7 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     wg.rho = fl[g];
     wg.eps = fl[NCELL + g];
     storeW(P, B, g, wg);
-    storeU(P, B, g, ptoc(m, P.gamma, wg, nf));
+    storeU_floor(P, B, g, ptoc(m, P.gamma, wg, nf));
     cst_at(P, B, g) = vb_zero(wg) ? CST_FLOOR : 0;
     add_faults(P, nf);
 }
@@ -684,7 +696,7 @@ __device__ __forceinline__ void floors_finish(const KP& P, int b, int c, const G
         Prim f = floor_prim(fl[c], fl[NCELL + c]);
         f.B[0] = Wa(P, b, WB0, c); f.B[1] = Wa(P, b, WB1, c); f.B[2] = Wa(P, b, WB2, c);
         store_floorW(P, b, c, f);
-        storeU(P, b, c, ptoc(m, P.gamma, f, nf));
+        storeU_floor(P, b, c, ptoc(m, P.gamma, f, nf));
         st = vb_zero(f) ? CST_FLOOR : 0;
     }
     cst_at(P, b, c) = st;
