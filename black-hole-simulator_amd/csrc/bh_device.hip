@@ -447,9 +447,17 @@ __global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blo
                 add_faults(P, nf);
                 return;
             }
-            for (int v = 0; v < 8; v++) {
-                Ua(P, b, v, c) = u[v];
-                Wa(P, b, v, c) = w[v];
+            {
+                // U.B and W.B of the ghost keep their words where the bits agree
+                double* up = P.U + (size_t)b * 8 * NCELL + c;
+                double* wp = P.W + (size_t)b * 8 * NCELL + c;
+                double ob[6];
+                for (int v = 0; v < 3; v++) { ob[v] = up[(UB0 + v) * NCELL]; ob[3 + v] = wp[(WB0 + v) * NCELL]; }
+                for (int v = 0; v < 5; v++) { up[v * NCELL] = u[v]; wp[v * NCELL] = w[v]; }
+                for (int v = 0; v < 3; v++) {
+                    store_changed(up + (UB0 + v) * NCELL, ob[v], u[UB0 + v]);
+                    store_changed(wp + (WB0 + v) * NCELL, ob[3 + v], w[WB0 + v]);
+                }
             }
             // The source holds this same U at the same metric bits, so its
             // known con2prim outcome is the ghost's: success returns the
@@ -582,8 +590,18 @@ __device__ __forceinline__ void push_zghost(const KP& P, int b, int i, int j, in
     if (B < 0 || !P.ghsame[B * 6 + (lo ? 5 : 4)]) return;
     const int g = lindx(i, j, lo ? k + BLK : k - BLK);
     if (st == 1) {
-        storeU(P, B, g, u);
-        storeW(P, B, g, w);
+        double* up = P.U + (size_t)B * 8 * NCELL + g;
+        double* wp = P.W + (size_t)B * 8 * NCELL + g;
+        const double ub0 = up[UB0 * NCELL], ub1 = up[UB1 * NCELL], ub2 = up[UB2 * NCELL];
+        const double wb0 = wp[WB0 * NCELL], wb1 = wp[WB1 * NCELL], wb2 = wp[WB2 * NCELL];
+        up[UD * NCELL] = u.D; up[US0 * NCELL] = u.S[0]; up[US1 * NCELL] = u.S[1]; up[US2 * NCELL] = u.S[2];
+        up[UTAU * NCELL] = u.tau;
+        wp[WRHO * NCELL] = w.rho; wp[WEPS * NCELL] = w.eps;
+        wp[WV0 * NCELL] = w.v[0]; wp[WV1 * NCELL] = w.v[1]; wp[WV2 * NCELL] = w.v[2];
+        store_changed(up + UB0 * NCELL, ub0, u.B[0]); store_changed(up + UB1 * NCELL, ub1, u.B[1]);
+        store_changed(up + UB2 * NCELL, ub2, u.B[2]);
+        store_changed(wp + WB0 * NCELL, wb0, w.B[0]); store_changed(wp + WB1 * NCELL, wb1, w.B[1]);
+        store_changed(wp + WB2 * NCELL, wb2, w.B[2]);
         cst_at(P, B, g) = 1;
         return;
     }
@@ -880,7 +898,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     Cons Fl = hlld(L, R, UL, UR, m, P.gamma, 0, ct, nf);
     double* Fo = P.F + ((size_t)b * 3 + dir) * 8 * NFC + lf;
     Fo[0] = Fl.D; Fo[NFC] = Fl.S[0]; Fo[2 * NFC] = Fl.S[1]; Fo[3 * NFC] = Fl.S[2];
-    Fo[4 * NFC] = Fl.tau; Fo[5 * NFC] = Fl.B[0]; Fo[6 * NFC] = Fl.B[1]; Fo[7 * NFC] = Fl.B[2];
+    Fo[4 * NFC] = Fl.tau;
+    // the field fluxes keep their words where the bits agree (field-free regions)
+    const double f5 = Fo[5 * NFC], f6 = Fo[6 * NFC], f7 = Fo[7 * NFC];
+    store_changed(Fo + 5 * NFC, f5, Fl.B[0]);
+    store_changed(Fo + 6 * NFC, f6, Fl.B[1]);
+    store_changed(Fo + 7 * NFC, f7, Fl.B[2]);
     add_faults(P, nf);
 }
 
