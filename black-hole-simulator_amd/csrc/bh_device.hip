@@ -2315,6 +2315,8 @@ public:
         }
         KP P = kp();
         HIPCHK(hipMemsetAsync(Bf.p, 0, Bf.n * sizeof(double), stream));
+        // every cell gets a new state: no con2prim outcome is known for it
+        HIPCHK(hipMemsetAsync(cst.p, 0, cst.n, stream));
         launch("k_ic_atmosphere", k_ic_atmosphere, dim3(nb * WG_CELL), P);
         DBuf<unsigned long long> red;
         red.alloc(2);

@@ -288,6 +288,27 @@ def test_initial_data_on_device_matches_oracle(name):
         assert np.array_equal(np.signbit(x), np.signbit(y)), (name, nm, "signed zeros")
 
 
+def test_initial_data_over_stepped_tree():
+    """Device initial data written over a tree that has already stepped, then
+    one step, equals the same on a fresh tree bit for bit: no con2prim outcome
+    remembered for the old state is reused."""
+    t, _, _ = _device_initial("R4")
+    for _ in range(2):
+        try:
+            bhfluid.RK2Integrator(configs.CFL).step(t)
+        except bhfluid.DomainError:
+            pass
+    t.init_torus_device()
+    f, _, _ = _device_initial("R4")
+    _same_state(f, t)
+    for tree in (t, f):
+        try:
+            bhfluid.RK2Integrator(configs.CFL).step(tree)
+        except bhfluid.DomainError:
+            pass
+    _same_state(f, t)
+
+
 @pytest.mark.parametrize("name", ["R2", "R3"])
 def test_initial_data_and_regrid_on_device(name):
     """Initial data and regrid with the state never leaving the GPU equal the
