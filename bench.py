@@ -30,8 +30,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "black-hole-simulator_amd"))
 sys.path.insert(0, str(ROOT / "tests"))
 
-METRIC = json.loads((ROOT / "BASELINE.json").read_text())["metric"] if (ROOT / "BASELINE.json").exists() else \
-    "zone-updates/s"
+METRIC = "zone-updates/s FP64 vs mixed, AMR torus @L1(\u03c1)\u22641e-5; p99 run latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
 # compulsory FP64 state traffic of one zone-update (8 evolved words read and

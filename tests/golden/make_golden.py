@@ -2,7 +2,7 @@
 """Generate the golden fixtures from the compiled reference.
 
 Runs only where oracle/_ref/libbhref.so was built (the container that holds
-the reference sources); the GPU box never runs it.  Writes:
+the reference sources); no GPU run executes it.  Writes:
 
 * pointwise.npz -- inputs and outputs of the reference's metric::comp,
   conserved::ptoc / ctop and hlld on seeded random points;
