@@ -401,7 +401,9 @@ __device__ __forceinline__ bool mirror_source(const KP& P, int b, int i, int j, 
 // fills one ghost cell from its face neighbour (copy, coarse prolongation or
 // domain-boundary mirror; edge and corner ghosts are not filled, quirk Q27),
 // then runs the ghost con2prim (failure keeps v and B, quirk Q29).
-__global__ __launch_bounds__(TPB) void k_ghost(KP P, const int* __restrict__ blocks) {
+// Four wavefronts per SIMD (128 VGPRs) measured 3 % faster than the three
+// that its register count would otherwise allow.
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_ghost(KP P, const int* __restrict__ blocks) {
     const int b = blocks[blockIdx.x / WG_GHOST];
     const int t = (blockIdx.x % WG_GHOST) * TPB + threadIdx.x;
     if (t >= NGHOST) return;
