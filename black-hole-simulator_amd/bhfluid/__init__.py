@@ -117,6 +117,7 @@ def lib():
         _sig(L, "bh_tree_init_torus_device", _i, [_v, _dp])
         _sig(L, "bh_math2_batch", _i, [_i, _i, _dp, _dp, _dp, _i])
         _sig(L, "bh_tree_use_graph", _i, [_v, _i])
+        _sig(L, "bh_tree_set_exact_copies", _i, [_v, _i])
         _sig(L, "bh_profile_only", _i, [_v, ctypes.c_char_p])
         _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
@@ -275,6 +276,11 @@ class AmrTree:
             self._check(rc, "init_torus_device")
         return out.value
 
+    def set_exact_copies(self, enable: bool = True):
+        """Testing switch: with False every copied ghost runs its own con2prim
+        (no reuse of the source zone's outcome); results are identical."""
+        self._check(self._L.bh_tree_set_exact_copies(self._h, int(bool(enable))), "set_exact_copies")
+
     def use_graph(self, enable: bool = True):
         """Replay the step from a captured HIP graph (default) or launch its
         kernels one by one; identical results."""
@@ -404,7 +410,13 @@ class RK2Integrator:
         n = ctypes.c_int(0)
         rc = tree._L.bh_rk2_advance(tree._h, self.cfl, int(nsteps if nsteps is not None else 2 ** 31 - 1),
                                     float(t_end if t_end is not None else np.inf), ctypes.byref(tt), ctypes.byref(n))
-        tree._check(rc, "advance")
+        try:
+            tree._check(rc, "advance")
+        except DomainError as e:
+            # the loop stopped after the step that faulted (where the
+            # reference's throws); that step is counted
+            e.t, e.steps = tt.value, n.value
+            raise
         return tt.value, n.value
 
 

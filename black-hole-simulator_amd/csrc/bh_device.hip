@@ -93,6 +93,11 @@ struct KP {
     const double* gth;   // [nb][8000] polar angle of each cell centre
     const int* ghtab;    // [nb][12]
     const unsigned char* ghsame;  // [nb][6] copied ghosts share their source's metric bits
+    // [nb] bit 2d+s: the two layers next to side (d, s) of the block are read
+    // by a same-level copy into a side whose ghosts do not reuse the source's
+    // con2prim outcome (ghsame 0); such a ghost keeps the source's
+    // pre-con2prim v and B when its own con2prim fails (grid.cpp:559-566)
+    const int* pmask;
     // [nb][2] the leaf whose high (0) / low (1) z-side ghosts copy this
     // leaf's low (0) / high (1) z layers and are filled by push_zghost, or -1
     const int* push_to;
@@ -383,15 +388,17 @@ __device__ __forceinline__ int cell_of(int dim, int a, int t1, int t2) {
     return dim == 0 ? lindx(a, t1, t2) : dim == 1 ? lindx(t1, a, t2) : lindx(t1, t2, a);
 }
 
-// Active zones whose U a ghost fill mirrors (domain-boundary sides): their
-// pre-con2prim W is read by the fill, so the early pass leaves them alone.
+// Active zones whose pre-con2prim W a ghost fill reads: those a
+// domain-boundary side mirrors, and those a same-level copy without outcome
+// reuse reads (pmask).  The early pass leaves them to the con2prim phase.
 __device__ __forceinline__ bool mirror_source(const KP& P, int b, int i, int j, int k) {
     const int* gt = P.ghtab + b * 12;
+    const int pm = P.pmask[b];
     const int x[3] = {i, j, k};
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        if (gt[(2 * d) * 2] == GH_BNDY && x[d] < GH) return true;
-        if (gt[(2 * d + 1) * 2] == GH_BNDY && x[d] >= BLK - GH) return true;
+        if ((gt[(2 * d) * 2] == GH_BNDY || (pm >> (2 * d) & 1)) && x[d] < GH) return true;
+        if ((gt[(2 * d + 1) * 2] == GH_BNDY || (pm >> (2 * d + 1) & 1)) && x[d] >= BLK - GH) return true;
     }
     return false;
 }
@@ -1650,6 +1657,19 @@ __global__ __launch_bounds__(TPB) void k_mark_same(KP P, const int* __restrict__
     if (threadIdx.x == 0) same[b * 6 + side] = differ ? 0 : 1;
 }
 
+// Protect mask of the early con2prim pass (KP::pmask): for every leaf side
+// that copies from a same-level block without reusing the source's outcome,
+// mark the source's opposite side.
+__global__ void k_pmask(KP P, const int* __restrict__ leaves, int nleaves, int* __restrict__ pmask) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nleaves * 6) return;
+    const int b = leaves[t / 6], side = t % 6;
+    const int kind = P.ghtab[b * 12 + side * 2], q = P.ghtab[b * 12 + side * 2 + 1];
+    if (kind != GH_COPY || q < 0 || P.ghsame[b * 6 + side]) return;
+    const int dim = side / 2, s = side % 2;
+    atomicOr(pmask + q, 1 << (2 * dim + (1 - s)));
+}
+
 // ----------------------------------------------------------- state dump
 // W of the active zones of leaves l0 .. l0 + n - 1, packed [n][8][4096]
 // (k fastest) for the dump file.
@@ -1805,6 +1825,10 @@ public:
     DBuf<double> diagpart;          // per-workgroup diagnostic partials, then the 8 sums
     DBuf<int> ghtab, leaves, waveblk, lvl, pushto;
     DBuf<unsigned char> ghsame;
+    DBuf<int> pmask;                // [nb] sides read by copies without outcome reuse
+    // Reuse of a copied ghost's con2prim outcome where the metric bits agree
+    // (off: every copied ghost runs its own con2prim; a testing switch)
+    bool exact_copies = true;
     std::vector<int> wave_off;  // offsets into waveblk
     DBuf<short> cell_order;
     DBuf<unsigned long long> scal;  // [0] dt bits, [1] faults
@@ -1892,7 +1916,7 @@ public:
         P.U = U.p; P.W = W.p; P.Bf = Bf.p; P.Us = Us.p; P.F = F.p;
         P.gct = gct.p; P.gcc = gcc.p; P.gfl = gfl.p; P.gfc = gfc.p; P.gchr = gchr.p; P.dxyz = dxyz.p; P.org = org.p;
         P.gth = gth.p;
-        P.ghtab = ghtab.p; P.ghsame = ghsame.p; P.push_to = pushto.p;
+        P.ghtab = ghtab.p; P.ghsame = ghsame.p; P.pmask = pmask.p; P.push_to = pushto.p;
         P.cell_order = cell_order.p;
         P.dtbits = scal.p; P.faults = scal.p + 1;
         P.cst = cst.p;
@@ -1922,7 +1946,9 @@ public:
         leaves.alloc(std::max<size_t>(1, plan.leaves.size()));
         if (!plan.leaves.empty()) leaves.upload(plan.leaves.data(), plan.leaves.size(), stream);
         ghsame.alloc(plan.same.size());
-        if (g) {
+        if (!exact_copies) {
+            HIPCHK(hipMemsetAsync(ghsame.p, 0, ghsame.n, stream));
+        } else if (g) {
             mark_exact_copies(*g, plan);
             ghsame.upload(plan.same.data(), plan.same.size(), stream);
         } else {
@@ -1930,6 +1956,14 @@ public:
             if (!plan.leaves.empty())
                 hipLaunchKernelGGL(k_mark_same, dim3(plan.leaves.size() * 6), dim3(TPB), 0, stream, kp(),
                                    (const int*)leaves.p, ghsame.p);
+            HIPCHK(hipGetLastError());
+        }
+        pmask.alloc(std::max<size_t>(1, (size_t)nb));
+        HIPCHK(hipMemsetAsync(pmask.p, 0, pmask.n * sizeof(int), stream));
+        if (!plan.leaves.empty()) {
+            const int n = (int)plan.leaves.size() * 6;
+            hipLaunchKernelGGL(k_pmask, dim3((n + TPB - 1) / TPB), dim3(TPB), 0, stream, kp(), (const int*)leaves.p,
+                               (int)plan.leaves.size(), pmask.p);
             HIPCHK(hipGetLastError());
         }
         {
@@ -2636,14 +2670,18 @@ int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_i
         double tt = t_io ? *t_io : 0.0;
         int n = 0;
         long long nf = 0;
+        // the reference's time loop ends at the first equation-of-state
+        // fault (std::domain_error out of step); here the faulting step
+        // completes, is counted, and the loop stops after it
         while (tt < t_end && n < nsteps) {
             t->dev->step(cfl);
             t->dev->sync();
             tt += t->dev->read_dt();
             n++;
+            nf += t->dev->take_faults();
+            if (nf) break;
         }
         device_changed(t);
-        nf = t->dev->take_faults();
         if (t_io) *t_io = tt;
         if (steps_out) *steps_out = n;
         return finish_faults(t, nf);
@@ -2653,6 +2691,18 @@ int bh_profile_only(bh_tree* t, const char* name) {
     return guarded(t, [&] {
         if (!t->dev) t->dev = std::make_unique<DeviceTree>();
         t->dev->profile_only = name ? name : "";
+        return BH_OK;
+    });
+}
+int bh_tree_set_exact_copies(bh_tree* t, int enable) {
+    return guarded(t, [&] {
+        if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+        if ((enable != 0) != t->dev->exact_copies) {
+            if (t->dev_valid) need_host(t);
+            t->dev->exact_copies = enable != 0;
+            t->dev->graph_release();
+            t->layout_dirty = true;
+        }
         return BH_OK;
     });
 }

@@ -92,7 +92,10 @@ int bh_rk2_rkstg(bh_tree* t, double dt, int stage);
 /* patch::floors on every patch, as at integrator.cpp:269-273 */
 int bh_tree_floors(bh_tree* t);
 /* The time loop of src/fluid/main.cpp:169-173: step while t < t_end and
- * steps < nsteps; *t_io is advanced, *steps_out counts steps taken. */
+ * steps < nsteps; *t_io is advanced, *steps_out counts steps taken.  Where the
+ * reference's loop ends with std::domain_error (the first step whose equation
+ * of state sees rho <= 0), this loop completes that step, counts it in
+ * *t_io and *steps_out, stops, and returns BH_EDOMAIN. */
 int bh_rk2_advance(bh_tree* t, double cfl, int nsteps, double t_end, double* t_io, int* steps_out);
 /* nsteps consecutive steps enqueued back to back with a single host
  * synchronisation at the end (the step sequence of bh_rk2_advance without a
@@ -103,6 +106,11 @@ int bh_rk2_steps(bh_tree* t, double cfl, int nsteps, double* dt_out);
  * one by one (0).  Results are identical; no reference counterpart.  Timing with
  * bh_profile always launches one by one. */
 int bh_tree_use_graph(bh_tree* t, int enable);
+/* Testing switch without a reference counterpart: enable 0 makes every copied
+ * ghost cell run its own con2prim instead of reusing the source zone's outcome
+ * where the two cells' metric bits agree (the default, enable 1).  Results are
+ * identical either way; the switch exercises the non-reuse ghost path. */
+int bh_tree_set_exact_copies(bh_tree* t, int enable);
 /* Build the device copy (geometry, ghost plan, state upload) ahead of the
  * first step, so that set-up is not timed with it. */
 int bh_tree_prepare(bh_tree* t);
