@@ -1,0 +1,1574 @@
+// The spherical Kerr-Schild engine on gfx950: the GRMHD step of src/fluid
+// (rk2integrator::step, src/fluid/solver/integrator.cpp:262-275) for any
+// number of independent runs (tori of their own grid, spin and field) whose
+// blocks of every refinement level are advanced together in shared launches.
+//
+// One step = begin (per-run dt reset) and two RK stages; a stage is
+//   A  con2prim of every active zone with floors (integrator.cpp:201-222,
+//      grid.cpp:331-358), CFL reduction per run in the first stage
+//      (integrator.cpp:156-189)
+//   -- the only point where blocks must wait for each other --
+//   B  ghost fill of primitives and face fields from same-level, coarser or
+//      finer leaves and the domain boundary (grid.cpp:390-571), PLM + HLL
+//      fluxes on every face a block needs (grid.cpp:255-329), constrained
+//      transport with the EMFs formed in place (transport.cpp:11-58), flux
+//      divergence + geometric sources + RK2 and the cell field from the new
+//      faces (integrator.cpp:18-153, 238-259, transport.cpp:60-75).
+// Phase B reads other blocks only through primitives (written in A) and
+// face fields of the previous stage (double-buffered), so every block of
+// phase B is independent.  Two launch schedules give identical results:
+// one kernel per sub-phase over all blocks, or one persistent kernel per
+// stage whose workgroups pull blocks from a queue and cross one grid
+// barrier between A and B.
+//
+// State is structure-of-arrays per block slot; fluxes are FP64 or (mixed
+// precision) FP32 words.  Reconstruction, Riemann fluxes and EMFs run in the
+// flux type; con2prim, sources, the update and all reductions in FP64.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/bhsph.h"
+#include "bh_common.hpp"
+#include "bh_crmath.hpp"
+#include "bh_crtrig.hpp"
+#include "sph_common.hpp"
+#include "sph_geom.hpp"
+#include "sph_physics.hpp"
+#include "sph_torus.hpp"
+
+namespace sph {
+
+#define SCHK(x)                                                                                    \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct DevCR {
+    __device__ static double log(double x) { return bh::cr_log(x); }
+    __device__ static double exp(double x) { return bh::cr_exp(x); }
+    __device__ static double pow(double x, double y) { return bh::cr_pow(x, y); }
+};
+
+// one level of one run, with its device tables
+struct DRL {
+    int run, level, nr, nth, nph, nbr, nbth, nbph;
+    int coarser, finer;
+    int npt;
+    long long zone0;
+    double dth, dph;
+    const int* lut;
+    const double* gc;
+    const double* gf;
+    const double* dr;
+};
+
+struct KPS {
+    int bz, sz, ncell, nint, nf;
+    const Blk* blk;       // [slot]
+    const DRL* rl;        // [rl]
+    const RunPar* run;    // [run]
+    RunCtr* ctr;          // [run]
+    double* U;            // [slot][8][ncell]
+    double* W;            // [slot][8][ncell]
+    double* Us;           // [slot][5][nint]
+    double* Bf0;          // [slot][3][nf]
+    double* Bf1;
+    void* F;              // [slot][3][NFW][nf] flux words (double or float)
+    const int* list;      // block slots to process
+    int nlist;
+    const int* ghc;       // ghost cells of a block: packed (k+2)<<16 | (j+2)<<8 | (i+2)
+    int nghc;
+    const int* ghf;       // ghost faces: d<<24 | packed cell
+    int nghf;
+    const int* flf;       // flux faces
+    int nflf;
+    const int* owf;       // own faces
+    int nowf;
+    // persistent schedule
+    unsigned int* qctr;   // [0] stage-A queue, [1] barrier arrivals, [2] stage-B queue, [3] timeout flag
+};
+
+__device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+__device__ __forceinline__ int leaf_at(const KPS& P, const DRL& v, int gi, int gj, int gk) {
+    const int b1 = gi / BX, b2 = gj / BY, b3 = gk / P.bz;
+    return v.lut[((size_t)b3 * v.nbth + b2) * v.nbr + b1];
+}
+__device__ __forceinline__ void bc_cell(const DRL& v, int& gi, int& gj, int& gk, int& flip) {
+    if (gj < 0) { gj = -1 - gj; flip ^= 1; }
+    if (gj >= v.nth) { gj = 2 * v.nth - 1 - gj; flip ^= 1; }
+    if (gi < 0) gi = 0;
+    if (gi >= v.nr) gi = v.nr - 1;
+    gk = ((gk % v.nph) + v.nph) % v.nph;
+}
+__device__ __forceinline__ void fetch_w(const KPS& P, int s, int gi, int gj, int gk, int flip, double w[8]) {
+    const Blk B = P.blk[s];
+    const int c = cidx(gi - B.bi * BX, gj - B.bj * BY, gk - B.bk * P.bz);
+    const double* Ws = P.W + (size_t)s * 8 * P.ncell;
+#pragma unroll
+    for (int q = 0; q < 8; q++) w[q] = Ws[(size_t)q * P.ncell + c];
+    if (flip) { w[WU2] = -w[WU2]; w[WB2] = -w[WB2]; }
+}
+__device__ __forceinline__ bool leaf_w(const KPS& P, const DRL& v, int gi, int gj, int gk, double w[8]) {
+    int flip = 0;
+    bc_cell(v, gi, gj, gk, flip);
+    const int s = leaf_at(P, v, gi, gj, gk);
+    if (s < 0) return false;
+    fetch_w(P, s, gi, gj, gk, flip, w);
+    return true;
+}
+
+// ghost cell (i, j, k) of block b
+__device__ void ghost_cell(const KPS& P, int b, int i, int j, int k) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    int gi = B.bi * BX + i, gj = B.bj * BY + j, gk = B.bk * P.bz + k, flip = 0;
+    bc_cell(v, gi, gj, gk, flip);
+    double out[8];
+    const int s = leaf_at(P, v, gi, gj, gk);
+    if (s >= 0) {
+        fetch_w(P, s, gi, gj, gk, flip, out);
+    } else {
+        bool done = false;
+        if (v.coarser >= 0) {
+            const DRL& vc = P.rl[v.coarser];
+            const int c1 = gi >> 1, c2 = gj >> 1, c3 = gk >> 1;
+            double wc[8];
+            if (leaf_w(P, vc, c1, c2, c3, wc)) {
+                double acc[8];
+#pragma unroll
+                for (int q = 0; q < 8; q++) acc[q] = wc[q];
+                const int par[3] = {gi & 1, gj & 1, gk & 1};
+                for (int d = 0; d < 3; d++) {
+                    double wm[8], wp[8];
+                    const int o0 = d == 0, o1 = d == 1, o2 = d == 2;
+                    const bool hm = leaf_w(P, vc, c1 - o0, c2 - o1, c3 - o2, wm);
+                    const bool hp = leaf_w(P, vc, c1 + o0, c2 + o1, c3 + o2, wp);
+                    const double off = par[d] ? 0.25 : -0.25;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const double sl = (hm && hp) ? minmod(wc[q] - wm[q], wp[q] - wc[q]) : 0.0;
+                        acc[q] = acc[q] + sl * off;
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < 8; q++) out[q] = acc[q];
+                if (flip) { out[WU2] = -out[WU2]; out[WB2] = -out[WB2]; }
+                done = true;
+            }
+        }
+        if (!done) {
+            const DRL& vf = P.rl[v.finer];
+            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            for (int c = 0; c < 2; c++)
+                for (int bb = 0; bb < 2; bb++)
+                    for (int a = 0; a < 2; a++) {
+                        double wf[8];
+                        const int fi = 2 * gi + a, fj = 2 * gj + bb, fk = 2 * gk + c;
+                        fetch_w(P, leaf_at(P, vf, fi, fj, fk), fi, fj, fk, 0, wf);
+#pragma unroll
+                        for (int q = 0; q < 8; q++) acc[q] += wf[q];
+                    }
+#pragma unroll
+            for (int q = 0; q < 8; q++) out[q] = acc[q] * 0.125;
+            if (flip) { out[WU2] = -out[WU2]; out[WB2] = -out[WB2]; }
+        }
+    }
+    double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    const int c = cidx(i, j, k);
+#pragma unroll
+    for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + c] = out[q];
+}
+
+__device__ __forceinline__ void bc_face(const DRL& v, int d, int& gi, int& gj, int& gk, double& sgn) {
+    if (d == 1) {
+        if (gj < 0) { gj = -gj; sgn = -sgn; }
+        if (gj > v.nth) { gj = 2 * v.nth - gj; sgn = -sgn; }
+    } else {
+        if (gj < 0) gj = -1 - gj;
+        if (gj >= v.nth) gj = 2 * v.nth - 1 - gj;
+    }
+    if (d != 0) {
+        if (gi < 0) gi = 0;
+        if (gi >= v.nr) gi = v.nr - 1;
+    }
+    gk = ((gk % v.nph) + v.nph) % v.nph;
+}
+// the stored field of face (d; gi, gj, gk) at level v from a leaf that has
+// it as an own face: the leaf of the cell on the face's high side, else the
+// one on its low side; false if neither is a leaf
+__device__ __forceinline__ bool face_val(const KPS& P, const double* Bf, const DRL& v, int d, int gi, int gj, int gk,
+                                         double& out) {
+    const int N = d == 0 ? v.nr : d == 1 ? v.nth : v.nph;
+    const int gd = d == 0 ? gi : d == 1 ? gj : gk;
+    for (int side = 0; side < 2; side++) {
+        const int cd = gd - side;
+        if (cd < 0 || cd >= N) continue;
+        const int s = leaf_at(P, v, d == 0 ? cd : gi, d == 1 ? cd : gj, d == 2 ? cd : gk);
+        if (s < 0) continue;
+        const Blk B = P.blk[s];
+        out = Bf[((size_t)s * 3 + d) * P.nf + fidx(d, gi - B.bi * BX, gj - B.bj * BY, gk - B.bk * P.bz)];
+        return true;
+    }
+    return false;
+}
+
+__device__ void ghost_face(const KPS& P, double* Bf, int b, int d, int i, int j, int k) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    int gi = B.bi * BX + i, gj = B.bj * BY + j, gk = B.bk * P.bz + k;
+    double sgn = 1.0;
+    bc_face(v, d, gi, gj, gk, sgn);
+    double x, res;
+    if (face_val(P, Bf, v, d, gi, gj, gk, x)) {
+        res = sgn * x;
+    } else {
+        const int g[3] = {gi, gj, gk};
+        bool done = false;
+        if (v.coarser >= 0) {
+            const DRL& vc = P.rl[v.coarser];
+            int c[3] = {g[0] >> 1, g[1] >> 1, g[2] >> 1};
+            int cc[3] = {c[0], c[1], c[2]};
+            const int nd = d == 0 ? vc.nr : d == 1 ? vc.nth : vc.nph;
+            if (cc[d] == nd) cc[d] -= 1;
+            if (leaf_at(P, vc, cc[0], cc[1], cc[2]) >= 0) {
+                if ((g[d] & 1) == 0) {
+                    c[d] = g[d] >> 1;
+                    double y;
+                    face_val(P, Bf, vc, d, c[0], c[1], c[2], y);
+                    res = sgn * y;
+                } else {
+                    int lo[3] = {c[0], c[1], c[2]}, hi[3] = {c[0], c[1], c[2]};
+                    lo[d] = (g[d] - 1) >> 1;
+                    hi[d] = (g[d] + 1) >> 1;
+                    double y0, y1;
+                    face_val(P, Bf, vc, d, lo[0], lo[1], lo[2], y0);
+                    face_val(P, Bf, vc, d, hi[0], hi[1], hi[2], y1);
+                    res = sgn * (0.5 * (y0 + y1));
+                }
+                done = true;
+            }
+        }
+        if (!done) {
+            const DRL& vf = P.rl[v.finer];
+            const int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+            double acc = 0.0;
+            for (int bb = 0; bb < 2; bb++)
+                for (int a = 0; a < 2; a++) {
+                    int f[3] = {2 * g[0], 2 * g[1], 2 * g[2]};
+                    f[t1] += a;
+                    f[t2] += bb;
+                    double y;
+                    face_val(P, Bf, vf, d, f[0], f[1], f[2], y);
+                    acc += y;
+                }
+            res = sgn * (acc * 0.25);
+        }
+    }
+    Bf[((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k)] = res;
+}
+
+__device__ __forceinline__ void unpack(int x, int& i, int& j, int& k) {
+    i = (x & 0xff) - 2;
+    j = ((x >> 8) & 0xff) - 2;
+    k = ((x >> 16) & 0xff) - 2;
+}
+
+// ---------------------------------------------------------------- phase A
+// con2prim + floors of one active zone; returns the zone's dt candidate
+// (before the CFL factor) when want_dt, else 1e300.
+__device__ double c2p_zone(const KPS& P, int b, int a, bool want_dt, int& nfail, int& nfl) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+    const int c = cidx(i, j, k);
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const Met<double> m = load_met<double>(v.gc, v.npt, p);
+    double* Ub = P.U + (size_t)b * 8 * P.ncell;
+    double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    double u[8], w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) u[q] = Ub[(size_t)q * P.ncell + c];
+    const double rfl = v.gc[(size_t)C_RFL * v.npt + p], efl = v.gc[(size_t)C_EFL * v.npt + p];
+    // con2prim; a failed inversion (cold inversion or, for an unrecoverable
+    // state, the floor state) always rewrites U, floors where they change W
+    const int rc = c2p(m, R.gam, u, w);
+    if (rc == 2) floor_state(rfl, efl, w);
+    nfail += rc != 0;
+    const int fl = apply_floors(m, R.gam, rfl, efl, R.sigma_max, R.w_max, w);
+    nfl += fl;
+    const bool redo = rc != 0 || fl != 0;
+    if (redo) {
+        double un[8];
+        ptoc(m, R.gam, w, un);
+#pragma unroll
+        for (int q = 0; q < 5; q++) Ub[(size_t)q * P.ncell + c] = un[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + c] = w[q];
+    if (!want_dt) return 1e300;
+    const double dx[3] = {v.dr[gi + 2], v.dth, v.dph};
+    return cell_dt(m, R.gam, w, dx, P.bz == 1 ? 2 : 3);
+}
+
+// workgroup-wide min of a positive double and sums of two counters, then one
+// atomic per workgroup into the run's counters
+template <int TPB>
+__device__ void wg_commit(const KPS& P, int run, double dtc, int nfail, int nfl, bool want_dt) {
+    __shared__ unsigned long long s_dt[TPB / 64];
+    __shared__ int s_f[TPB / 64], s_l[TPB / 64];
+    unsigned long long bits = (unsigned long long)__double_as_longlong(dtc);
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long ob = __shfl_xor(bits, o);
+        bits = ob < bits ? ob : bits;
+        nfail += __shfl_xor(nfail, o);
+        nfl += __shfl_xor(nfl, o);
+    }
+    const int wv = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0) { s_dt[wv] = bits; s_f[wv] = nfail; s_l[wv] = nfl; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long mb = s_dt[0];
+        int f = s_f[0], l = s_l[0];
+        for (int q = 1; q < TPB / 64; q++) {
+            mb = s_dt[q] < mb ? s_dt[q] : mb;
+            f += s_f[q];
+            l += s_l[q];
+        }
+        if (want_dt) {
+            // the CFL factor is applied to the minimum (exact: multiplication
+            // by a positive constant preserves order)
+            const double dt = P.run[run].cfl * __longlong_as_double((long long)mb);
+            atomicMin(&P.ctr[run].dtbits, (unsigned long long)__double_as_longlong(dt));
+        }
+        if (f) atomicAdd(&P.ctr[run].c2p_fail, (unsigned long long)f);
+        if (l) atomicAdd(&P.ctr[run].floors, (unsigned long long)l);
+    }
+    __syncthreads();
+}
+
+template <int TPB>
+__device__ void phase_a_block(const KPS& P, int b, bool want_dt) {
+    double mn = 1e300;
+    int nfail = 0, nfl = 0;
+    for (int a = threadIdx.x; a < P.nint; a += TPB) mn = fmin(mn, c2p_zone(P, b, a, want_dt, nfail, nfl));
+    wg_commit<TPB>(P, P.rl[P.blk[b].rl].run, mn, nfail, nfl, want_dt);
+}
+
+// ---------------------------------------------------------------- phase B
+template <class Rt>
+__device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, int j, int k) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const double* tab = d == 0 ? v.gf : d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc;
+    const Met<Rt> m = load_met<Rt>(tab, v.npt, p);
+    const int o0 = d == 0, o1 = d == 1, o2 = d == 2;
+    const int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
+    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    Rt s[4][8];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+        const int cc = cidx(i + (c - 2) * o0, j + (c - 2) * o1, k + (c - 2) * o2);
+#pragma unroll
+        for (int q = 0; q < 8; q++) s[c][q] = (Rt)Wb[(size_t)q * P.ncell + cc];
+    }
+    Pt<Rt> PL, PR;
+    Rt pc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) pc[c] = (Rt)(R.gam - 1.0) * s[c][WRHO] * s[c][WEPS];
+    plm(s[0][WRHO], s[1][WRHO], s[2][WRHO], s[3][WRHO], PL.rho, PR.rho);
+    plm(pc[0], pc[1], pc[2], pc[3], PL.p, PR.p);
+#pragma unroll
+    for (int e = 0; e < 3; e++) plm(s[0][WU1 + e], s[1][WU1 + e], s[2][WU1 + e], s[3][WU1 + e], PL.u[e], PR.u[e]);
+    plm(s[0][WB1 + t1], s[1][WB1 + t1], s[2][WB1 + t1], s[3][WB1 + t1], PL.B[t1], PR.B[t1]);
+    plm(s[0][WB1 + t2], s[1][WB1 + t2], s[2][WB1 + t2], s[3][WB1 + t2], PL.B[t2], PR.B[t2]);
+    const Rt bn = m.sg > Rt(0) ? (Rt)Bf[((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k)] / m.sg : Rt(0);
+    PL.B[d] = bn;
+    PR.B[d] = bn;
+    const Rt gam = (Rt)R.gam;
+    pt_fill(m, gam, PL);
+    pt_fill(m, gam, PR);
+    Rt out[NFW];
+    hll(m, gam, PL, PR, d, out);
+    Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)d * NFW * P.nf + fidx(d, i, j, k);
+#pragma unroll
+    for (int q = 0; q < NFW; q++) F[(size_t)q * P.nf] = out[q];
+}
+
+template <class Rt>
+__device__ __forceinline__ Rt Fat(const KPS& P, const Rt* Fb, int d, int q, int i, int j, int k) {
+    return Fb[((size_t)d * NFW + q) * P.nf + fidx(d, i, j, k)];
+}
+template <class Rt>
+__device__ __forceinline__ Rt emf_z(const KPS& P, const Rt* Fb, int gj0, int nth, int i, int j, int k) {
+    const int gj = gj0 + j;
+    if (gj == 0 || gj == nth) return Rt(0);
+    return Rt(0.25) * (Fat(P, Fb, 1, 6, i, j, k) + Fat(P, Fb, 1, 6, i - 1, j, k) - Fat(P, Fb, 0, 5, i, j, k) -
+                       Fat(P, Fb, 0, 5, i, j - 1, k));
+}
+template <class Rt>
+__device__ __forceinline__ Rt emf_x(const KPS& P, const Rt* Fb, int gj0, int nth, int i, int j, int k) {
+    const int gj = gj0 + j;
+    if (gj == 0 || gj == nth) return Rt(0);
+    return Rt(0.25) * (Fat(P, Fb, 2, 6, i, j, k) + Fat(P, Fb, 2, 6, i, j - 1, k) - Fat(P, Fb, 1, 5, i, j, k) -
+                       Fat(P, Fb, 1, 5, i, j, k - 1));
+}
+template <class Rt>
+__device__ __forceinline__ Rt emf_y(const KPS& P, const Rt* Fb, int i, int j, int k) {
+    return Rt(0.25) * (Fat(P, Fb, 0, 6, i, j, k) + Fat(P, Fb, 0, 6, i, j, k - 1) - Fat(P, Fb, 2, 5, i, j, k) -
+                       Fat(P, Fb, 2, 5, i - 1, j, k));
+}
+
+// own face (d; i, j, k) of block b: face field of the stage
+template <class Rt>
+__device__ void face_update(const KPS& P, int b, int d, int i, int j, int k, double dt, int stage) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    const int gj0 = B.bj * BY;
+    const int gi = B.bi * BX + i;
+    double rhs;
+    if (d == 0) {
+        const double a1 = (double)(emf_z<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a2 = (double)(emf_y<Rt>(P, Fb, i, j, k + 1) - emf_y<Rt>(P, Fb, i, j, k));
+        rhs = -a1 / v.dth + a2 / v.dph;
+    } else if (d == 1) {
+        const double dri = v.dr[gi + 2];
+        const double a1 = (double)(emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k + 1) - emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a2 = (double)(emf_z<Rt>(P, Fb, gj0, v.nth, i + 1, j, k) - emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        rhs = -a1 / v.dph + a2 / dri;
+    } else {
+        const double dri = v.dr[gi + 2];
+        const double a1 = (double)(emf_y<Rt>(P, Fb, i + 1, j, k) - emf_y<Rt>(P, Fb, i, j, k));
+        const double a2 = (double)(emf_x<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        rhs = -a1 / dri + a2 / v.dth;
+    }
+    const size_t x = ((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k);
+    if (stage == 0) P.Bf1[x] = P.Bf0[x] + dt * rhs;
+    else P.Bf0[x] = 0.5 * (P.Bf0[x] + (P.Bf1[x] + dt * rhs));
+}
+
+template <class Rt>
+__device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+    const int c = cidx(i, j, k);
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const Met<double> m = load_met<double>(v.gc, v.npt, p);
+    double* Ub = P.U + (size_t)b * 8 * P.ncell;
+    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    double* Usb = P.Us + (size_t)b * 5 * P.nint;
+    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    double w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) w[q] = Wb[(size_t)q * P.ncell + c];
+    double src[5];
+    source(v.gc, v.npt, p, m, R.gam, w, src);
+    const double dri = v.dr[gi + 2];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const double dx = ((double)Fat(P, Fb, 0, q, i + 1, j, k) - (double)Fat(P, Fb, 0, q, i, j, k)) / dri;
+        const double dy = ((double)Fat(P, Fb, 1, q, i, j + 1, k) - (double)Fat(P, Fb, 1, q, i, j, k)) / v.dth;
+        const double dz = ((double)Fat(P, Fb, 2, q, i, j, k + 1) - (double)Fat(P, Fb, 2, q, i, j, k)) / v.dph;
+        const double dU = src[q] - (dx + dy + dz);
+        double* u = Ub + (size_t)q * P.ncell + c;
+        if (stage == 0) {
+            Usb[(size_t)q * P.nint + a] = *u;
+            *u = *u + dt * dU;
+        } else {
+            *u = 0.5 * (Usb[(size_t)q * P.nint + a] + (*u + dt * dU));
+        }
+    }
+    const double* Bn = (stage == 0 ? P.Bf1 : P.Bf0) + (size_t)b * 3 * P.nf;
+    Ub[(size_t)UB1 * P.ncell + c] = 0.5 * (Bn[fidx(0, i, j, k)] + Bn[fidx(0, i + 1, j, k)]);
+    Ub[(size_t)UB2 * P.ncell + c] = 0.5 * (Bn[P.nf + fidx(1, i, j, k)] + Bn[P.nf + fidx(1, i, j + 1, k)]);
+    Ub[(size_t)UB3 * P.ncell + c] = 0.5 * (Bn[2 * P.nf + fidx(2, i, j, k)] + Bn[2 * P.nf + fidx(2, i, j, k + 1)]);
+}
+
+__device__ __forceinline__ double run_dt(const KPS& P, int b) {
+    return dtbits_to(P.ctr[P.rl[P.blk[b].rl].run].dtbits);
+}
+
+// ---------------------------------------------------- multi-kernel schedule
+constexpr int TPB = 256;
+
+__global__ void k_begin(RunCtr* ctr, const int* runs, int nruns, unsigned int* qctr) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nruns) ctr[runs[t]].dtbits = 0x7ff0000000000000ull;  // +inf
+    if (t < 4) qctr[t] = 0;
+}
+
+__global__ __launch_bounds__(TPB) void k_c2p(KPS P, int want_dt) {
+    phase_a_block<TPB>(P, P.list[blockIdx.x], want_dt != 0);
+}
+
+__global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
+    const int per = (P.nghc + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (t >= P.nghc) return;
+    int i, j, k;
+    unpack(P.ghc[t], i, j, k);
+    ghost_cell(P, b, i, j, k);
+}
+
+__global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
+    const int per = (P.nghf + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (t >= P.nghf) return;
+    int i, j, k;
+    const int x = P.ghf[t];
+    unpack(x, i, j, k);
+    ghost_face(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
+}
+
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_flux(KPS P, int buf) {
+    const int per = (P.nflf + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (t >= P.nflf) return;
+    int i, j, k;
+    const int x = P.flf[t];
+    unpack(x, i, j, k);
+    flux_face<Rt>(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
+}
+
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_face(KPS P, int stage) {
+    const int per = (P.nowf + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (t >= P.nowf) return;
+    int i, j, k;
+    const int x = P.owf[t];
+    unpack(x, i, j, k);
+    face_update<Rt>(P, b, x >> 24, i, j, k, run_dt(P, b), stage);
+}
+
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_update(KPS P, int stage) {
+    const int per = (P.nint + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int a = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (a >= P.nint) return;
+    cell_update<Rt>(P, b, a, run_dt(P, b), stage);
+}
+
+// ---------------------------------------------------- persistent schedule
+// One launch per stage.  Workgroups pull blocks for phase A from a queue,
+// meet at one grid barrier (every workgroup of the launch is resident: the
+// grid is sized from the occupancy of this kernel), then pull blocks again
+// for phase B and run its four sub-phases with workgroup barriers between
+// them.  The barrier wait is bounded; a launch that is not co-resident sets
+// the timeout flag and every workgroup still exits.
+constexpr unsigned SPIN_LIMIT = 1u << 26;
+
+__device__ __forceinline__ void grid_barrier(unsigned int* arrive, unsigned int* flag, unsigned int n) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        atomicAdd(arrive, 1u);
+        unsigned spins = 0;
+        while (__hip_atomic_load(arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < n) {
+            __builtin_amdgcn_s_sleep(2);
+            if (++spins > SPIN_LIMIT) {
+                atomicExch(flag, 1u);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_stage(KPS P, int stage) {
+    __shared__ int s_b;
+    const bool want_dt = stage == 0;
+    for (;;) {
+        if (threadIdx.x == 0) s_b = (int)atomicAdd(&P.qctr[0], 1u);
+        __syncthreads();
+        const int q = s_b;
+        __syncthreads();
+        if (q >= P.nlist) break;
+        phase_a_block<TPB>(P, P.list[q], want_dt);
+    }
+    grid_barrier(&P.qctr[1], &P.qctr[3], gridDim.x);
+    if (P.qctr[3]) return;
+    double* Bc = stage == 0 ? P.Bf0 : P.Bf1;
+    for (;;) {
+        if (threadIdx.x == 0) s_b = (int)atomicAdd(&P.qctr[2], 1u);
+        __syncthreads();
+        const int q = s_b;
+        __syncthreads();
+        if (q >= P.nlist) break;
+        const int b = P.list[q];
+        const double dt = run_dt(P, b);
+        for (int t = threadIdx.x; t < P.nghc; t += TPB) {
+            int i, j, k;
+            unpack(P.ghc[t], i, j, k);
+            ghost_cell(P, b, i, j, k);
+        }
+        for (int t = threadIdx.x; t < P.nghf; t += TPB) {
+            int i, j, k;
+            const int x = P.ghf[t];
+            unpack(x, i, j, k);
+            ghost_face(P, Bc, b, x >> 24, i, j, k);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < P.nflf; t += TPB) {
+            int i, j, k;
+            const int x = P.flf[t];
+            unpack(x, i, j, k);
+            flux_face<Rt>(P, Bc, b, x >> 24, i, j, k);
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < P.nowf; t += TPB) {
+            int i, j, k;
+            const int x = P.owf[t];
+            unpack(x, i, j, k);
+            face_update<Rt>(P, b, x >> 24, i, j, k, dt, stage);
+        }
+        __syncthreads();
+        for (int a = threadIdx.x; a < P.nint; a += TPB) cell_update<Rt>(P, b, a, dt, stage);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------- initial data
+struct TorusDev {
+    TorusPar T;
+    double beta, acut, pamp;
+    U128 st, inc;
+};
+
+// torus density of every stored cell (for the corner averages) into rt
+// (0 outside the torus), the primitive state of the active zones with the
+// pressure in the eps slot, and the run's rho max (bit pattern) in mx[0]
+__global__ __launch_bounds__(TPB) void k_ic_torus(KPS P, TorusDev D, double* rt, unsigned long long* mx) {
+    const int per = (P.ncell + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int c = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (c >= P.ncell) return;
+    const int i = c % SX - G, j = (c / SX) % SY - G, k = c / (SX * SY) - G;
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const long long np = v.npt;
+    const double rr = v.gc[C_R * np + p], s = v.gc[C_SIN * np + p], co = v.gc[C_COS * np + p];
+    const double al = v.gc[M_ALPHA * np + p], b1 = v.gc[M_B1 * np + p];
+    const double g11 = v.gc[M_G11 * np + p], g13 = v.gc[M_G13 * np + p], g33 = v.gc[M_G33 * np + p];
+    const double beta_l1 = g11 * b1, beta_l3 = g13 * b1;
+    const double gtt = -al * al + beta_l1 * b1, gtp = beta_l3, gpp = g33;
+    double rho, pp, ur, uph;
+    const bool in = fm_cell<DevCR>(D.T, rr, s, co, gtt, gtp, gpp, b1, rho, pp, ur, uph) != 0;
+    rt[(size_t)b * P.ncell + c] = in ? rho : 0.0;
+    const bool act = i >= 0 && i < BX && j >= 0 && j < BY && k >= 0 && k < P.bz;
+    if (in && act) {
+        double* Wb = P.W + (size_t)b * 8 * P.ncell;
+        Wb[(size_t)WRHO * P.ncell + c] = rho;
+        Wb[(size_t)WEPS * P.ncell + c] = pp;
+        Wb[(size_t)WU1 * P.ncell + c] = ur;
+        Wb[(size_t)WU2 * P.ncell + c] = 0.0;
+        Wb[(size_t)WU3 * P.ncell + c] = uph;
+        atomicMax(mx, (unsigned long long)__double_as_longlong(rho));
+    }
+}
+
+__device__ __forceinline__ double ic_corner(const KPS& P, const double* rt, int b, double irm, double acut, int i,
+                                            int j, int k) {
+    const double* r = rt + (size_t)b * P.ncell;
+    const double q = 0.25 * (r[cidx(i - 1, j - 1, k)] * irm + r[cidx(i, j - 1, k)] * irm +
+                             r[cidx(i - 1, j, k)] * irm + r[cidx(i, j, k)] * irm);
+    return fmax(q - acut, 0.0);
+}
+
+// face fields of the own r and theta faces from the corner potential
+__global__ __launch_bounds__(TPB) void k_ic_faces(KPS P, TorusDev D, const double* rt,
+                                                  const unsigned long long* mx) {
+    const int per = (P.nowf + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (t >= P.nowf) return;
+    int i, j, k;
+    const int x = P.owf[t];
+    unpack(x, i, j, k);
+    const int d = x >> 24;
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const double irm = 1.0 / __longlong_as_double((long long)mx[0]);
+    double val = 0.0;
+    if (d == 0) {
+        val = (ic_corner(P, rt, b, irm, D.acut, i, j + 1, k) - ic_corner(P, rt, b, irm, D.acut, i, j, k)) / v.dth;
+    } else if (d == 1) {
+        const int gi = B.bi * BX + i;
+        val = -(ic_corner(P, rt, b, irm, D.acut, i + 1, j, k) - ic_corner(P, rt, b, irm, D.acut, i, j, k)) /
+              v.dr[gi + 2];
+    }
+    P.Bf0[((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k)] = val;
+}
+
+__device__ __forceinline__ void ic_cell_b(const KPS& P, int b, int i, int j, int k, double sg, double* B) {
+    const double* F0 = P.Bf0 + (size_t)b * 3 * P.nf;
+    B[0] = 0.5 * (F0[fidx(0, i, j, k)] + F0[fidx(0, i + 1, j, k)]) / sg;
+    B[1] = 0.5 * (F0[P.nf + fidx(1, i, j, k)] + F0[P.nf + fidx(1, i, j + 1, k)]) / sg;
+    B[2] = 0.5 * (F0[2 * P.nf + fidx(2, i, j, k)] + F0[2 * P.nf + fidx(2, i, j, k + 1)]) / sg;
+}
+
+// normalised torus / atmosphere state of the active zones, its cell field,
+// and the maxima of p_gas (mx[1]) and p_mag (mx[2])
+__global__ __launch_bounds__(TPB) void k_ic_cells(KPS P, TorusDev D, const double* rt, unsigned long long* mx) {
+    const int per = (P.nint + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int a = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (a >= P.nint) return;
+    const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const int cc = cidx(i, j, k);
+    const long long np = v.npt;
+    const Met<double> m = load_met<double>(v.gc, np, p);
+    const double irm = 1.0 / __longlong_as_double((long long)mx[0]);
+    double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    double w[8];
+    if (rt[(size_t)b * P.ncell + cc] > 0.0) {
+        w[WRHO] = Wb[(size_t)WRHO * P.ncell + cc] * irm;
+        w[WEPS] = Wb[(size_t)WEPS * P.ncell + cc] * irm;
+        w[WU1] = Wb[(size_t)WU1 * P.ncell + cc];
+        w[WU2] = 0.0;
+        w[WU3] = Wb[(size_t)WU3 * P.ncell + cc];
+    } else {
+        w[WRHO] = v.gc[C_RFL * np + p];
+        w[WEPS] = (R.gam - 1.0) * w[WRHO] * v.gc[C_EFL * np + p];
+        w[WU1] = w[WU2] = w[WU3] = 0.0;
+    }
+    ic_cell_b(P, b, i, j, k, m.sg, w + WB1);
+#pragma unroll
+    for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + cc] = w[q];
+    double Bl[3], ul[3];
+    lower3(m.g, w + WB1, Bl);
+    lower3(m.g, w + WU1, ul);
+    const double W2 = 1.0 + dot3(ul, w + WU1);
+    const double Bu = dot3(Bl, w + WU1);
+    const double b2 = (dot3(Bl, w + WB1) + Bu * Bu) / W2;
+    atomicMax(mx + 1, (unsigned long long)__double_as_longlong(w[WEPS]));
+    atomicMax(mx + 2, (unsigned long long)__double_as_longlong(0.5 * b2));
+}
+
+__global__ __launch_bounds__(TPB) void k_ic_scale(KPS P, const unsigned long long* mx, double beta, double* scale_out) {
+    const int per = (3 * P.nf + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    const double pg = __longlong_as_double((long long)mx[1]), pm = __longlong_as_double((long long)mx[2]);
+    const double scale = pm > 0.0 ? sqrt(pg / (beta * pm)) : 0.0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) *scale_out = scale;
+    if (t >= 3 * P.nf) return;
+    P.Bf0[(size_t)b * 3 * P.nf + t] *= scale;
+}
+
+__global__ __launch_bounds__(TPB) void k_ic_final(KPS P, TorusDev D, const double* rt) {
+    const int per = (P.nint + TPB - 1) / TPB;
+    const int b = P.list[blockIdx.x / per];
+    const int a = (blockIdx.x % per) * TPB + threadIdx.x;
+    if (a >= P.nint) return;
+    const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int gi = B.bi * BX + i, gj = B.bj * BY + j, gk = B.bk * P.bz + k;
+    const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+    const int cc = cidx(i, j, k);
+    const long long np = v.npt;
+    const Met<double> m = load_met<double>(v.gc, np, p);
+    double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    double* Ub = P.U + (size_t)b * 8 * P.ncell;
+    double w[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) w[q] = Wb[(size_t)q * P.ncell + cc];
+    ic_cell_b(P, b, i, j, k, m.sg, w + WB1);
+    double pp = w[WEPS];
+    if (rt[(size_t)b * P.ncell + cc] > 0.0) {
+        const unsigned long long n = (unsigned long long)(v.zone0 + ((long long)gk * v.nth + gj) * v.nr + gi);
+        const double u = pcg64_uniform(D.st, D.inc, n);
+        pp = pp * (1.0 + D.pamp * (2.0 * u - 1.0));
+    }
+    w[WEPS] = pp / ((R.gam - 1.0) * w[WRHO]);
+    const double rfl = v.gc[C_RFL * np + p], efl = v.gc[C_EFL * np + p];
+    if (w[WRHO] < rfl) w[WRHO] = rfl;
+    if (w[WEPS] < efl) w[WEPS] = efl;
+#pragma unroll
+    for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + cc] = w[q];
+    double u8[8];
+    ptoc(m, R.gam, w, u8);
+#pragma unroll
+    for (int q = 0; q < 5; q++) Ub[(size_t)q * P.ncell + cc] = u8[q];
+    const double* F0 = P.Bf0 + (size_t)b * 3 * P.nf;
+    Ub[(size_t)UB1 * P.ncell + cc] = 0.5 * (F0[fidx(0, i, j, k)] + F0[fidx(0, i + 1, j, k)]);
+    Ub[(size_t)UB2 * P.ncell + cc] = 0.5 * (F0[P.nf + fidx(1, i, j, k)] + F0[P.nf + fidx(1, i, j + 1, k)]);
+    Ub[(size_t)UB3 * P.ncell + cc] = 0.5 * (F0[2 * P.nf + fidx(2, i, j, k)] + F0[2 * P.nf + fidx(2, i, j, k + 1)]);
+}
+
+// ---------------------------------------------------- diagnostics
+// per block: mass, Killing energy (fixed-order tree sums over the block's
+// zones), max |div B| and the divergence stencil's scale
+__global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
+    __shared__ double sm[4][TPB];
+    const int b = P.list[blockIdx.x];
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const double* Ub = P.U + (size_t)b * 8 * P.ncell;
+    const double* F0 = P.Bf0 + (size_t)b * 3 * P.nf;
+    double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0;
+    for (int a = threadIdx.x; a < P.nint; a += TPB) {
+        const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+        const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+        const int c = cidx(i, j, k);
+        const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+        const long long np = v.npt;
+        const double dri = v.dr[gi + 2];
+        const double vol = dri * v.dth * v.dph;
+        ms += Ub[(size_t)UD * P.ncell + c] * vol;
+        const double al = v.gc[M_ALPHA * np + p];
+        const double bS = v.gc[M_B1 * np + p] * Ub[(size_t)US1 * P.ncell + c] +
+                          v.gc[M_B2 * np + p] * Ub[(size_t)US2 * P.ncell + c] +
+                          v.gc[M_B3 * np + p] * Ub[(size_t)US3 * P.ncell + c];
+        en += (al * (Ub[(size_t)UTAU * P.ncell + c] + Ub[(size_t)UD * P.ncell + c]) - bS) * vol;
+        const double x0 = F0[fidx(0, i + 1, j, k)], x1 = F0[fidx(0, i, j, k)];
+        const double y0 = F0[P.nf + fidx(1, i, j + 1, k)], y1 = F0[P.nf + fidx(1, i, j, k)];
+        const double z0 = F0[2 * P.nf + fidx(2, i, j, k + 1)], z1 = F0[2 * P.nf + fidx(2, i, j, k)];
+        const double t0 = (x0 - x1) / dri, t1 = (y0 - y1) / v.dth, t2 = (z0 - z1) / v.dph;
+        dm = fmax(dm, fabs(t0 + t1 + t2));
+        sc = fmax(sc, (fabs(x0) + fabs(x1)) / dri + (fabs(y0) + fabs(y1)) / v.dth + (fabs(z0) + fabs(z1)) / v.dph);
+    }
+    sm[0][threadIdx.x] = ms; sm[1][threadIdx.x] = en; sm[2][threadIdx.x] = dm; sm[3][threadIdx.x] = sc;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            sm[0][threadIdx.x] += sm[0][threadIdx.x + o];
+            sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
+            sm[2][threadIdx.x] = fmax(sm[2][threadIdx.x], sm[2][threadIdx.x + o]);
+            sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + o]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[(size_t)blockIdx.x * 4 + threadIdx.x] = sm[threadIdx.x][0];
+}
+
+// ---------------------------------------------------- host side
+template <class T>
+struct DB {
+    T* p = nullptr;
+    size_t n = 0;
+    DB() = default;
+    DB(const DB&) = delete;
+    DB& operator=(const DB&) = delete;
+    ~DB() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t m) {
+        if (m <= n && p) return;
+        release();
+        SCHK(hipMalloc(&p, std::max<size_t>(1, m) * sizeof(T)));
+        n = m;
+    }
+    void upload(const T* h, size_t m, hipStream_t s) {
+        alloc(m);
+        if (m) SCHK(hipMemcpyAsync(p, h, m * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+};
+
+struct HostRun {
+    bool live = false;
+    RunPar par{};
+    bhs_grid grid{};
+    std::vector<int> rls;      // RL indices of its levels
+    std::vector<int> slots;    // its block slots, in the caller's block order
+    std::vector<std::unique_ptr<DB<double>>> tabs;  // per level: gc, gf, dr
+    std::vector<std::unique_ptr<DB<int>>> luts;
+};
+
+inline int pack_cell(int i, int j, int k) { return ((k + 2) << 16) | ((j + 2) << 8) | (i + 2); }
+
+class Engine {
+public:
+    Lay L;
+    long long cap = 0;
+    hipStream_t stream = nullptr;
+    DB<double> U, W, Us, Bf0, Bf1, F, scratch;
+    DB<Blk> blk;
+    DB<DRL> rl;
+    DB<RunPar> runp;
+    DB<RunCtr> ctr;
+    DB<int> ghc, ghf, flf, owf;
+    int nghc = 0, nghf = 0, nflf = 0, nowf = 0;
+    DB<int> list;                 // active block slots (sorted by run)
+    DB<int> runlist;              // active runs
+    DB<unsigned int> qctr;
+    DB<double> diagbuf;
+    DB<unsigned long long> icmax;
+    DB<double> icscale;
+    std::vector<Blk> hblk;
+    std::vector<DRL> hrl;
+    std::vector<HostRun> runs;
+    std::vector<int> free_slots;
+    std::vector<int> act_runs;    // runs of the current active list
+    std::vector<int> act_slots;
+    bool mixed = false;
+    int mode = 0;                 // 0 multi-kernel, 1 persistent
+    int wg_per_cu = 0;            // persistent grid: 0 = occupancy
+    bool tables_dirty = true;
+    std::string err;
+    // graph of the step for the current active set
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool use_graph = true;
+    int graph_key = -1;
+
+    explicit Engine(int bz, long long capacity) {
+        L = make_lay(bz);
+        cap = capacity;
+        SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        U.alloc((size_t)cap * 8 * L.ncell);
+        W.alloc((size_t)cap * 8 * L.ncell);
+        Us.alloc((size_t)cap * 5 * L.nint);
+        Bf0.alloc((size_t)cap * 3 * L.nf);
+        Bf1.alloc((size_t)cap * 3 * L.nf);
+        F.alloc((size_t)cap * 3 * NFW * L.nf);  // FP64 words; the FP32 path uses the first half of each block's
+        SCHK(hipMemsetAsync(Bf0.p, 0, Bf0.n * sizeof(double), stream));
+        SCHK(hipMemsetAsync(Bf1.p, 0, Bf1.n * sizeof(double), stream));
+        SCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
+        SCHK(hipMemsetAsync(W.p, 0, W.n * sizeof(double), stream));
+        SCHK(hipMemsetAsync(U.p, 0, U.n * sizeof(double), stream));
+        blk.alloc(cap);
+        hblk.assign(cap, Blk{-1, 0, 0, 0});
+        for (long long s = cap - 1; s >= 0; s--) free_slots.push_back((int)s);
+        qctr.alloc(4);
+        SCHK(hipMemsetAsync(qctr.p, 0, 4 * sizeof(unsigned), stream));
+        icmax.alloc(4);
+        icscale.alloc(1);
+        build_item_lists();
+    }
+    ~Engine() {
+        graph_release();
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void graph_release() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+        graph_key = -1;
+    }
+
+    void build_item_lists() {
+        std::vector<int> gc_, gf_, ff_, ow_;
+        const int n[3] = {BX, BY, L.bz};
+        for (int k = -G; k < L.bz + G; k++)
+            for (int j = -G; j < BY + G; j++)
+                for (int i = -G; i < BX + G; i++)
+                    if (!(i >= 0 && i < BX && j >= 0 && j < BY && k >= 0 && k < L.bz)) gc_.push_back(pack_cell(i, j, k));
+        for (int d = 0; d < 3; d++) {
+            int lo[3], hi[3];
+            for (int e = 0; e < 3; e++) { lo[e] = e == d ? 0 : -1; hi[e] = n[e]; }
+            for (int k = lo[2]; k <= hi[2]; k++)
+                for (int j = lo[1]; j <= hi[1]; j++)
+                    for (int i = lo[0]; i <= hi[0]; i++) {
+                        const int x = (d << 24) | pack_cell(i, j, k);
+                        ff_.push_back(x);
+                        const int c[3] = {i, j, k};
+                        bool own = true;
+                        for (int e = 0; e < 3; e++) {
+                            if (e == d) own = own && c[e] >= 0 && c[e] <= n[e];
+                            else own = own && c[e] >= 0 && c[e] < n[e];
+                        }
+                        if (own) ow_.push_back(x);
+                        else gf_.push_back(x);
+                    }
+        }
+        nghc = (int)gc_.size(); nghf = (int)gf_.size(); nflf = (int)ff_.size(); nowf = (int)ow_.size();
+        ghc.upload(gc_.data(), gc_.size(), stream);
+        ghf.upload(gf_.data(), gf_.size(), stream);
+        flf.upload(ff_.data(), ff_.size(), stream);
+        owf.upload(ow_.data(), ow_.size(), stream);
+        SCHK(hipStreamSynchronize(stream));
+    }
+
+    KPS kp(const int* lst, int nlst) {
+        KPS P;
+        P.bz = L.bz; P.sz = L.sz; P.ncell = L.ncell; P.nint = L.nint; P.nf = L.nf;
+        P.blk = blk.p; P.rl = rl.p; P.run = runp.p; P.ctr = ctr.p;
+        P.U = U.p; P.W = W.p; P.Us = Us.p; P.Bf0 = Bf0.p; P.Bf1 = Bf1.p; P.F = F.p;
+        P.list = lst; P.nlist = nlst;
+        P.ghc = ghc.p; P.nghc = nghc; P.ghf = ghf.p; P.nghf = nghf;
+        P.flf = flf.p; P.nflf = nflf; P.owf = owf.p; P.nowf = nowf;
+        P.qctr = qctr.p;
+        return P;
+    }
+
+    // ---- runs
+    int add_run(const bhs_physics& ph, const bhs_grid& g, int nb, const int* blocks) {
+        if (nb <= 0) throw std::invalid_argument("a run needs at least one block");
+        if (g.nlevels < 1 || g.nlevels > 8) throw std::invalid_argument("nlevels must be 1..8");
+        if (L.bz == 1 && g.nlevels != 1) throw std::invalid_argument("axisymmetric runs have one level");
+        if (g.nr % BX || g.nth % BY || (L.bz > 1 && g.nph % L.bz)) throw std::invalid_argument("grid not a whole number of blocks");
+        if (!(g.r_in > 0.0) || !(g.r_out > g.r_in)) throw std::invalid_argument("need 0 < r_in < r_out");
+        if ((long long)free_slots.size() < nb) throw std::length_error("not enough free block slots");
+        int id = -1;
+        for (size_t r = 0; r < runs.size(); r++)
+            if (!runs[r].live) { id = (int)r; break; }
+        if (id < 0) { id = (int)runs.size(); runs.emplace_back(); }
+        HostRun& R = runs[id];
+        R = HostRun();
+        R.live = true;
+        R.grid = g;
+        R.par.a = ph.a; R.par.Q = ph.Q; R.par.gam = ph.gamma; R.par.cfl = ph.cfl;
+        R.par.rho_fl = ph.rho_floor; R.par.eps_fl = ph.eps_floor; R.par.sigma_max = ph.sigma_max;
+        R.par.w_max = ph.gamma_max; R.par.nlev = g.nlevels; R.par.nz = L.bz;
+        // levels and their tables
+        std::vector<std::vector<int>> lut(g.nlevels);
+        long long z0 = 0;
+        std::vector<DRL> lv(g.nlevels);
+        for (int l = 0; l < g.nlevels; l++) {
+            DRL v{};
+            const int f = 1 << l;
+            v.run = id; v.level = l;
+            v.nr = g.nr * f; v.nth = g.nth * f; v.nph = L.bz == 1 ? 1 : g.nph * f;
+            v.nbr = v.nr / BX; v.nbth = v.nth / BY; v.nbph = v.nph / L.bz;
+            v.npt = (v.nr + 5) * (v.nth + 5);
+            v.zone0 = z0;
+            z0 += (long long)v.nr * v.nth * v.nph;
+            v.dth = M_PI / v.nth;
+            v.dph = 2.0 * M_PI / v.nph;
+            lut[l].assign((size_t)v.nbr * v.nbth * v.nbph, -1);
+            lv[l] = v;
+        }
+        for (int q = 0; q < nb; q++) {
+            const int l = blocks[4 * q], bi = blocks[4 * q + 1], bj = blocks[4 * q + 2], bk = blocks[4 * q + 3];
+            if (l < 0 || l >= g.nlevels) throw std::invalid_argument("block level out of range");
+            const DRL& v = lv[l];
+            if (bi < 0 || bi >= v.nbr || bj < 0 || bj >= v.nbth || bk < 0 || bk >= v.nbph)
+                throw std::invalid_argument("block coordinates out of range");
+            int& e = lut[l][((size_t)bk * v.nbth + bj) * v.nbr + bi];
+            if (e >= 0) throw std::invalid_argument("block listed twice");
+            e = q;
+        }
+        check_cover(lv, lut, blocks, nb);
+        // slots
+        R.slots.resize(nb);
+        for (int q = 0; q < nb; q++) {
+            R.slots[q] = free_slots.back();
+            free_slots.pop_back();
+        }
+        for (int l = 0; l < g.nlevels; l++)
+            for (int& e : lut[l])
+                if (e >= 0) e = R.slots[e];
+        const double lnr0 = std::log(g.r_in);
+        for (int l = 0; l < g.nlevels; l++) {
+            DRL& v = lv[l];
+            const double dlnr = (std::log(g.r_out) - std::log(g.r_in)) / v.nr;
+            std::vector<double> gc, gf, dr;
+            build_level_tables(ph.a, ph.Q, ph.rho_floor, ph.eps_floor, v.nr, v.nth, lnr0, dlnr, gc, gf, dr);
+            std::vector<double> all;
+            all.reserve(gc.size() + gf.size() + dr.size());
+            all.insert(all.end(), gc.begin(), gc.end());
+            all.insert(all.end(), gf.begin(), gf.end());
+            all.insert(all.end(), dr.begin(), dr.end());
+            auto t = std::make_unique<DB<double>>();
+            t->upload(all.data(), all.size(), stream);
+            v.gc = t->p;
+            v.gf = t->p + gc.size();
+            v.dr = t->p + gc.size() + gf.size();
+            auto lu = std::make_unique<DB<int>>();
+            lu->upload(lut[l].data(), lut[l].size(), stream);
+            v.lut = lu->p;
+            SCHK(hipStreamSynchronize(stream));
+            R.tabs.push_back(std::move(t));
+            R.luts.push_back(std::move(lu));
+        }
+        // RL records
+        for (int l = 0; l < g.nlevels; l++) {
+            int slot = -1;
+            for (size_t x = 0; x < hrl.size(); x++)
+                if (hrl[x].run < 0) { slot = (int)x; break; }
+            if (slot < 0) { slot = (int)hrl.size(); hrl.push_back(DRL{}); }
+            hrl[slot] = lv[l];
+            R.rls.push_back(slot);
+        }
+        for (int l = 0; l < g.nlevels; l++) {
+            hrl[R.rls[l]].coarser = l > 0 ? R.rls[l - 1] : -1;
+            hrl[R.rls[l]].finer = l + 1 < g.nlevels ? R.rls[l + 1] : -1;
+        }
+        R.par.rl0 = R.rls[0];
+        for (int q = 0; q < nb; q++)
+            hblk[R.slots[q]] = Blk{R.rls[blocks[4 * q]], blocks[4 * q + 1], blocks[4 * q + 2], blocks[4 * q + 3]};
+        tables_dirty = true;
+        return id;
+    }
+
+    // Every ghost cell of every block must resolve: a leaf at its own level,
+    // a coarser leaf (one level down), or finer leaves (one level up) whose
+    // eight children are all leaves.
+    static void check_cover(const std::vector<DRL>& lv, const std::vector<std::vector<int>>& lut, const int* blocks,
+                            int nb) {
+        auto leaf = [&](int l, int gi, int gj, int gk) -> bool {
+            const DRL& v = lv[l];
+            if (gj < 0) gj = -1 - gj;
+            if (gj >= v.nth) gj = 2 * v.nth - 1 - gj;
+            gi = std::min(std::max(gi, 0), v.nr - 1);
+            gk = ((gk % v.nph) + v.nph) % v.nph;
+            const int bz = v.nph / v.nbph;
+            return lut[l][((size_t)(gk / bz) * v.nbth + gj / BY) * v.nbr + gi / BX] >= 0;
+        };
+        for (int q = 0; q < nb; q++) {
+            const int l = blocks[4 * q];
+            const DRL& v = lv[l];
+            const int bz = v.nph / v.nbph;
+            // sample the block's neighbourhood one and two cells outside its
+            // faces, edges and corners, at transverse offsets 0, 8 and 15
+            // (every block one level finer that a face touches)
+            const int offs[3] = {0, 8, 15};
+            for (int dk = -1; dk <= 1; dk++)
+                for (int dj = -1; dj <= 1; dj++)
+                    for (int di = -1; di <= 1; di++)
+                        for (int oc = 0; oc < 27; oc++) {
+                        const int oi = offs[oc % 3], oj = offs[(oc / 3) % 3], ok = bz > 1 ? offs[oc / 9] : 0;
+                        if ((di != 0 && oc % 3) || (dj != 0 && (oc / 3) % 3) || ((dk != 0 || bz == 1) && oc / 9)) continue;
+                        const int gi = blocks[4 * q + 1] * BX + (di < 0 ? -1 : di > 0 ? BX : oi);
+                        const int gj = blocks[4 * q + 2] * BY + (dj < 0 ? -1 : dj > 0 ? BY : oj);
+                        const int gk = blocks[4 * q + 3] * bz + (dk < 0 ? -1 : dk > 0 ? bz : ok);
+                        for (int s = 0; s < 2; s++) {  // both cells of the two ghost layers' coarse cell
+                            const int ggi = gi + (di < 0 ? -s : di > 0 ? s : 0);
+                            const int ggj = gj + (dj < 0 ? -s : dj > 0 ? s : 0);
+                            const int ggk = gk + (dk < 0 ? -s : dk > 0 ? s : 0);
+                            if (leaf(l, ggi, ggj, ggk)) continue;
+                            if (l > 0 && leaf(l - 1, ggi >> 1, ggj >> 1, ggk >> 1)) continue;
+                            bool fine = l + 1 < (int)lv.size();
+                            if (fine) {
+                                int Gi = ggi, Gj = ggj, Gk = ggk;
+                                if (Gj < 0) Gj = -1 - Gj;
+                                if (Gj >= v.nth) Gj = 2 * v.nth - 1 - Gj;
+                                Gi = std::min(std::max(Gi, 0), v.nr - 1);
+                                Gk = ((Gk % v.nph) + v.nph) % v.nph;
+                                for (int c = 0; c < 8 && fine; c++)
+                                    fine = leaf(l + 1, 2 * Gi + (c & 1), 2 * Gj + ((c >> 1) & 1), 2 * Gk + (c >> 2));
+                            }
+                            if (!fine)
+                                throw std::invalid_argument("block table: a ghost region has no leaf at the same, the "
+                                                            "next coarser or the next finer level (levels must be "
+                                                            "nested and 2:1 balanced)");
+                        }
+                    }
+        }
+    }
+
+    void remove_run(int id) {
+        HostRun& R = runs.at(id);
+        if (!R.live) throw std::invalid_argument("no such run");
+        SCHK(hipStreamSynchronize(stream));
+        for (int s : R.slots) {
+            hblk[s] = Blk{-1, 0, 0, 0};
+            free_slots.push_back(s);
+        }
+        for (int x : R.rls) hrl[x].run = -1;
+        R = HostRun();
+        tables_dirty = true;
+        graph_release();
+    }
+
+    void sync_tables() {
+        if (!tables_dirty) return;
+        SCHK(hipStreamSynchronize(stream));
+        blk.upload(hblk.data(), hblk.size(), stream);
+        rl.upload(hrl.data(), std::max<size_t>(1, hrl.size()), stream);
+        std::vector<RunPar> rp(std::max<size_t>(1, runs.size()));
+        for (size_t r = 0; r < runs.size(); r++) rp[r] = runs[r].par;
+        runp.upload(rp.data(), rp.size(), stream);
+        if (ctr.n < rp.size()) {
+            ctr.alloc(rp.size() + 16);
+            SCHK(hipMemsetAsync(ctr.p, 0, ctr.n * sizeof(RunCtr), stream));
+        }
+        SCHK(hipStreamSynchronize(stream));
+        tables_dirty = false;
+        graph_release();
+        act_runs.clear();
+    }
+
+    void set_active(const std::vector<int>& rr) {
+        sync_tables();
+        if (rr == act_runs && list.p) return;
+        std::vector<int> sl;
+        for (int r : rr) {
+            if (r < 0 || r >= (int)runs.size() || !runs[r].live) throw std::invalid_argument("no such run");
+            sl.insert(sl.end(), runs[r].slots.begin(), runs[r].slots.end());
+        }
+        list.upload(sl.data(), sl.size(), stream);
+        runlist.upload(rr.data(), rr.size(), stream);
+        SCHK(hipStreamSynchronize(stream));
+        act_runs = rr;
+        act_slots = sl;
+        graph_release();
+    }
+
+    // ---- step
+    int persistent_grid() {
+        int occ = 0;
+        if (mixed) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<float>, TPB, 0));
+        else SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<double>, TPB, 0));
+        int dev = 0, ncu = 0;
+        SCHK(hipGetDevice(&dev));
+        SCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        int per = occ;
+        if (wg_per_cu > 0) per = std::min(per, wg_per_cu);
+        per = std::max(per, 1);
+        return std::max(1, std::min(ncu * per, (int)act_slots.size()));
+    }
+
+    template <class Rt>
+    void enqueue_step() {
+        const int nl = (int)act_slots.size();
+        const int nr = (int)act_runs.size();
+        KPS P = kp(list.p, nl);
+        hipLaunchKernelGGL(k_begin, dim3((std::max(nr, 4) + 63) / 64), dim3(64), 0, stream, ctr.p, runlist.p, nr, qctr.p);
+        for (int stage = 0; stage < 2; stage++) {
+            if (mode == 1) {
+                if (stage == 1) hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, stream, ctr.p, runlist.p, 0, qctr.p);
+                hipLaunchKernelGGL(k_stage<Rt>, dim3(persistent_grid()), dim3(TPB), 0, stream, P, stage);
+                continue;
+            }
+            hipLaunchKernelGGL(k_c2p, dim3(nl), dim3(TPB), 0, stream, P, stage == 0 ? 1 : 0);
+            const int buf = stage;
+            hipLaunchKernelGGL(k_ghost_cells, dim3(nl * ((nghc + TPB - 1) / TPB)), dim3(TPB), 0, stream, P);
+            hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
+            hipLaunchKernelGGL(k_flux<Rt>, dim3(nl * ((nflf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
+            hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+            hipLaunchKernelGGL(k_update<Rt>, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+        }
+        SCHK(hipGetLastError());
+    }
+    void enqueue() {
+        if (mixed) enqueue_step<float>();
+        else enqueue_step<double>();
+    }
+    int key() const { return (mixed ? 1 : 0) | (mode << 1) | (wg_per_cu << 3); }
+    void step() {
+        if (act_slots.empty()) throw std::invalid_argument("no active runs");
+        if (!use_graph) {
+            enqueue();
+            return;
+        }
+        if (!gexec || graph_key != key()) {
+            graph_release();
+            SCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            enqueue();
+            SCHK(hipStreamEndCapture(stream, &graph));
+            SCHK(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            graph_key = key();
+        }
+        SCHK(hipGraphLaunch(gexec, stream));
+    }
+    void sync() {
+        SCHK(hipGetLastError());
+        SCHK(hipStreamSynchronize(stream));
+    }
+    void check_barrier() {
+        if (mode != 1) return;
+        unsigned f = 0;
+        SCHK(hipMemcpyAsync(&f, qctr.p + 3, sizeof f, hipMemcpyDeviceToHost, stream));
+        SCHK(hipStreamSynchronize(stream));
+        if (f) throw std::runtime_error("persistent stage: grid barrier timed out (workgroups not co-resident)");
+    }
+    void read_ctr(std::vector<RunCtr>& out) {
+        out.resize(runs.size());
+        if (!runs.empty()) SCHK(hipMemcpyAsync(out.data(), ctr.p, runs.size() * sizeof(RunCtr), hipMemcpyDeviceToHost, stream));
+        SCHK(hipStreamSynchronize(stream));
+    }
+
+    // con2prim of one run's active zones (W of the current U), no dt
+    void c2p_run(int r) {
+        set_active({r});
+        KPS P = kp(list.p, (int)act_slots.size());
+        hipLaunchKernelGGL(k_c2p, dim3(act_slots.size()), dim3(TPB), 0, stream, P, 0);
+        sync();
+    }
+
+    void init_torus(int r, const bhs_torus& t, double* info) {
+        set_active({r});
+        const HostRun& R = runs[r];
+        TorusDev D;
+        D.T.a = R.par.a; D.T.gam = R.par.gam; D.T.r_in = t.r_in; D.T.r_max = t.r_max; D.T.kappa = t.kappa;
+        D.T.l = fm_l(R.par.a, t.r_max);
+        D.T.lnh_in = fm_lnh_raw<HostCR>(D.T, t.r_in, 1.0, 0.0);
+        D.beta = t.beta; D.acut = t.a_cut; D.pamp = t.perturbation;
+        D.st = U128{t.pcg_state_hi, t.pcg_state_lo};
+        D.inc = U128{t.pcg_inc_hi, t.pcg_inc_lo};
+        const int nl = (int)act_slots.size();
+        scratch.alloc((size_t)cap * L.ncell);
+        KPS P = kp(list.p, nl);
+        // clear the run's state: face fields (incl. phi faces), W
+        for (int s : R.slots) {
+            SCHK(hipMemsetAsync(Bf0.p + (size_t)s * 3 * L.nf, 0, 3 * L.nf * sizeof(double), stream));
+            SCHK(hipMemsetAsync(Bf1.p + (size_t)s * 3 * L.nf, 0, 3 * L.nf * sizeof(double), stream));
+            SCHK(hipMemsetAsync(W.p + (size_t)s * 8 * L.ncell, 0, 8 * L.ncell * sizeof(double), stream));
+            SCHK(hipMemsetAsync(U.p + (size_t)s * 8 * L.ncell, 0, 8 * L.ncell * sizeof(double), stream));
+        }
+        SCHK(hipMemsetAsync(icmax.p, 0, 4 * sizeof(unsigned long long), stream));
+        hipLaunchKernelGGL(k_ic_torus, dim3(nl * ((L.ncell + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+                           icmax.p);
+        hipLaunchKernelGGL(k_ic_faces, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+                           icmax.p);
+        hipLaunchKernelGGL(k_ic_cells, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+                           icmax.p);
+        hipLaunchKernelGGL(k_ic_scale, dim3(nl * ((3 * L.nf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, icmax.p,
+                           t.beta, icscale.p);
+        hipLaunchKernelGGL(k_ic_final, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p);
+        SCHK(hipGetLastError());
+        unsigned long long mx[4];
+        double sc = 0.0;
+        SCHK(hipMemcpyAsync(mx, icmax.p, sizeof mx, hipMemcpyDeviceToHost, stream));
+        SCHK(hipMemcpyAsync(&sc, icscale.p, sizeof sc, hipMemcpyDeviceToHost, stream));
+        sync();
+        if (info) {
+            for (int q = 0; q < 3; q++) std::memcpy(&info[q], &mx[q], sizeof(double));
+            info[3] = sc;
+        }
+        double rm;
+        std::memcpy(&rm, &mx[0], sizeof rm);
+        if (!(rm > 0.0)) throw std::domain_error("no torus zone on the grid");
+    }
+    struct HostCR {
+        static double log(double x) { return bh::cr_log(x); }
+        static double exp(double x) { return bh::cr_exp(x); }
+        static double pow(double x, double y) { return bh::cr_pow(x, y); }
+    };
+
+    void get_state(int r, double* hU, double* hW, double* hB) {
+        const HostRun& R = runs.at(r);
+        sync();
+        for (size_t q = 0; q < R.slots.size(); q++) {
+            const int s = R.slots[q];
+            if (hU) SCHK(hipMemcpyAsync(hU + q * 8 * L.ncell, U.p + (size_t)s * 8 * L.ncell, 8 * L.ncell * sizeof(double), hipMemcpyDeviceToHost, stream));
+            if (hW) SCHK(hipMemcpyAsync(hW + q * 8 * L.ncell, W.p + (size_t)s * 8 * L.ncell, 8 * L.ncell * sizeof(double), hipMemcpyDeviceToHost, stream));
+            if (hB) SCHK(hipMemcpyAsync(hB + q * 3 * L.nf, Bf0.p + (size_t)s * 3 * L.nf, 3 * L.nf * sizeof(double), hipMemcpyDeviceToHost, stream));
+        }
+        sync();
+    }
+    void set_state(int r, const double* hU, const double* hW, const double* hB) {
+        const HostRun& R = runs.at(r);
+        sync();
+        for (size_t q = 0; q < R.slots.size(); q++) {
+            const int s = R.slots[q];
+            if (hU) SCHK(hipMemcpyAsync(U.p + (size_t)s * 8 * L.ncell, hU + q * 8 * L.ncell, 8 * L.ncell * sizeof(double), hipMemcpyHostToDevice, stream));
+            if (hW) SCHK(hipMemcpyAsync(W.p + (size_t)s * 8 * L.ncell, hW + q * 8 * L.ncell, 8 * L.ncell * sizeof(double), hipMemcpyHostToDevice, stream));
+            if (hB) SCHK(hipMemcpyAsync(Bf0.p + (size_t)s * 3 * L.nf, hB + q * 3 * L.nf, 3 * L.nf * sizeof(double), hipMemcpyHostToDevice, stream));
+        }
+        sync();
+    }
+
+    void diagnostics(int r, double* out) {
+        set_active({r});
+        const int nl = (int)act_slots.size();
+        diagbuf.alloc((size_t)nl * 4);
+        KPS P = kp(list.p, nl);
+        hipLaunchKernelGGL(k_diag, dim3(nl), dim3(TPB), 0, stream, P, diagbuf.p);
+        std::vector<double> h((size_t)nl * 4);
+        SCHK(hipMemcpyAsync(h.data(), diagbuf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        std::vector<RunCtr> c;
+        read_ctr(c);
+        double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0;
+        for (int q = 0; q < nl; q++) {
+            ms += h[4 * q];
+            en += h[4 * q + 1];
+            dm = std::max(dm, h[4 * q + 2]);
+            sc = std::max(sc, h[4 * q + 3]);
+        }
+        out[0] = ms; out[1] = en; out[2] = dm; out[3] = sc;
+        out[4] = (double)c[r].c2p_fail;
+        out[5] = (double)c[r].floors;
+    }
+};
+
+}  // namespace sph
+
+// ============================================================ C ABI
+using namespace sph;
+
+struct bhs_engine {
+    std::unique_ptr<Engine> e;
+    std::string err;
+};
+
+namespace {
+thread_local std::string g_err;
+template <class Fn>
+int guard(bhs_engine* h, Fn&& fn) {
+    if (!h) return BHS_EINVAL;
+    try {
+        return fn();
+    } catch (const std::invalid_argument& x) {
+        h->err = x.what();
+        return BHS_EINVAL;
+    } catch (const std::length_error& x) {
+        h->err = x.what();
+        return BHS_ENOMEM;
+    } catch (const std::domain_error& x) {
+        h->err = x.what();
+        return BHS_ESTATE;
+    } catch (const std::exception& x) {
+        h->err = x.what();
+        return BHS_EDEVICE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int bhs_create(int bz, long long max_blocks, bhs_engine** out) {
+    if (!out) return BHS_EINVAL;
+    *out = nullptr;
+    if (bz != 1 && bz != BX) { g_err = "bz must be 1 (axisymmetric) or 16"; return BHS_EINVAL; }
+    if (max_blocks <= 0) { g_err = "max_blocks must be positive"; return BHS_EINVAL; }
+    try {
+        auto h = std::make_unique<bhs_engine>();
+        h->e = std::make_unique<Engine>(bz, max_blocks);
+        *out = h.release();
+        return BHS_OK;
+    } catch (const std::exception& x) {
+        g_err = x.what();
+        return BHS_EDEVICE;
+    }
+}
+void bhs_destroy(bhs_engine* h) { delete h; }
+const char* bhs_last_error(const bhs_engine* h) { return h ? h->err.c_str() : g_err.c_str(); }
+
+long long bhs_block_bytes(int bz) {
+    const Lay L = make_lay(bz);
+    return (long long)sizeof(double) * (2LL * 8 * L.ncell + 5LL * L.nint + 2LL * 3 * L.nf + 3LL * NFW * L.nf);
+}
+
+int bhs_run_add(bhs_engine* h, const bhs_physics* ph, const bhs_grid* g, int nblocks, const int* blocks, int* run) {
+    return guard(h, [&] {
+        if (!ph || !g || !blocks || !run) throw std::invalid_argument("null argument");
+        *run = h->e->add_run(*ph, *g, nblocks, blocks);
+        return BHS_OK;
+    });
+}
+int bhs_run_remove(bhs_engine* h, int run) {
+    return guard(h, [&] {
+        h->e->remove_run(run);
+        return BHS_OK;
+    });
+}
+int bhs_run_nblocks(bhs_engine* h, int run, int* nb) {
+    return guard(h, [&] {
+        if (run < 0 || run >= (int)h->e->runs.size() || !h->e->runs[run].live) throw std::invalid_argument("no such run");
+        *nb = (int)h->e->runs[run].slots.size();
+        return BHS_OK;
+    });
+}
+int bhs_run_init_torus(bhs_engine* h, int run, const bhs_torus* t, double* info) {
+    return guard(h, [&] {
+        if (!t) throw std::invalid_argument("null torus");
+        h->e->init_torus(run, *t, info);
+        return BHS_OK;
+    });
+}
+int bhs_run_get_state(bhs_engine* h, int run, double* U, double* W, double* Bf) {
+    return guard(h, [&] {
+        h->e->get_state(run, U, W, Bf);
+        return BHS_OK;
+    });
+}
+int bhs_run_set_state(bhs_engine* h, int run, const double* U, const double* W, const double* Bf) {
+    return guard(h, [&] {
+        h->e->set_state(run, U, W, Bf);
+        return BHS_OK;
+    });
+}
+int bhs_set_precision(bhs_engine* h, int mixed) {
+    return guard(h, [&] {
+        h->e->mixed = mixed != 0;
+        return BHS_OK;
+    });
+}
+int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
+    return guard(h, [&] {
+        if (wg_per_cu < 0 || wg_per_cu > 32) throw std::invalid_argument("wg_per_cu out of range");
+        h->e->mode = persistent ? 1 : 0;
+        h->e->wg_per_cu = wg_per_cu;
+        return BHS_OK;
+    });
+}
+int bhs_use_graph(bhs_engine* h, int enable) {
+    return guard(h, [&] {
+        h->e->use_graph = enable != 0;
+        h->e->graph_release();
+        return BHS_OK;
+    });
+}
+int bhs_steps(bhs_engine* h, int nruns, const int* runs, int nsteps, double* dt_out) {
+    return guard(h, [&] {
+        if (nruns <= 0 || !runs || nsteps < 0) throw std::invalid_argument("bad run list or step count");
+        Engine& E = *h->e;
+        E.set_active(std::vector<int>(runs, runs + nruns));
+        std::vector<RunCtr> c;
+        for (int s = 0; s < nsteps; s++) {
+            E.step();
+            if (dt_out) {
+                // dt of each run: read after the step (it is also the value the
+                // step used)
+                E.read_ctr(c);
+                for (int q = 0; q < nruns; q++)
+                    dt_out[(size_t)s * nruns + q] = __builtin_bit_cast(double, c[runs[q]].dtbits);
+            }
+        }
+        E.sync();
+        E.check_barrier();
+        return BHS_OK;
+    });
+}
+int bhs_c2p(bhs_engine* h, int run) {
+    return guard(h, [&] {
+        h->e->c2p_run(run);
+        return BHS_OK;
+    });
+}
+int bhs_run_diagnostics(bhs_engine* h, int run, double* out) {
+    return guard(h, [&] {
+        h->e->diagnostics(run, out);
+        return BHS_OK;
+    });
+}
+int bhs_sync(bhs_engine* h) {
+    return guard(h, [&] {
+        h->e->sync();
+        return BHS_OK;
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+/* bhsph: C ABI of the spherical Kerr-Schild engine (MI355X, gfx950).
+ *
+ * The reference's fluid hot path is rk2integrator::step on an amrtree of
+ * Cartesian patches (src/fluid/rk2.hpp:24, src/fluid/solver/integrator.cpp:
+ * 262-275; entry point bh_rk2_step in bhfluid.h).  This engine runs the same
+ * step structure -- ghost fill, con2prim, PLM + Riemann fluxes, constrained
+ * transport, flux divergence + geometric sources, RK2, floors -- on blocks of a
+ * logarithmic-r x theta x phi grid in Kerr-Schild coordinates with
+ * self-consistent physics, for any number of independent runs advanced in
+ * shared launches.  Each entry point names the reference member it stands in
+ * for; plain pointers and sizes only; every call returns 0 or a negative code
+ * with the reason in bhs_last_error().
+ *
+ * Layout of one block (16 x 16 x bz active zones in r, theta, phi; bz = 16,
+ * or 1 for an axisymmetric engine; two ghost layers):
+ *   cell index  ((k+2)*20 + (j+2))*20 + (i+2),  sz = bz + 4 stored phi layers
+ *   U [8][20*20*sz]  sqrt(gamma) x (D, S_r, S_theta, S_phi, tau, B^r, B^theta, B^phi)
+ *   W [8][20*20*sz]  rho, eps, W v^r, W v^theta, W v^phi, B^r, B^theta, B^phi
+ *   Bf[3][nf]        sqrt(gamma) B^d on the faces of direction d (face index
+ *                    as the cell index with 21 entries along d);
+ *                    nf = max(21*20*sz, 20*20*(sz+1))
+ * A run's state arrays hold its blocks in the order of its block table.
+ * Block table: blocks[n][4] = level, bi, bj, bk (block coordinates at that
+ * level; level l has 2^l times the level-0 zones in each direction).
+ */
+#ifndef BHSPH_H
+#define BHSPH_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BHS_OK 0
+#define BHS_EINVAL (-1)
+#define BHS_EDEVICE (-2)
+#define BHS_ESTATE (-3)
+#define BHS_ENOMEM (-4)
+
+typedef struct bhs_engine bhs_engine;
+
+/* Spacetime, gas and floors of one run: the reference's compile-time
+ * parameters (src/fluid/main.cpp:61-81) and floor constants
+ * (src/fluid/grid.hpp:140-143): rho_floor * r^-3/2, eps_floor * r^-3/2, plus
+ * a magnetisation ceiling b^2/rho <= sigma_max and a Lorentz-factor ceiling. */
+typedef struct {
+    double a, Q, gamma, cfl;
+    double rho_floor, eps_floor, sigma_max, gamma_max;
+} bhs_physics;
+
+/* Level-0 grid: nr x nth x nph zones over r in [r_in, r_out] (logarithmic),
+ * theta in [0, pi], phi in [0, 2 pi) (nph ignored by an axisymmetric engine);
+ * nlevels refinement levels. */
+typedef struct {
+    int nr, nth, nph, nlevels;
+    double r_in, r_out;
+} bhs_grid;
+
+/* Fishbone-Moncrief torus (torus::init::fm_init and B_pot_init,
+ * src/fluid/solver/initconds.cpp:121-668): inner edge, pressure maximum,
+ * polytropic constant; A_phi = max(rho/rho_max - a_cut, 0), the field scaled
+ * so that max p_gas / max p_mag = beta; pressure times 1 + perturbation (2u-1)
+ * with u output n of numpy's PCG64 stream of the given 128-bit state and
+ * increment, n = the zone's index in the run (level by level, phi slowest,
+ * r fastest). */
+typedef struct {
+    double r_in, r_max, kappa, beta, a_cut, perturbation;
+    unsigned long long pcg_state_hi, pcg_state_lo, pcg_inc_hi, pcg_inc_lo;
+} bhs_torus;
+
+/* A device engine with room for max_blocks blocks of shape 16 x 16 x bz
+ * (grid::amrtree's storage, src/fluid/grid.hpp:109-144). */
+int bhs_create(int bz, long long max_blocks, bhs_engine** out);
+void bhs_destroy(bhs_engine* e);
+const char* bhs_last_error(const bhs_engine* e);
+/* Device bytes one block slot takes (state, RK store, face fields, fluxes). */
+long long bhs_block_bytes(int bz);
+
+/* Add a run with a caller-owned block table (grid::amrtree's quilt,
+ * src/fluid/grid.hpp:119, after regrid, grid.cpp:739-835).  Levels must be
+ * nested and 2:1 balanced.  *run receives its id. */
+int bhs_run_add(bhs_engine* e, const bhs_physics* phys, const bhs_grid* grid, int nblocks, const int* blocks,
+                int* run);
+/* Free a run's block slots and tables. */
+int bhs_run_remove(bhs_engine* e, int run);
+int bhs_run_nblocks(bhs_engine* e, int run, int* nblocks);
+/* Initial data on the device (main.cpp:94-159 with the torus above).
+ * info[4] = rho_max (unnormalised), max p_gas, max p_mag before scaling, the
+ * field scale factor. */
+int bhs_run_init_torus(bhs_engine* e, int run, const bhs_torus* torus, double* info);
+int bhs_run_get_state(bhs_engine* e, int run, double* U, double* W, double* Bf);
+int bhs_run_set_state(bhs_engine* e, int run, const double* U, const double* W, const double* Bf);
+
+/* nsteps of rk2integrator::step (integrator.cpp:262-275) for the listed
+ * runs together, each with its own CFL dt; dt_out[nsteps][nruns] (may be
+ * NULL). */
+int bhs_steps(bhs_engine* e, int nruns, const int* runs, int nsteps, double* dt_out);
+/* con2prim with floors of a run's active zones (W of the current U;
+ * integrator.cpp:201-222). */
+int bhs_c2p(bhs_engine* e, int run);
+/* out[6] = rest mass, Killing energy (sum of alpha (tau + D) - beta^j S_j,
+ * densitised), max |div B| over the active zones (constrans::maxdivB,
+ * src/fluid/grid/transport.cpp:77-89), the divergence stencil's scale,
+ * con2prim failures and floored zones since the run was added. */
+int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
+
+/* Launch configuration (no reference counterpart): mixed 1 = FP32
+ * reconstruction, Riemann fluxes and EMFs (con2prim, sources, update and
+ * reductions FP64); persistent 1 = one queue-driven kernel per RK stage with
+ * wg_per_cu workgroups per compute unit (0 = as many as fit); graphs on by
+ * default. */
+int bhs_set_precision(bhs_engine* e, int mixed);
+int bhs_set_schedule(bhs_engine* e, int persistent, int wg_per_cu);
+int bhs_use_graph(bhs_engine* e, int enable);
+int bhs_sync(bhs_engine* e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

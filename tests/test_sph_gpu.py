@@ -1,0 +1,177 @@
+"""GPU parity of the spherical Kerr-Schild engine against its oracle, through
+the C ABI (include/bhsph.h).
+
+The oracle (oracle/sphoracle.cpp) restates the engine's orchestration in
+sequential CPU loops and shares only the pointwise physics headers, which
+tests/test_sph_oracle.py pins against mathematics.  The bar is the north
+star's: relative L-infinity (per variable, max |a - b| / max |b|) at most
+1e-10 on primitives after 10 steps, FP64; the device and the oracle perform
+the same floating-point operations, so the measured difference is 0.
+"""
+import numpy as np
+import pytest
+
+from bhfluid import sph
+from sph_oracle import OracleRun
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def _cfg(grid, bz):
+    p = sph.PHYSICS
+    return dict(bz=bz, a=p["a"], Q=p["Q"], gamma=p["gamma"], cfl=p["cfl"], rho_fl=p["rho_floor"],
+                eps_fl=p["eps_floor"], sigma_max=p["sigma_max"], w_max=p["gamma_max"], nr=grid["nr"],
+                nth=grid["nth"], nph=grid["nph"], r_in=grid["r_in"], r_out=grid["r_out"], nlev=grid["nlevels"])
+
+
+def _tor():
+    t = sph.TORUS
+    return dict(r_in=t["r_in"], r_max=t["r_max"], kappa=t["kappa"], beta=t["beta"], acut=t["a_cut"],
+                pert=t["perturbation"])
+
+
+def rel_linf(a, b):
+    """max over variables of max|a - b| / max|b| over the zones where both
+    are finite; the non-finite masks must agree."""
+    assert (np.isfinite(a) == np.isfinite(b)).all(), "non-finite masks differ"
+    worst = 0.0
+    for v in range(a.shape[1]):
+        x, y = a[:, v], b[:, v]
+        m = np.isfinite(x) & np.isfinite(y)
+        den = np.abs(y[m]).max() if m.any() else 0.0
+        if den > 0:
+            worst = max(worst, float(np.abs(x[m] - y[m]).max() / den))
+    return worst
+
+
+def make_pair(grid, bz, blocks, mixed=False, persistent=False, max_blocks=None):
+    eng = sph.Engine(bz, max_blocks or len(blocks))
+    eng.set_precision(mixed)
+    eng.set_schedule(persistent)
+    run = eng.add_run(sph.PHYSICS, grid, blocks)
+    info_g = run.init_torus(sph.TORUS, 42)
+    orc = OracleRun(_cfg(grid, bz), blocks)
+    info_o = orc.init_torus(_tor(), 42)
+    return eng, run, orc, info_g, info_o
+
+
+def compare_states(run, orc, what):
+    Ug, Wg, Bg = run.state()
+    Uo, Wo, Bo = orc.state()
+    ru = rel_linf(run.active(Ug), run.active(Uo))
+    rw = rel_linf(run.active(Wg), run.active(Wo))
+    rb = rel_linf(Bg, Bo)
+    nd = int((Ug != Uo).sum() + (Bg != Bo).sum())
+    print(f"[{what}] rel Linf U {ru:.3e} W {rw:.3e} Bf {rb:.3e}; differing words {nd}", flush=True)
+    return ru, rw, rb
+
+
+def _steps(eng, run, orc, nsteps, mixed, name):
+    for s in range(nsteps):
+        dtg = eng.step([run])[0, 0]
+        dto = orc.step(mixed)
+        assert dtg == dto or abs(dtg - dto) <= TOL * abs(dto), (s, dtg, dto)
+        ru, rw, rb = compare_states(run, orc, f"{name} step {s}")
+        assert ru <= TOL and rb <= TOL, (s, ru, rb)
+    run.c2p()
+    orc.c2p()
+    ru, rw, rb = compare_states(run, orc, f"{name} primitives after {nsteps} steps")
+    assert rw <= TOL
+
+
+def test_c1_initial_data_matches_oracle():
+    g = sph.c1_grid()
+    eng, run, orc, ig, io = make_pair(g, 1, sph.uniform_blocks(g, 1))
+    print("init info gpu", ig, "oracle", io)
+    assert np.array_equal(ig, io)
+    ru, rw, rb = compare_states(run, orc, "C1 initial data")
+    assert ru == 0.0 and rw == 0.0 and rb == 0.0
+
+
+def test_c1_ten_steps_fp64():
+    g = sph.c1_grid()
+    eng, run, orc, _, _ = make_pair(g, 1, sph.uniform_blocks(g, 1))
+    _steps(eng, run, orc, 10, False, "C1")
+    d = run.diagnostics()
+    assert d["divb_max"] <= 1e-13 * d["divb_scale"]
+
+
+def test_c1_ten_steps_mixed():
+    g = sph.c1_grid()
+    eng, run, orc, _, _ = make_pair(g, 1, sph.uniform_blocks(g, 1), mixed=True)
+    _steps(eng, run, orc, 10, True, "C1 mixed")
+
+
+def test_c2_cut_down_ten_steps_fp64():
+    """C2's 3D torus on a 64 x 32 x 32 grid (16 blocks), 10 steps."""
+    g = dict(sph.c2_grid(), nr=64, nth=32, nph=32)
+    eng, run, orc, _, _ = make_pair(g, 16, sph.uniform_blocks(g, 16))
+    _steps(eng, run, orc, 10, False, "C2/64")
+
+
+def test_amr_three_levels_fp64():
+    """Three levels (4, 40 and 192 blocks): prolongation, restriction and
+    face-field transfer across levels."""
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    assert sorted(set(blocks[:, 0].tolist())) == [0, 1, 2]
+    eng, run, orc, _, _ = make_pair(g, 16, blocks)
+    _steps(eng, run, orc, 4, False, "AMR3")
+
+
+def test_persistent_schedule_identical():
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    a = sph.Engine(16, len(blocks))
+    b = sph.Engine(16, len(blocks))
+    b.set_schedule(True)
+    ra = a.add_run(sph.PHYSICS, g, blocks)
+    rb = b.add_run(sph.PHYSICS, g, blocks)
+    ra.init_torus(sph.TORUS, 42)
+    rb.init_torus(sph.TORUS, 42)
+    for s in range(3):
+        da = a.step([ra])
+        db = b.step([rb])
+        assert np.array_equal(da, db)
+        for x, y in zip(ra.state(), rb.state()):
+            assert np.array_equal(x, y), f"step {s}"
+
+
+def test_graph_replays_identical_to_eager():
+    g = dict(sph.c2_grid(), nr=64, nth=32, nph=32)
+    blocks = sph.uniform_blocks(g, 16)
+    a = sph.Engine(16, len(blocks))
+    b = sph.Engine(16, len(blocks))
+    b.use_graph(False)
+    ra = a.add_run(sph.PHYSICS, g, blocks)
+    rb = b.add_run(sph.PHYSICS, g, blocks)
+    ra.init_torus(sph.TORUS, 42)
+    rb.init_torus(sph.TORUS, 42)
+    for s in range(3):
+        a.step([ra])
+        b.step([rb])
+        ra.diagnostics()  # eager work between replays
+        for x, y in zip(ra.state(), rb.state()):
+            assert np.array_equal(x, y), f"replay {s}"
+
+
+def test_c2_full_size_stays_finite_div_b_round_off():
+    """C2 at its full size (256 x 128 x 128, 1024 blocks), 10 steps FP64:
+    finite, max |div B| at round-off of the stencil scale, mass and energy
+    within the boundary outflow of their initial values."""
+    g = sph.c2_grid()
+    blocks = sph.uniform_blocks(g, 16)
+    eng = sph.Engine(16, len(blocks))
+    run = eng.add_run(sph.PHYSICS, g, blocks)
+    run.init_torus(sph.TORUS, 42)
+    d0 = run.diagnostics()
+    eng.step([run], 10)
+    run.c2p()
+    d = run.diagnostics()
+    U, W, Bf = run.state()
+    assert np.isfinite(run.active(U)).all() and np.isfinite(run.active(W)).all()
+    print("C2 diagnostics", d0, d)
+    assert d["divb_max"] <= 1e-13 * d["divb_scale"]
+    assert abs(d["mass"] - d0["mass"]) <= 1e-4 * d0["mass"]
