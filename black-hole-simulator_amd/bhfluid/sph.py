@@ -81,6 +81,11 @@ def lib():
         sig("bhs_set_schedule", _i, [_v, _i, _i])
         sig("bhs_use_graph", _i, [_v, _i])
         sig("bhs_sync", _i, [_v])
+        sig("bhs_steps_timed", _i, [_v, _i, _ip, _i, _dp])
+        sig("bhs_set_variants", _i, [_v, _i, _i, _i])
+        sig("bhs_variant_info", _i, [_i, _ip, _ip])
+        sig("bhs_profile", _i, [_v, _i, ctypes.c_char_p])
+        sig("bhs_profile_report", _i, [_v, ctypes.c_char_p, _i])
         _sigs_done = True
     return L
 
@@ -153,6 +158,16 @@ class Engine:
     def set_schedule(self, persistent: bool, wg_per_cu: int = 0):
         self._check(self._L.bhs_set_schedule(self._h, int(bool(persistent)), int(wg_per_cu)), "set_schedule")
 
+    def set_variants(self, flux: int = 0, update: int = 0, c2p: int = 0):
+        self._check(self._L.bhs_set_variants(self._h, int(flux), int(update), int(c2p)), "set_variants")
+
+    @staticmethod
+    def variant_info(v: int):
+        tb, mw = ctypes.c_int(), ctypes.c_int()
+        if lib().bhs_variant_info(int(v), ctypes.byref(tb), ctypes.byref(mw)):
+            raise BhError(f"no launch variant {v}")
+        return tb.value, mw.value
+
     def use_graph(self, enable: bool = True):
         self._check(self._L.bhs_use_graph(self._h, int(bool(enable))), "use_graph")
 
@@ -167,6 +182,29 @@ class Engine:
 
     def sync(self):
         self._check(self._L.bhs_sync(self._h), "sync")
+
+    def step_timed(self, runs: Sequence["Run"], nsteps: int) -> np.ndarray:
+        """``nsteps`` steps (graph replays) with a HIP event between
+        consecutive steps; returns each step's device time in ms."""
+        ids = np.array([r.id for r in runs], dtype=np.int32)
+        ms = np.zeros(nsteps)
+        self._check(self._L.bhs_steps_timed(self._h, len(ids), ids.ctypes.data_as(_ip), int(nsteps), _p(ms)),
+                    "steps_timed")
+        return ms
+
+    def profile(self, enable: bool = True, only: str = ""):
+        self._check(self._L.bhs_profile(self._h, int(bool(enable)), only.encode()), "profile")
+
+    def profile_report(self) -> Dict[str, dict]:
+        buf = ctypes.create_string_buffer(1 << 16)
+        n = self._L.bhs_profile_report(self._h, buf, len(buf))
+        if n < 0:
+            self._check(n, "profile_report")
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, cnt, tot = line.split()
+            out[name] = dict(launches=int(cnt), total_ms=float(tot), avg_ms=float(tot) / max(1, int(cnt)))
+        return out
 
 
 class Run:

@@ -367,46 +367,86 @@ __device__ void phase_a_block(const KPS& P, int b, bool want_dt) {
 }
 
 // ---------------------------------------------------------------- phase B
-template <class Rt>
-__device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, int j, int k) {
+// the face of direction D (a template parameter: every state index is then
+// static and the states stay in registers)
+template <class Rt, int D>
+__device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int b, int i, int j, int k) {
+    constexpr int d = D;
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
     const RunPar& R = P.run[v.run];
     const int gi = B.bi * BX + i, gj = B.bj * BY + j;
     const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
     const double* tab = d == 0 ? v.gf : d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc;
-    const Met<Rt> m = load_met<Rt>(tab, v.npt, p);
-    const int o0 = d == 0, o1 = d == 1, o2 = d == 2;
-    const int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
-    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
-    Rt s[4][8];
+    // the face metric: of gamma^ij only gamma^nn enters the flux
+    Met<Rt> m;
+    {
+        const long long np = v.npt;
+        m.alpha = (Rt)tab[M_ALPHA * np + p];
+        m.beta[0] = (Rt)tab[M_B1 * np + p];
+        m.beta[1] = (Rt)tab[M_B2 * np + p];
+        m.beta[2] = (Rt)tab[M_B3 * np + p];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-        const int cc = cidx(i + (c - 2) * o0, j + (c - 2) * o1, k + (c - 2) * o2);
-#pragma unroll
-        for (int q = 0; q < 8; q++) s[c][q] = (Rt)Wb[(size_t)q * P.ncell + cc];
+        for (int q = 0; q < 6; q++) {
+            m.g[q] = (Rt)tab[(M_G11 + q) * np + p];
+            m.gi[q] = Rt(0);
+        }
+        constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
+        m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
+        m.sg = (Rt)tab[M_SG * np + p];
     }
+    constexpr int o0 = d == 0, o1 = d == 1, o2 = d == 2;
+    constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
+    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    int cc[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) cc[c] = cidx(i + (c - 2) * o0, j + (c - 2) * o1, k + (c - 2) * o2);
+    // one variable at a time: four stencil loads, then its two face values
+    auto ld = [&](int q, Rt x[4]) {
+#pragma unroll
+        for (int c = 0; c < 4; c++) x[c] = (Rt)Wb[(size_t)q * P.ncell + cc[c]];
+    };
     Pt<Rt> PL, PR;
-    Rt pc[4];
+    {
+        Rt r4[4], e4[4], pc[4];
+        ld(WRHO, r4);
+        ld(WEPS, e4);
 #pragma unroll
-    for (int c = 0; c < 4; c++) pc[c] = (Rt)(R.gam - 1.0) * s[c][WRHO] * s[c][WEPS];
-    plm(s[0][WRHO], s[1][WRHO], s[2][WRHO], s[3][WRHO], PL.rho, PR.rho);
-    plm(pc[0], pc[1], pc[2], pc[3], PL.p, PR.p);
+        for (int c = 0; c < 4; c++) pc[c] = (Rt)(R.gam - 1.0) * r4[c] * e4[c];
+        plm(r4[0], r4[1], r4[2], r4[3], PL.rho, PR.rho);
+        plm(pc[0], pc[1], pc[2], pc[3], PL.p, PR.p);
+    }
 #pragma unroll
-    for (int e = 0; e < 3; e++) plm(s[0][WU1 + e], s[1][WU1 + e], s[2][WU1 + e], s[3][WU1 + e], PL.u[e], PR.u[e]);
-    plm(s[0][WB1 + t1], s[1][WB1 + t1], s[2][WB1 + t1], s[3][WB1 + t1], PL.B[t1], PR.B[t1]);
-    plm(s[0][WB1 + t2], s[1][WB1 + t2], s[2][WB1 + t2], s[3][WB1 + t2], PL.B[t2], PR.B[t2]);
+    for (int e = 0; e < 3; e++) {
+        Rt x[4];
+        ld(WU1 + e, x);
+        plm(x[0], x[1], x[2], x[3], PL.u[e], PR.u[e]);
+    }
+    {
+        Rt x[4];
+        ld(WB1 + t1, x);
+        plm(x[0], x[1], x[2], x[3], PL.B[t1], PR.B[t1]);
+        ld(WB1 + t2, x);
+        plm(x[0], x[1], x[2], x[3], PL.B[t2], PR.B[t2]);
+    }
     const Rt bn = m.sg > Rt(0) ? (Rt)Bf[((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k)] / m.sg : Rt(0);
     PL.B[d] = bn;
     PR.B[d] = bn;
     const Rt gam = (Rt)R.gam;
-    pt_fill(m, gam, PL);
-    pt_fill(m, gam, PR);
+    Side<Rt> SL, SR;
+    face_side(m, gam, PL, d, SL);
+    face_side(m, gam, PR, d, SR);
     Rt out[NFW];
-    hll(m, gam, PL, PR, d, out);
+    hll(m, SL, SR, out);
     Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)d * NFW * P.nf + fidx(d, i, j, k);
 #pragma unroll
     for (int q = 0; q < NFW; q++) F[(size_t)q * P.nf] = out[q];
+}
+template <class Rt>
+__device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, int j, int k) {
+    if (d == 0) flux_face_d<Rt, 0>(P, Bf, b, i, j, k);
+    else if (d == 1) flux_face_d<Rt, 1>(P, Bf, b, i, j, k);
+    else flux_face_d<Rt, 2>(P, Bf, b, i, j, k);
 }
 
 template <class Rt>
@@ -515,8 +555,11 @@ __global__ void k_begin(RunCtr* ctr, const int* runs, int nruns, unsigned int* q
     if (t < 4) qctr[t] = 0;
 }
 
-__global__ __launch_bounds__(TPB) void k_c2p(KPS P, int want_dt) {
-    phase_a_block<TPB>(P, P.list[blockIdx.x], want_dt != 0);
+// Launch variants the autotuner chooses from: workgroup size TB and a
+// minimum of MW wavefronts per SIMD (MW > 1 caps the registers per thread).
+template <int TB, int MW>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_c2p(KPS P, int want_dt) {
+    phase_a_block<TB>(P, P.list[blockIdx.x], want_dt != 0);
 }
 
 __global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
@@ -540,11 +583,11 @@ __global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
     ghost_face(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
 }
 
-template <class Rt>
-__global__ __launch_bounds__(TPB) void k_flux(KPS P, int buf) {
-    const int per = (P.nflf + TPB - 1) / TPB;
+template <class Rt, int TB, int MW>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_flux(KPS P, int buf) {
+    const int per = (P.nflf + TB - 1) / TB;
     const int b = P.list[blockIdx.x / per];
-    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    const int t = (blockIdx.x % per) * TB + threadIdx.x;
     if (t >= P.nflf) return;
     int i, j, k;
     const int x = P.flf[t];
@@ -564,11 +607,11 @@ __global__ __launch_bounds__(TPB) void k_face(KPS P, int stage) {
     face_update<Rt>(P, b, x >> 24, i, j, k, run_dt(P, b), stage);
 }
 
-template <class Rt>
-__global__ __launch_bounds__(TPB) void k_update(KPS P, int stage) {
-    const int per = (P.nint + TPB - 1) / TPB;
+template <class Rt, int TB, int MW>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_update(KPS P, int stage) {
+    const int per = (P.nint + TB - 1) / TB;
     const int b = P.list[blockIdx.x / per];
-    const int a = (blockIdx.x % per) * TPB + threadIdx.x;
+    const int a = (blockIdx.x % per) * TB + threadIdx.x;
     if (a >= P.nint) return;
     cell_update<Rt>(P, b, a, run_dt(P, b), stage);
 }
@@ -877,6 +920,40 @@ __global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
     if (threadIdx.x < 4) out[(size_t)blockIdx.x * 4 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// ---------------------------------------------------- launch variants
+constexpr int NVAR = 6;
+constexpr int VAR_TB[NVAR] = {256, 256, 128, 128, 64, 64};
+constexpr int VAR_MW[NVAR] = {1, 4, 1, 4, 1, 4};
+
+#define SPH_VARIANTS(M) M(0, 256, 1) M(1, 256, 4) M(2, 128, 1) M(3, 128, 4) M(4, 64, 1) M(5, 64, 4)
+
+template <class Rt>
+void launch_flux(int v, int nl, int nitems, hipStream_t st, KPS P, int buf) {
+    switch (v) {
+#define L_(i, tb, mw) \
+    case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+        SPH_VARIANTS(L_)
+#undef L_
+    }
+}
+template <class Rt>
+void launch_update(int v, int nl, int nitems, hipStream_t st, KPS P, int stage) {
+    switch (v) {
+#define L_(i, tb, mw) \
+    case i: hipLaunchKernelGGL((k_update<Rt, tb, mw>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, stage); break;
+        SPH_VARIANTS(L_)
+#undef L_
+    }
+}
+inline void launch_c2p(int v, int nl, hipStream_t st, KPS P, int want_dt) {
+    switch (v) {
+#define L_(i, tb, mw) \
+    case i: hipLaunchKernelGGL((k_c2p<tb, mw>), dim3(nl), dim3(tb), 0, st, P, want_dt); break;
+        SPH_VARIANTS(L_)
+#undef L_
+    }
+}
+
 // ---------------------------------------------------- host side
 template <class T>
 struct DB {
@@ -942,6 +1019,7 @@ public:
     bool mixed = false;
     int mode = 0;                 // 0 multi-kernel, 1 persistent
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
+    int var_flux = 0, var_update = 0, var_c2p = 0;  // launch variants (VAR_TB, VAR_MW)
     bool tables_dirty = true;
     std::string err;
     // graph of the step for the current active set
@@ -1249,25 +1327,84 @@ public:
         return std::max(1, std::min(ncu * per, (int)act_slots.size()));
     }
 
+    // Per-kernel timing with HIP events on the engine's stream (eager
+    // launches only): every launch, or the launches of one kernel name.
+    bool profile = false;
+    std::string prof_only;
+    struct PendEv {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<PendEv> pend;
+    std::vector<hipEvent_t> evpool;
+    std::map<std::string, std::pair<long long, double>> prof_acc;
+    hipEvent_t ev_get() {
+        if (!evpool.empty()) {
+            hipEvent_t e = evpool.back();
+            evpool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        SCHK(hipEventCreate(&e));
+        return e;
+    }
+    template <class Fn>
+    void tm(const char* name, Fn&& fn) {
+        if (!profile || (!prof_only.empty() && prof_only != name)) {
+            fn();
+            return;
+        }
+        PendEv p{name, ev_get(), ev_get()};
+        SCHK(hipEventRecord(p.a, stream));
+        fn();
+        SCHK(hipEventRecord(p.b, stream));
+        pend.push_back(p);
+    }
+    void prof_collect() {
+        for (auto& p : pend) {
+            float ms = 0.f;
+            SCHK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto& acc = prof_acc[p.name];
+            acc.first++;
+            acc.second += ms;
+            evpool.push_back(p.a);
+            evpool.push_back(p.b);
+        }
+        pend.clear();
+    }
+
     template <class Rt>
     void enqueue_step() {
         const int nl = (int)act_slots.size();
         const int nr = (int)act_runs.size();
         KPS P = kp(list.p, nl);
-        hipLaunchKernelGGL(k_begin, dim3((std::max(nr, 4) + 63) / 64), dim3(64), 0, stream, ctr.p, runlist.p, nr, qctr.p);
+        tm("k_begin", [&] {
+            hipLaunchKernelGGL(k_begin, dim3((std::max(nr, 4) + 63) / 64), dim3(64), 0, stream, ctr.p, runlist.p, nr,
+                               qctr.p);
+        });
         for (int stage = 0; stage < 2; stage++) {
             if (mode == 1) {
-                if (stage == 1) hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, stream, ctr.p, runlist.p, 0, qctr.p);
-                hipLaunchKernelGGL(k_stage<Rt>, dim3(persistent_grid()), dim3(TPB), 0, stream, P, stage);
+                if (stage == 1)
+                    tm("k_begin", [&] {
+                        hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, stream, ctr.p, runlist.p, 0, qctr.p);
+                    });
+                const int ng = persistent_grid();
+                tm("k_stage", [&] { hipLaunchKernelGGL(k_stage<Rt>, dim3(ng), dim3(TPB), 0, stream, P, stage); });
                 continue;
             }
-            hipLaunchKernelGGL(k_c2p, dim3(nl), dim3(TPB), 0, stream, P, stage == 0 ? 1 : 0);
+            tm("k_c2p", [&] { launch_c2p(var_c2p, nl, stream, P, stage == 0 ? 1 : 0); });
             const int buf = stage;
-            hipLaunchKernelGGL(k_ghost_cells, dim3(nl * ((nghc + TPB - 1) / TPB)), dim3(TPB), 0, stream, P);
-            hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
-            hipLaunchKernelGGL(k_flux<Rt>, dim3(nl * ((nflf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
-            hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
-            hipLaunchKernelGGL(k_update<Rt>, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+            tm("k_ghost_cells", [&] {
+                hipLaunchKernelGGL(k_ghost_cells, dim3(nl * ((nghc + TPB - 1) / TPB)), dim3(TPB), 0, stream, P);
+            });
+            tm("k_ghost_faces", [&] {
+                hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
+            });
+            tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, stream, P, buf); });
+            tm("k_face", [&] {
+                hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+            });
+            tm("k_update", [&] { launch_update<Rt>(var_update, nl, L.nint, stream, P, stage); });
         }
         SCHK(hipGetLastError());
     }
@@ -1275,10 +1412,13 @@ public:
         if (mixed) enqueue_step<float>();
         else enqueue_step<double>();
     }
-    int key() const { return (mixed ? 1 : 0) | (mode << 1) | (wg_per_cu << 3); }
+    int key() const {
+        return (mixed ? 1 : 0) | (mode << 1) | (wg_per_cu << 3) | (var_flux << 9) | (var_update << 12) |
+               (var_c2p << 15);
+    }
     void step() {
         if (act_slots.empty()) throw std::invalid_argument("no active runs");
-        if (!use_graph) {
+        if (!use_graph || profile) {
             enqueue();
             return;
         }
@@ -1295,6 +1435,7 @@ public:
     void sync() {
         SCHK(hipGetLastError());
         SCHK(hipStreamSynchronize(stream));
+        if (!pend.empty()) prof_collect();
     }
     void check_barrier() {
         if (mode != 1) return;
@@ -1313,7 +1454,7 @@ public:
     void c2p_run(int r) {
         set_active({r});
         KPS P = kp(list.p, (int)act_slots.size());
-        hipLaunchKernelGGL(k_c2p, dim3(act_slots.size()), dim3(TPB), 0, stream, P, 0);
+        launch_c2p(var_c2p, (int)act_slots.size(), stream, P, 0);
         sync();
     }
 
@@ -1524,6 +1665,22 @@ int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
         return BHS_OK;
     });
 }
+int bhs_set_variants(bhs_engine* h, int flux, int update, int c2p) {
+    return guard(h, [&] {
+        for (int v : {flux, update, c2p})
+            if (v < 0 || v >= NVAR) throw std::invalid_argument("launch variant out of range");
+        h->e->var_flux = flux;
+        h->e->var_update = update;
+        h->e->var_c2p = c2p;
+        return BHS_OK;
+    });
+}
+int bhs_variant_info(int v, int* tpb, int* min_waves) {
+    if (v < 0 || v >= NVAR || !tpb || !min_waves) return BHS_EINVAL;
+    *tpb = VAR_TB[v];
+    *min_waves = VAR_MW[v];
+    return BHS_OK;
+}
 int bhs_use_graph(bhs_engine* h, int enable) {
     return guard(h, [&] {
         h->e->use_graph = enable != 0;
@@ -1550,6 +1707,57 @@ int bhs_steps(bhs_engine* h, int nruns, const int* runs, int nsteps, double* dt_
         E.sync();
         E.check_barrier();
         return BHS_OK;
+    });
+}
+int bhs_steps_timed(bhs_engine* h, int nruns, const int* runs, int nsteps, double* ms_per_step) {
+    return guard(h, [&] {
+        if (nruns <= 0 || !runs || nsteps < 0 || !ms_per_step) throw std::invalid_argument("bad arguments");
+        Engine& E = *h->e;
+        E.set_active(std::vector<int>(runs, runs + nruns));
+        std::vector<hipEvent_t> ev(nsteps + 1);
+        for (auto& e : ev) SCHK(hipEventCreate(&e));
+        SCHK(hipEventRecord(ev[0], E.stream));
+        for (int s = 0; s < nsteps; s++) {
+            E.step();
+            SCHK(hipEventRecord(ev[s + 1], E.stream));
+        }
+        E.sync();
+        for (int s = 0; s < nsteps; s++) {
+            float ms = 0.f;
+            SCHK(hipEventElapsedTime(&ms, ev[s], ev[s + 1]));
+            ms_per_step[s] = ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        E.check_barrier();
+        return BHS_OK;
+    });
+}
+int bhs_profile(bhs_engine* h, int enable, const char* only) {
+    return guard(h, [&] {
+        Engine& E = *h->e;
+        E.sync();
+        E.profile = enable != 0;
+        E.prof_only = only ? only : "";
+        E.prof_acc.clear();
+        E.graph_release();
+        return BHS_OK;
+    });
+}
+int bhs_profile_report(bhs_engine* h, char* buf, int len) {
+    return guard(h, [&] {
+        Engine& E = *h->e;
+        E.sync();
+        std::string out;
+        char line[256];
+        for (auto& kv : E.prof_acc) {
+            std::snprintf(line, sizeof line, "%s %lld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+            out += line;
+        }
+        if (buf && len > 0) {
+            std::strncpy(buf, out.c_str(), (size_t)len - 1);
+            buf[len - 1] = 0;
+        }
+        return (int)out.size();
     });
 }
 int bhs_c2p(bhs_engine* h, int run) {

@@ -84,7 +84,7 @@ SPH_HD R rabs(R a) { return a < R(0) ? -a : a; }
 template <class R>
 struct Pt {
     R rho, p, u[3], B[3];   // inputs (u = W v^i)
-    R W, v[3], vl[3], Bl[3];
+    R W, iW, v[3], vl[3], Bl[3];
     R Bv, B2, b2, rhoh, ps;  // B.v, B^2, b^2, rho h, p + b^2/2
 };
 
@@ -95,6 +95,7 @@ SPH_HD void pt_fill(const Met<R>& m, R gam, Pt<R>& s) {
     const R uu = dot3(ul, s.u);
     s.W = sqrt(R(1) + uu);
     const R iW = R(1) / s.W;
+    s.iW = iW;
 #pragma unroll
     for (int d = 0; d < 3; d++) {
         s.v[d] = s.u[d] * iW;
@@ -111,12 +112,11 @@ SPH_HD void pt_fill(const Met<R>& m, R gam, Pt<R>& s) {
 // conserved variables, undensitised: D, S_1..3, tau (the field is B itself)
 template <class R>
 SPH_HD void pt_cons(const Pt<R>& s, R c[5]) {
-    const R W2 = s.W * s.W;
-    const R Z = s.rhoh * W2;
+    const R Z = s.rhoh * s.W * s.W;
     c[0] = s.rho * s.W;
 #pragma unroll
     for (int d = 0; d < 3; d++) c[1 + d] = (Z + s.B2) * s.vl[d] - s.Bv * s.Bl[d];
-    c[4] = Z + s.B2 - s.p - R(0.5) * (s.B2 / W2 + s.Bv * s.Bv) - c[0];
+    c[4] = Z + s.B2 - s.p - R(0.5) * (s.B2 * s.iW * s.iW + s.Bv * s.Bv) - c[0];
 }
 
 // Flux in direction n, undensitised, 8 words (D, S_1..3, tau, B^1..3); the
@@ -127,17 +127,17 @@ SPH_HD void pt_flux(const Met<R>& m, const Pt<R>& s, const R c[5], int n, R f[8]
 #pragma unroll
     for (int d = 0; d < 3; d++) V[d] = m.alpha * s.v[d] - m.beta[d];
     const R Vn = V[n];
-    const R iW = R(1) / s.W;
     const R ab0 = s.W * s.Bv;  // alpha b^0
     const R Bn = s.B[n];
+    const R aBn = m.alpha * Bn * s.iW;
     f[0] = c[0] * Vn;
 #pragma unroll
     for (int j = 0; j < 3; j++) {
-        const R bj = s.Bl[j] * iW + ab0 * s.vl[j];  // b_j
-        f[1 + j] = c[1 + j] * Vn - m.alpha * bj * Bn * iW;
+        const R bj = s.Bl[j] * s.iW + ab0 * s.vl[j];  // b_j
+        f[1 + j] = c[1 + j] * Vn - bj * aBn;
     }
     f[1 + n] += m.alpha * s.ps;
-    f[4] = c[4] * Vn + m.alpha * (s.ps * s.v[n] - ab0 * Bn * iW);
+    f[4] = c[4] * Vn + m.alpha * s.ps * s.v[n] - ab0 * aBn;
 #pragma unroll
     for (int k = 0; k < 3; k++) f[5 + k] = s.B[k] * Vn - Bn * V[k];
     f[5 + n] = R(0);
@@ -145,14 +145,14 @@ SPH_HD void pt_flux(const Met<R>& m, const Pt<R>& s, const R c[5], int n, R f[8]
 
 // Fast magnetosonic signal speeds in direction n (coordinate speeds dx^n/dt):
 // the sound-speed formula of Banyuls et al. 1997 with c_s replaced by the
-// fast speed c_f^2 = c_s^2 + v_A^2 - c_s^2 v_A^2 (hlld.cpp:34-39).
+// fast speed (hlld.cpp:34-39), c_f^2 = c_s^2 + v_A^2 - c_s^2 v_A^2, which
+// for c_s^2 = gam p / (rho h) and v_A^2 = b^2 / (rho h + b^2) is
+// (gam p + b^2) / (rho h + b^2).
 template <class R>
 SPH_HD void pt_speeds(const Met<R>& m, const Pt<R>& s, R gam, int n, R& lm, R& lp) {
-    const R cs2 = gam * s.p / s.rhoh;
-    const R va2 = s.b2 / (s.rhoh + s.b2);
-    R cf2 = cs2 + va2 - cs2 * va2;
+    R cf2 = (gam * s.p + s.b2) / (s.rhoh + s.b2);
     cf2 = rmin(cf2, R(1));
-    const R v2 = R(1) - R(1) / (s.W * s.W);
+    const R v2 = R(1) - s.iW * s.iW;
     const R ginn = n == 0 ? m.gi[0] : n == 1 ? m.gi[3] : m.gi[5];
     const R vn = s.v[n];
     const R den = R(1) - v2 * cf2;
@@ -160,38 +160,46 @@ SPH_HD void pt_speeds(const Met<R>& m, const Pt<R>& s, R gam, int n, R& lm, R& l
     disc = rmax(disc, R(0));
     const R sq = sqrt(cf2 * disc);
     const R a = vn * (R(1) - cf2);
-    lp = m.alpha * (a + sq) / den - m.beta[n];
-    lm = m.alpha * (a - sq) / den - m.beta[n];
+    const R ad = m.alpha / den;
+    lp = (a + sq) * ad - m.beta[n];
+    lm = (a - sq) * ad - m.beta[n];
 }
 
-// HLL flux between face states L and R in direction n, densitised by the
-// face's sqrt(gamma); out[NFW] = D, S_1..3, tau, then the two transverse
-// field fluxes in cyclic order after n.
+// One side of a face: the seven conserved words the HLL flux mixes (D,
+// S_1..3, tau and the two transverse fields in cyclic order after n), their
+// fluxes and the signal speeds.
 template <class R>
-SPH_HD void hll(const Met<R>& m, R gam, const Pt<R>& L, const Pt<R>& Rt, int n, R out[NFW]) {
-    R cL[5], cR[5], fL[8], fR[8];
-    pt_cons(L, cL);
-    pt_cons(Rt, cR);
-    pt_flux(m, L, cL, n, fL);
-    pt_flux(m, Rt, cR, n, fR);
-    R lmL, lpL, lmR, lpR;
-    pt_speeds(m, L, gam, n, lmL, lpL);
-    pt_speeds(m, Rt, gam, n, lmR, lpR);
-    const R sl = rmin(rmin(lmL, lmR), R(0));
-    const R sr = rmax(rmax(lpL, lpR), R(0));
-    const R den = sr - sl;
-    const R id = den > R(0) ? R(1) / den : R(0);
-    const R slsr = sl * sr;
+struct Side {
+    R u[7], f[7], lm, lp;
+};
+template <class R>
+SPH_HD void face_side(const Met<R>& m, R gam, Pt<R>& s, int n, Side<R>& o) {
+    pt_fill(m, gam, s);
+    R c[5], f[8];
+    pt_cons(s, c);
+    pt_flux(m, s, c, n, f);
+    pt_speeds(m, s, gam, n, o.lm, o.lp);
     const int t1 = n == 2 ? 0 : n + 1, t2 = n == 0 ? 2 : n - 1;
-    R uL[7], uR[7], FL[7], FR[7];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        uL[q] = cL[q]; uR[q] = cR[q]; FL[q] = fL[q]; FR[q] = fR[q];
+        o.u[q] = c[q];
+        o.f[q] = f[q];
     }
-    uL[5] = L.B[t1]; uR[5] = Rt.B[t1]; FL[5] = fL[5 + t1]; FR[5] = fR[5 + t1];
-    uL[6] = L.B[t2]; uR[6] = Rt.B[t2]; FL[6] = fL[5 + t2]; FR[6] = fR[5 + t2];
+    o.u[5] = s.B[t1]; o.f[5] = f[5 + t1];
+    o.u[6] = s.B[t2]; o.f[6] = f[5 + t2];
+}
+
+// HLL flux between face sides L and R, densitised by the face's
+// sqrt(gamma); out[NFW] in the order of Side.
+template <class R>
+SPH_HD void hll(const Met<R>& m, const Side<R>& L, const Side<R>& Rt, R out[NFW]) {
+    const R sl = rmin(rmin(L.lm, Rt.lm), R(0));
+    const R sr = rmax(rmax(L.lp, Rt.lp), R(0));
+    const R den = sr - sl;
+    const R id = den > R(0) ? m.sg / den : R(0);
+    const R slsr = sl * sr;
 #pragma unroll
-    for (int q = 0; q < 7; q++) out[q] = m.sg * ((sr * FL[q] - sl * FR[q] + slsr * (uR[q] - uL[q])) * id);
+    for (int q = 0; q < 7; q++) out[q] = (sr * L.f[q] - sl * Rt.f[q] + slsr * (Rt.u[q] - L.u[q])) * id;
 }
 
 // Limited linear (minmod) reconstruction at the face between cells c-1 and

@@ -94,6 +94,15 @@ int bhs_run_set_state(bhs_engine* e, int run, const double* U, const double* W, 
  * runs together, each with its own CFL dt; dt_out[nsteps][nruns] (may be
  * NULL). */
 int bhs_steps(bhs_engine* e, int nruns, const int* runs, int nsteps, double* dt_out);
+/* bhs_steps with a HIP event recorded on the engine's stream between
+ * consecutive steps: ms_per_step[nsteps] = each step's device time (the
+ * steps replay the captured graph).  No reference counterpart. */
+int bhs_steps_timed(bhs_engine* e, int nruns, const int* runs, int nsteps, double* ms_per_step);
+/* Per-kernel timing with HIP events (measurement only; launches one by
+ * one): enable, optionally restricted to one kernel name; the report is
+ * "name launches total_ms" lines; returns the report's length. */
+int bhs_profile(bhs_engine* e, int enable, const char* only);
+int bhs_profile_report(bhs_engine* e, char* buf, int len);
 /* con2prim with floors of a run's active zones (W of the current U;
  * integrator.cpp:201-222). */
 int bhs_c2p(bhs_engine* e, int run);
@@ -111,6 +120,12 @@ int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
 int bhs_set_precision(bhs_engine* e, int mixed);
 int bhs_set_schedule(bhs_engine* e, int persistent, int wg_per_cu);
 int bhs_use_graph(bhs_engine* e, int enable);
+/* Launch variants of the flux, update and con2prim kernels (0 .. 5):
+ * workgroup size and minimum wavefronts per SIMD of each are given by
+ * bhs_variant_info.  Results do not depend on the variant; the autotuner
+ * (bhfluid.tune) times them. */
+int bhs_set_variants(bhs_engine* e, int flux, int update, int c2p);
+int bhs_variant_info(int variant, int* tpb, int* min_waves);
 int bhs_sync(bhs_engine* e);
 
 #ifdef __cplusplus

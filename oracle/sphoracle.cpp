@@ -345,10 +345,11 @@ void fluxes(Run& R, const std::vector<double>& Bf) {
                         PL.B[d] = bn;
                         PR.B[d] = bn;
                         const Rt gam = (Rt)R.gam;
-                        pt_fill(m, gam, PL);
-                        pt_fill(m, gam, PR);
+                        Side<Rt> SL, SR;
+                        face_side(m, gam, PL, d, SL);
+                        face_side(m, gam, PR, d, SR);
                         Rt out[NFW];
-                        hll(m, gam, PL, PR, d, out);
+                        hll(m, SL, SR, out);
                         for (int q = 0; q < NFW; q++)
                             R.F[(((size_t)b * 3 + d) * NFW + q) * L.nf + fidx(d, i, j, k)] = (double)out[q];
                     }
@@ -859,9 +860,10 @@ void os_hll_point(const double* met17, double gam, const double* wl, const doubl
     L.rho = wl[0]; L.p = (gam - 1.0) * wl[0] * wl[1];
     Rr.rho = wr[0]; Rr.p = (gam - 1.0) * wr[0] * wr[1];
     for (int d = 0; d < 3; d++) { L.u[d] = wl[2 + d]; L.B[d] = wl[5 + d]; Rr.u[d] = wr[2 + d]; Rr.B[d] = wr[5 + d]; }
-    pt_fill(m, gam, L);
-    pt_fill(m, gam, Rr);
-    hll(m, gam, L, Rr, n, out7);
+    Side<double> SL, SR;
+    face_side(m, gam, L, n, SL);
+    face_side(m, gam, Rr, n, SR);
+    hll(m, SL, SR, out7);
 }
 // source terms at a cell: gc holds the NGC words of one point
 void os_source_point(const double* gc, double gam, const double* w, double* out5) {
