@@ -1,21 +1,34 @@
 #!/usr/bin/env python3
-"""Benchmark: zone-updates/s of the FP64 GRMHD step on one MI355X.
+"""Benchmark: zone-updates/s of the GRMHD fluid step on one MI355X, FP64 and
+mixed precision, on the 3-level AMR Fishbone-Moncrief torus; serving p50/p99.
 
-Workload (the reference's own production set-up, src/fluid/main.cpp:61-168):
-Fishbone-Moncrief torus (a=0.9, Q=0.1, Gamma=5/3, r_in=4.5, r_max=12) with the
-vector-potential field, on [-50M, 50M]^3 tiled by 8^3 root patches of 16^3
-zones, after one regrid: 4544 patches, 4040 leaves on two levels
-(16.5 M leaf zones).  The arithmetic is the reference's (every quirk kept),
-in FP64.  A step is one rk2integrator::step (dtcomp, two RK stages, floors)
-over the whole tree.  W untimed warm-up steps, then the initial data is
-restored and K timed steps advance it.  Inputs are resident in HBM
-when the timed region starts.
+Workload (the AMR torus the metric is quoted on): a = 0.9375, Gamma = 13/9,
+r_in = 6, r_max = 12, A_phi = max(rho/rho_max - 0.2, 0) scaled to beta = 100,
++-4 % PCG64(42) pressure perturbation, floors rho = 1e-7 r^-1.5,
+eps = 1e-5 r^-1.5, CFL 0.4, on a log-r (1.2 - 40) x theta x phi grid of
+16^3-zone blocks with two ghost layers over three levels (level 0 of 128 x 64
+x 64 zones, refined where rho > 0.1 rho_max and 2:1 balanced: 1976 blocks,
+56 / 384 / 1536 per level, 8.09 M zones).  The physics is the reference's
+step (src/fluid/solver/integrator.cpp:262-275) in its self-consistent form
+(csrc/sph_physics.hpp).  A step is one RK2 step of the whole tree.
 
-Prints one JSON line.  Per-kernel durations come from HIP events recorded
-on the tree's stream over the timed region; the dominant kernel's roofline
-uses its algorithmic bytes per launch (see ALGO_BYTES_PER_BLOCK).  The CPU
-baseline is the oracle (the C++/OpenMP restatement of the reference) on a
-bounded sample of the same grid, on this host's cores.
+Passes, each from freshly computed initial data (the state is resident in
+HBM when timing starts):
+  1. autotuning of the launch configuration (bhfluid.tune), per precision;
+  2. timed: W warm-up steps, then K steps replayed from the captured HIP
+     graph with an event between consecutive steps; value = zones / median
+     step time (FP64; the mixed-precision figure beside it);
+  3. accuracy: the same steps_acc steps in FP64 and in mixed precision from
+     the same initial data; L1(rho), mass and energy drift, max |div B|;
+  4. per-kernel HIP-event breakdown (eager launches) and the dominant
+     kernel's roofline from its launches timed alone;
+  5. CPU baseline: the oracle (C++/OpenMP restatement) stepping the same tree
+     from the same initial data on this host's cores;
+  6. serving: requests with Poisson arrivals, grids 64^3 .. 192 x 96 x 96
+     (uniform or two levels), spin ~ U(0, 0.99), beta in {30, 100, 300},
+     100 steps each, continuous batching under an HBM block budget, mixed
+     precision; p50 / p99 latency and aggregate zone-updates/s.
+Prints one JSON line.
 """
 import argparse
 import json
@@ -30,67 +43,54 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT / "black-hole-simulator_amd"))
 sys.path.insert(0, str(ROOT / "tests"))
 
-METRIC = "zone-updates/s FP64 vs mixed, AMR torus @L1(\u03c1)\u22641e-5; p99 run latency"
+METRIC = "zone-updates/s FP64 vs mixed, AMR torus @L1(ρ)≤1e-5; p99 run latency"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
-# compulsory FP64 state traffic of one zone-update (8 evolved words read and
-# written per stage plus the stored hydro words): 37 words
+# compulsory state traffic of one zone-update (FP64 state in both precisions):
+# stage 1 reads U^n (8 words) and writes U* (8); stage 2 reads U* (8) and the
+# stored hydro words of U^n (5) and writes U^{n+1} (8): 37 words
 BYTES_PER_ZONE_UPDATE = 296
-
-DOM = 50.0
-NQLT = 8
-A_SPIN, Q_CHARGE, GAMMA, CFL = 0.9, 0.1, 5.0 / 3.0, 0.4
-
-# Algorithmic HBM bytes per block and launch: the state a kernel must read
-# and write at least once (FP64), excluding static geometry tables and
-# scratch that a fused kernel would keep on chip.
-#   k_flux   : W on the 7168 stencil cells (8 words) + 13056 normal face B
-#              in, 13056 faces x 8 flux words out
-#   k_c2p    : U in, W out on 8000 cells
-#   k_c2p_pre: U in, W out on 4096 cells
-#   k_ghost  : U, W of 3072 face-ghost sources in, U, W of 3904 ghost cells out
-#   k_update : U, W, RK store in (4096 x 21 words), U and store out (4096 x 13),
-#              fluxes 5 words on 3 x 4352 faces in
-#   k_bfupd  : 6 flux-B words on 3 x 4352 faces in (EMFs formed in place),
-#              3 x 5832 face B in/out
-#   k_f2cb   : 3 x 8400 face B in, 6859 cells x 6 words out
-#   k_floors : U, W in and out on 4096 cells
-#   k_dtcomp : 5 W words on 4096 cells in
-W8 = 8
-ALGO_BYTES_PER_BLOCK = {
-    "k_flux": (7168 * 8 + 13056 + 13056 * 8) * W8,
-    "k_c2p": (8000 * 16) * W8,
-    "k_c2p_pre": (4096 * 16) * W8,
-    "k_ghost": (3072 * 16 + 3904 * 16) * W8,
-    "k_update": (4096 * (21 + 13) + 3 * 4352 * 5) * W8,
-    "k_bfupd": (3 * 4352 * 6 + 2 * 3 * 5832) * W8,
-    "k_f2cb": (3 * 8400 + 6859 * 6) * W8,
-    "k_floors": (4096 * 32) * W8,
-    "k_dtcomp": (4096 * 5) * W8,
-    "k_logs": (8000 * 4) * W8,
-    "k_dtreduce": 0,
-    "k_brent": 0,
-    "k_qreset": 0,
-}
-# launches per step and the blocks each launch covers ("leaf" or "all")
-KERNEL_SCOPE = {"k_flux": "all", "k_bfupd": "all", "k_f2cb": "all", "k_update": "all",
-                "k_floors": "all", "k_c2p": "leaf", "k_ghost": "leaf", "k_dtcomp": "leaf", "k_logs": "all",
-                "k_dtreduce": "leaf", "k_c2p_pre": "leaf", "k_brent": "leaf", "k_qreset": "leaf"}
-# HBM bytes per dispatch measured with rocprofv3 (FETCH_SIZE and WRITE_SIZE,
-# each scaled by what the counter reads for a copy of known size)
-TRAFFIC_TABLE = ROOT / "profiles" / "traffic_main_torus.md"
+STAGE_BYTES = (16 * 8, 21 * 8)  # per zone-update, stage 1 and stage 2
+PROFILES = ROOT / "profiles"
 
 
-FP64_TABLE = ROOT / "profiles" / "fp64_main_torus.md"
+def c3_setup(sph):
+    g = sph.c3_grid()
+    blocks = sph.amr_blocks(g, 3)
+    return g, blocks
 
 
-def measured_step_flops():
-    """Executed FP64 flops of one step from the committed counter summary."""
-    if not FP64_TABLE.exists():
+def make_engine(sph, g, blocks, mixed):
+    eng = sph.Engine(16, len(blocks))
+    eng.set_precision(mixed)
+    run = eng.add_run(sph.PHYSICS, g, blocks)
+    run.init_torus(sph.TORUS, 42)
+    return eng, run
+
+
+def measured_traffic(kernel, precision):
+    """Copy-calibrated HBM bytes per dispatch of ``kernel`` from the committed
+    counter summary (profiles/traffic_c3_amr_torus.md), or None."""
+    f = PROFILES / "traffic_c3_amr_torus.md"
+    if not f.exists():
         return None
-    for line in FP64_TABLE.read_text().splitlines():
+    for line in f.read_text().splitlines():
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) == 7 and cols[0] == "**step**":
+        if len(cols) >= 6 and cols[0] == kernel and cols[1] == precision:
+            try:
+                return float(cols[5]) * 1e9
+            except ValueError:
+                return None
+    return None
+
+
+def measured_flops(precision):
+    f = PROFILES / "flops_c3_amr_torus.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 3 and cols[0] == "step" and cols[1] == precision:
             try:
                 return float(cols[2])
             except ValueError:
@@ -98,209 +98,170 @@ def measured_step_flops():
     return None
 
 
-def measured_traffic(kernel):
-    """Scaled HBM bytes per dispatch of `kernel` from the committed counter
-    summary, or None when the summary has no row for it."""
-    if not TRAFFIC_TABLE.exists():
-        return None
-    for line in TRAFFIC_TABLE.read_text().splitlines():
-        cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) == 7 and cols[0] == kernel:
-            try:
-                return float(cols[6]) * 1e9
-            except ValueError:
-                return None
-    return None
+def timed(sph, tune, g, blocks, mixed, steps, warmup, tune_log):
+    eng, run = make_engine(sph, g, blocks, mixed)
+    t = tune.tune(eng, [run], mixed, use_cache=False, log=tune_log)
+    # fresh initial data for the timed pass
+    run.init_torus(sph.TORUS, 42)
+    eng.step([run], warmup, want_dt=False)
+    ms = eng.step_timed([run], steps)
+    return eng, run, ms, t["config"]
 
 
-def build_tree(dom=DOM, nqlt=NQLT, regrid=True):
-    import bhfluid
+def accuracy(sph, g, blocks, nsteps, cfg64, cfgmx):
+    from bhfluid import tune
+    out = {}
+    states = {}
+    for name, mixed, cfg in (("fp64", False, cfg64), ("mixed", True, cfgmx)):
+        eng, run = make_engine(sph, g, blocks, mixed)
+        tune.apply(eng, cfg)
+        d0 = run.diagnostics()
+        dts = eng.step([run], nsteps)[:, 0]
+        run.c2p()
+        d1 = run.diagnostics()
+        U, W, Bf = run.state()
+        rho = run.active(W)[:, 0].copy()
+        states[name] = dict(rho=rho, d0=d0, d1=d1, t=float(dts.sum()), finite=bool(np.isfinite(run.active(W)).all()))
+        del eng, run, U, W, Bf
+    # L1 of rho weighted by each zone's coordinate volume, relative to the
+    # FP64 run's
+    w = sph.zone_volumes(g, blocks)
+    a, b = states["mixed"]["rho"], states["fp64"]["rho"]
+    l1 = float((np.abs(a - b) * w).sum() / (np.abs(b) * w).sum())
+    f, m = states["fp64"], states["mixed"]
+    out.update(steps=nsteps, L1_rho=l1,
+               mass_drift=(m["d1"]["mass"] - f["d1"]["mass"]) / f["d1"]["mass"],
+               energy_drift=(m["d1"]["energy"] - f["d1"]["energy"]) / f["d1"]["energy"],
+               fp64_mass_change=(f["d1"]["mass"] - f["d0"]["mass"]) / f["d0"]["mass"],
+               fp64_energy_change=(f["d1"]["energy"] - f["d0"]["energy"]) / f["d0"]["energy"],
+               divb_rel_fp64=f["d1"]["divb_max"] / f["d1"]["divb_scale"],
+               divb_rel_mixed=m["d1"]["divb_max"] / m["d1"]["divb_scale"],
+               t_fp64=f["t"], t_mixed=m["t"], finite=f["finite"] and m["finite"],
+               c2p_failures_fp64=f["d1"]["c2p_fail"], c2p_failures_mixed=m["d1"]["c2p_fail"])
+    return out
 
-    t = bhfluid.AmrTree((-dom,) * 3, (dom,) * 3, nqlt, 1.0, A_SPIN, Q_CHARGE, GAMMA)
-    t0 = time.perf_counter()
-    bhfluid.torus_setup(t, regrid=regrid)
-    t.host_setup_s = time.perf_counter() - t0
-    return t
+
+def kernel_breakdown(sph, g, blocks, mixed, cfg, steps):
+    """Per-kernel HIP-event times (eager launches), then the dominant kernel
+    timed alone (events around its launches only)."""
+    from bhfluid import tune
+    eng, run = make_engine(sph, g, blocks, mixed)
+    tune.apply(eng, cfg)
+    eng.profile(True)
+    eng.step([run], steps, want_dt=False)
+    rep = eng.profile_report()
+    dom = max((k for k in rep if k != "k_begin"), key=lambda k: rep[k]["total_ms"])
+    eng.profile(True, only=dom)
+    eng.step([run], steps, want_dt=False)
+    alone = eng.profile_report()[dom]
+    eng.profile(False)
+    return rep, dom, alone
 
 
-def device_setup(nqlt):
-    """main.cpp:94-168 with the state on the GPU throughout (initial data and
-    regrid on the device) on a fresh tree of the bench grid; seconds of each
-    part (the first includes building the roots' geometry tables)."""
-    import bhfluid
-
-    t = bhfluid.AmrTree((-DOM,) * 3, (DOM,) * 3, nqlt, 1.0, A_SPIN, Q_CHARGE, GAMMA)
-    t0 = time.perf_counter()
-    t.init_torus_device()
-    bhfluid.device_sync()
-    t1 = time.perf_counter()
-    t.regrid_device()
-    bhfluid.device_sync()
-    t2 = time.perf_counter()
-    nb = t.nblocks
-    del t
-    return {"init_torus_device_s": round(t1 - t0, 3), "regrid_device_s": round(t2 - t1, 3), "patches": nb}
-
-
-def cpu_baseline():
-    """Oracle step on the central [-25M, 25M]^3 of the same grid (4^3 roots of
-    the same size, one regrid), one step, all host threads."""
-    import bhfluid
-    from oracle_libs import OracleTree, State, oracle_lib
-
-    t0 = time.perf_counter()
-    t = build_tree(dom=25.0, nqlt=4, regrid=True)
-    meta, edges = t.blocks()
-    U, W, Bf = t.state()
-    params = (1.0, A_SPIN, Q_CHARGE, GAMMA)
-    ot = OracleTree.from_state(State(meta, edges, U, W, Bf), params, libm=0)
+def cpu_baseline(sph, g, blocks, nsteps=2):
+    """The oracle stepping the C3 tree from the device's initial data (copied
+    to the host), FP64, all host threads; median over steps."""
+    from sph_oracle import OracleRun
+    eng, run = make_engine(sph, g, blocks, False)
+    U, W, Bf = run.state()
+    del eng, run
+    p = sph.PHYSICS
+    cfg = dict(bz=16, a=p["a"], Q=p["Q"], gamma=p["gamma"], cfl=p["cfl"], rho_fl=p["rho_floor"],
+               eps_fl=p["eps_floor"], sigma_max=p["sigma_max"], w_max=p["gamma_max"], nr=g["nr"], nth=g["nth"],
+               nph=g["nph"], r_in=g["r_in"], r_out=g["r_out"], nlev=g["nlevels"])
+    orc = OracleRun(cfg, blocks)
+    orc.set_state(U, W, Bf)
+    del U, W, Bf
+    times = []
+    for _ in range(nsteps):
+        t0 = time.perf_counter()
+        orc.step(False)
+        times.append(time.perf_counter() - t0)
+    med = float(np.median(times))
     threads = len(os.sched_getaffinity(0))
     omp = os.environ.get("OMP_NUM_THREADS")
     if omp:
         threads = min(threads, int(omp))
-    oracle_lib().or_set_threads(threads)
-    leaf_zones = int(meta[:, 4].sum()) * 4096
-    t1 = time.perf_counter()
-    ot.step(CFL)
-    el = time.perf_counter() - t1
-    return {
-        "value": leaf_zones / el,
-        "unit": "zone-updates/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": f"1 step of the oracle on the central [-25M,25M]^3 of the same grid (4^3 roots + 1 regrid, "
-                  f"{meta.shape[0]} patches, {leaf_zones} leaf zones), {el:.2f} s; setup {t1 - t0:.1f} s untimed",
-    }
+    zones = len(blocks) * 4096
+    return {"value": zones / med, "unit": "zone-updates/s", "cores": threads, "kind": "port",
+            "sample": f"{nsteps} FP64 steps of the oracle on the whole C3 tree ({len(blocks)} blocks, {zones} zones) "
+                      f"from the same initial data; median {med:.2f} s per step"}
 
 
-def step_roofline(value, ms_per_step, bench_grid):
-    """Whole-step bounds: compulsory state bytes per zone-update against HBM,
-    and the executed FP64 flops of one step (counter summary, bench grid only)
-    against the FP64 vector peak."""
-    gbs = value * BYTES_PER_ZONE_UPDATE / 1e9
-    out = {"bytes_per_zone_update": BYTES_PER_ZONE_UPDATE, "achieved_GBps": gbs, "hbm_frac": gbs / HBM_PEAK_GBS}
-    flops = measured_step_flops() if bench_grid else None
-    if flops:
-        tfs = flops / (ms_per_step * 1e-3) / 1e12
-        out.update({"fp64_flops_per_step": flops, "fp64_TFLOPs": tfs, "fp64_peak": FP64_PEAK_TFS,
-                    "fp64_frac": tfs / FP64_PEAK_TFS,
-                    "fp64_source": "rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 x 64 lanes (profiles/fp64_main_torus.md)"})
-    return out
+def serving(sph, nreq, rate, budget):
+    from bhfluid.serve import Server, make_requests
+    reqs = make_requests(nreq, rate)
+    srv = Server(budget, mixed=True)
+    s = srv.run(reqs)
+    s.update(rate_per_s=rate, budget_blocks=budget, steps_per_request=reqs[0].nsteps,
+             finite=all(np.isfinite(r.result["mass"]) for r in reqs),
+             precision="mixed (FP32 fluxes/EMFs, FP64 state)")
+    return s
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--accuracy-steps", type=int, default=500)
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=12.0)
+    ap.add_argument("--serve-budget", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-device-setup", action="store_true",
-                    help="skip timing the initial data and regrid on the device")
-    ap.add_argument("--nqlt", type=int, default=NQLT)
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
-                    help="before the timed region, run the traffic-counter calibration copy of MIB MiB")
+    ap.add_argument("--no-serve", action="store_true")
     args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(run with --gpus 1)", file=sys.stderr)
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the step runs on a single MI355X (every box here has one); "
+              "--gpus must be 1", file=sys.stderr)
         sys.exit(2)
 
-    import bhfluid
+    from bhfluid import sph, tune, device_info
 
-    copy = None
-    if args.copy_probe:
-        nbytes = args.copy_probe << 20
-        ms = bhfluid.copy_probe(nbytes)
-        copy = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms, "GBps_rw": 2 * nbytes / ms / 1e6}
+    g, blocks = c3_setup(sph)
+    zones = len(blocks) * 4096
+    per_level = [int((blocks[:, 0] == l).sum()) for l in range(3)]
+    tune_log = {"fp64": [], "mixed": []}
 
-    t_setup = time.time()
-    tree = build_tree(nqlt=args.nqlt)
-    meta, _ = tree.blocks()
-    nb = meta.shape[0]
-    nleaf = int(meta[:, 4].sum())
-    levels = np.bincount(meta[meta[:, 4] == 1, 0]).tolist()
-    leaf_zones = nleaf * 4096
-    tree.prepare()
-    setup_s = time.time() - t_setup
+    eng, run, ms64, cfg64 = timed(sph, tune, g, blocks, False, args.steps, args.warmup, tune_log["fp64"])
+    del eng, run
+    eng, run, msmx, cfgmx = timed(sph, tune, g, blocks, True, args.steps, args.warmup, tune_log["mixed"])
+    del eng, run
+    med64, medmx = float(np.median(ms64)), float(np.median(msmx))
+    value = zones / (med64 * 1e-3)
+    value_mx = zones / (medmx * 1e-3)
 
-    U0, W0, Bf0 = tree.state()
-    faults = 0
+    acc = accuracy(sph, g, blocks, args.accuracy_steps, cfg64, cfgmx)
 
-    def restart():
-        # every pass starts again from the initial data
-        tree.set_state(U0, W0, Bf0)
-        tree.prepare()
+    rep64, dom64, alone64 = kernel_breakdown(sph, g, blocks, False, cfg64, 4)
+    repmx, dommx, alonemx = kernel_breakdown(sph, g, blocks, True, cfgmx, 4)
+    # roofline of the dominant FP64 kernel: the per-stage share of the
+    # compulsory 296 B per zone-update (148 B: the mean of the two stages)
+    # over one launch, which covers every zone once
+    algo = (STAGE_BYTES[0] + STAGE_BYTES[1]) / 2 * zones
+    launches_per_step = max(1, round(rep64[dom64]["launches"] / 4))
+    ach = algo / (alone64["avg_ms"] * 1e-3) / 1e9
+    traffic = measured_traffic(dom64, "fp64")
+    roofline = {"bound": "hbm", "kernel": dom64, "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                "algo_bytes_per_launch": algo, "avg_launch_ms": alone64["avg_ms"],
+                "launches_per_step": launches_per_step,
+                "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, copy-calibrated "
+                                  "(profiles/traffic_c3_amr_torus.md)" if traffic else None}
+    flops = measured_flops("fp64")
+    step_roof = {"bytes_per_zone_update": BYTES_PER_ZONE_UPDATE,
+                 "achieved_GBps_fp64": value * BYTES_PER_ZONE_UPDATE / 1e9,
+                 "hbm_frac_fp64": value * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS,
+                 "achieved_GBps_mixed": value_mx * BYTES_PER_ZONE_UPDATE / 1e9,
+                 "hbm_frac_mixed": value_mx * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS}
+    if flops:
+        tfs = flops / (med64 * 1e-3) / 1e12
+        step_roof.update(fp64_flops_per_step=flops, fp64_TFLOPs=tfs, fp64_peak=FP64_PEAK_TFS,
+                         fp64_frac=tfs / FP64_PEAK_TFS,
+                         fp64_source="rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 (profiles/flops_c3_amr_torus.md)")
 
-    if args.warmup:
-        try:
-            bhfluid.run_steps(tree, args.warmup, CFL)
-        except bhfluid.DomainError:
-            pass
-        restart()
-    # per-kernel breakdown: the same steps with every kernel bracketed by HIP events
-    tree.profile(True)
-    try:
-        bhfluid.run_steps(tree, args.steps, CFL)
-    except bhfluid.DomainError:
-        pass
-    bhfluid.device_sync()
-    report = tree.profile_report()
-    tree.profile(False)
-    prof = {k: v for k, v in report.items() if not k.startswith("queue:")}
-    queued = {k[6:]: v[0] for k, v in report.items() if k.startswith("queue:")}
-    dom_name = max(prof, key=lambda k: prof[k][1])
-    restart()
-    del U0, W0, Bf0
-    # the timed steps: only the dominant kernel's launches carry events
-    tree.profile(True, only=dom_name)
-    bhfluid.device_sync()
-    t0 = time.perf_counter()
-    try:
-        dts = bhfluid.run_steps(tree, args.steps, CFL)
-    except bhfluid.DomainError as e:
-        faults += e.faults
-        dts = None
-    bhfluid.device_sync()
-    el = time.perf_counter() - t0
-    timed = tree.profile_report()
-    tree.profile(False)
-    # the analyser's state-only diagnostics on the final state (device
-    # reductions), timed separately from the steps
-    diag = tree.diagnostics()
-    t_d = time.perf_counter()
-    for _ in range(5):
-        diag = tree.diagnostics()
-    diag_ms = 1e3 * (time.perf_counter() - t_d) / 5
-    U, _, _ = tree.state()
-    leaf = meta[:, 4] == 1
-    Ui = U.reshape(nb, 8, 20, 20, 20)[leaf][:, :, 2:18, 2:18, 2:18]
-    nonfinite = int((~np.isfinite(Ui).all(axis=1)).sum())
-    del U, Ui
-    host_setup_s = tree.host_setup_s
-    del tree
-    dev_setup = device_setup(args.nqlt) if not args.no_device_setup else None
-
-    value = leaf_zones * args.steps / el
-    ms_per_step = 1e3 * el / args.steps
-    # dominant kernel (by total time in the breakdown pass), timed in the timed pass
-    n_launch, tot_ms = timed[dom_name]
-    avg_ms = tot_ms / n_launch
-    blocks = nb if KERNEL_SCOPE.get(dom_name, "leaf") == "all" else nleaf
-    if dom_name == "k_ghost":
-        blocks = nleaf / max(1, n_launch / (2 * args.steps))  # one launch per wave
-    algo = ALGO_BYTES_PER_BLOCK.get(dom_name, 0) * blocks
-    if dom_name == "k_brent":  # staged input (11 words) and bracket in, W out for each queued cell
-        algo = sum(queued.values()) / n_launch * 20 * W8
-    achieved = algo / (avg_ms * 1e-3) / 1e9
-    kernels = {k: {"launches": v[0], "total_ms": round(v[1], 3), "avg_ms": round(v[1] / v[0], 4)}
-               for k, v in sorted(prof.items(), key=lambda kv: -kv[1][1])}
-
-    cpu = None
-    if not args.no_cpu_baseline:
-        try:
-            cpu = cpu_baseline()
-        except Exception as e:  # reported, not fatal
-            cpu = {"error": repr(e)}
+    cpu = None if args.no_cpu_baseline else cpu_baseline(sph, g, blocks)
+    srv = None if args.no_serve else serving(sph, args.serve_requests, args.serve_rate, args.serve_budget)
 
     line = {
         "metric": METRIC,
@@ -309,54 +270,32 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": med64,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic",
+        "data": "synthetic: Fishbone-Moncrief torus initial data computed on the device (PCG64 seed 42 perturbation)",
         "config": {
-            "workload": f"main.cpp torus: FM torus a={A_SPIN} Q={Q_CHARGE} Gamma=5/3 on [-{DOM:g}M,{DOM:g}M]^3, "
-                        f"{args.nqlt}^3 roots + 1 regrid, reference arithmetic in FP64",
-            "patches": nb,
-            "leaves": nleaf,
-            "leaf_zones_per_level": [n * 4096 for n in levels],
-            "leaf_zones": leaf_zones,
-            "cfl": CFL,
-            "eos_faults_in_timed_steps": faults,
-            "nonfinite_leaf_zones_after": nonfinite,
-            "setup_s": round(setup_s, 1),
-            "kernels": kernels,
-            "kernels_source": "a separate pass of the same steps from the same initial data with every kernel "
-                              "bracketed by HIP events (events add ~4 % to the step); the timed pass carries "
-                              "events around the dominant kernel only",
-            "brent_queued_cells": queued,
-            "diagnostics_ms": round(diag_ms, 3),
-            "setup_host_torus_and_regrid_s": round(host_setup_s, 2),
-            "setup_on_device": dev_setup,
-            "diagnostics_after": {k: float(v) for k, v in diag.items()},
+            "workload": "3-level AMR FM torus (C3): a=0.9375, Gamma=13/9, r 1.2-40 log x theta x phi, level 0 "
+                        "128x64x64, 16^3 blocks + 2 ghosts refined where rho>0.1 rho_max, CFL 0.4, PLM minmod + HLL "
+                        "+ CT + RK2, consistent preset",
+            "blocks": int(len(blocks)), "blocks_per_level": per_level, "zones": zones,
+            "ms_per_step_mean": float(np.mean(ms64)), "ms_per_step_p90": float(np.percentile(ms64, 90)),
+            "timing": "median over K graph-replayed steps, HIP events between steps on the engine stream",
+            "launch_fp64": cfg64, "launch_mixed": cfgmx, "device": device_info(),
+            "kernels_fp64": rep64, "kernels_mixed": repmx,
+            "kernels_source": "a separate pass of 4 steps with eager launches bracketed by HIP events",
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": dom_name,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(dom_name),
-            "traffic_source": "rocprofv3 FETCH_SIZE/WRITE_SIZE per dispatch, copy-calibrated "
-                              "(profiles/traffic_main_torus.md)",
-            "algo_bytes_per_launch": algo,
-            "avg_launch_ms": avg_ms,
-        },
-        "step_roofline": step_roofline(value, ms_per_step, args.nqlt == NQLT),
+        "mixed": {"value": value_mx, "ms_per_step": medmx, "speedup_vs_fp64": value_mx / value,
+                  "dtype": "f32 reconstruction/HLL/EMF, f64 con2prim/sources/update/reductions",
+                  "dominant_kernel": dommx, "dominant_avg_launch_ms": alonemx["avg_ms"], **acc},
+        "roofline": roofline,
+        "step_roofline": step_roof,
         "cpu_baseline": cpu,
+        "serving": srv,
     }
-    if copy is not None:
-        line["copy_probe"] = copy
-    if dts is not None:
-        line["config"]["dt_first_last"] = [float(dts[0]), float(dts[-1])]
-    print(json.dumps(line), flush=True)
+    print(json.dumps(line))
 
 
 if __name__ == "__main__":

@@ -84,6 +84,7 @@ def lib():
         sig("bhs_steps_timed", _i, [_v, _i, _ip, _i, _dp])
         sig("bhs_set_variants", _i, [_v, _i, _i, _i])
         sig("bhs_variant_info", _i, [_i, _ip, _ip])
+        sig("bhs_set_xcd_remap", _i, [_v, _i])
         sig("bhs_profile", _i, [_v, _i, ctypes.c_char_p])
         sig("bhs_profile_report", _i, [_v, ctypes.c_char_p, _i])
         _sigs_done = True
@@ -160,6 +161,9 @@ class Engine:
 
     def set_variants(self, flux: int = 0, update: int = 0, c2p: int = 0):
         self._check(self._L.bhs_set_variants(self._h, int(flux), int(update), int(c2p)), "set_variants")
+
+    def set_xcd_remap(self, enable: bool = True):
+        self._check(self._L.bhs_set_xcd_remap(self._h, int(bool(enable))), "set_xcd_remap")
 
     @staticmethod
     def variant_info(v: int):
@@ -382,6 +386,21 @@ def c3_grid():
     rho_max; the level-0 grid is 128 x 64 x 64 zones (1976 blocks: 56, 384
     and 1536 per level, 8.09 M zones)."""
     return dict(nr=128, nth=64, nph=64, nlevels=3, r_in=R_IN, r_out=R_OUT)
+
+
+def zone_volumes(grid: dict, blocks: np.ndarray, bz: int = 16) -> np.ndarray:
+    """Coordinate volume dr dtheta dphi of every active zone, shaped like
+    ``Run.active(W)[:, q]``: [nb][bz][16][16] (phi, theta, r)."""
+    lnr0, lnr1 = math.log(grid["r_in"]), math.log(grid["r_out"])
+    out = np.empty((len(blocks), bz, BY, BX))
+    for b, (l, bi, bj, bk) in enumerate(blocks):
+        f = 1 << int(l)
+        nr, nth = grid["nr"] * f, grid["nth"] * f
+        nph = 1 if bz == 1 else grid["nph"] * f
+        i = np.arange(bi * BX, bi * BX + BX)
+        dr = np.exp(lnr0 + (i + 1) * (lnr1 - lnr0) / nr) - np.exp(lnr0 + i * (lnr1 - lnr0) / nr)
+        out[b] = dr[None, None, :] * (math.pi / nth) * (2 * math.pi / nph)
+    return out
 
 
 def zone_count(blocks: np.ndarray, bz: int) -> int:

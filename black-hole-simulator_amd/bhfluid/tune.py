@@ -1,0 +1,102 @@
+"""Autotuner of the spherical engine's launch configuration.
+
+The step's result does not depend on how it is launched (every schedule and
+variant is bit-identical), so the tuner only times.  It searches, one
+dimension at a time, the schedule (one kernel per sub-phase, or one
+persistent queue-driven kernel per RK stage with 1 or 2 workgroups per
+compute unit) and, for the sub-phase schedule, the workgroup size and
+occupancy hint of the flux, update and con2prim kernels; each candidate is
+timed over a few graph-replayed steps of the caller's own runs.  The choice
+is cached per (device, block shape, block count, precision) in a JSON file
+(``$BHFLUID_TUNE_CACHE``, default ``~/.cache/bhfluid/tune.json``).
+"""
+from __future__ import annotations
+
+import json
+import os
+from pathlib import Path
+from typing import Dict, List, Optional
+
+import numpy as np
+
+from . import sph
+
+DEFAULT = dict(persistent=0, wg_per_cu=0, flux=0, update=0, c2p=0, xcd=1)
+
+
+def _cache_path() -> Path:
+    p = os.environ.get("BHFLUID_TUNE_CACHE")
+    return Path(p) if p else Path.home() / ".cache" / "bhfluid" / "tune.json"
+
+
+def _key(eng: sph.Engine, runs, mixed: bool) -> str:
+    from . import device_info
+    nb = sum(r.nb for r in runs)
+    return f"{device_info().split(';')[0]}|bz{eng.bz}|nb{nb}|{'mixed' if mixed else 'fp64'}"
+
+
+def apply(eng: sph.Engine, cfg: dict):
+    eng.set_schedule(bool(cfg["persistent"]), int(cfg["wg_per_cu"]))
+    eng.set_variants(int(cfg["flux"]), int(cfg["update"]), int(cfg["c2p"]))
+    eng.set_xcd_remap(bool(cfg.get("xcd", 1)))
+
+
+def _time(eng, runs, cfg, steps: int) -> float:
+    apply(eng, cfg)
+    eng.step(runs, 2, want_dt=False)        # capture + warm
+    ms = eng.step_timed(runs, steps)
+    return float(np.median(ms))
+
+
+def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_cache: bool = True,
+         log: Optional[list] = None) -> Dict:
+    """Pick the fastest launch configuration for ``runs`` on ``eng`` (which
+    must already hold their initial data; their state advances while the
+    candidates are timed), apply it and return it with its timings."""
+    key = _key(eng, runs, mixed)
+    path = _cache_path()
+    cache = {}
+    if use_cache and path.exists():
+        try:
+            cache = json.loads(path.read_text())
+        except (OSError, ValueError):
+            cache = {}
+    if use_cache and key in cache:
+        cfg = cache[key]["config"]
+        apply(eng, cfg)
+        return dict(config=cfg, cached=True, timings=cache[key]["timings"])
+    timings = []
+    best = dict(DEFAULT)
+    best_ms = _time(eng, runs, best, steps)
+    timings.append((dict(best), best_ms))
+    for wg in (1, 2):
+        c = dict(best, persistent=1, wg_per_cu=wg)
+        t = _time(eng, runs, c, steps)
+        timings.append((c, t))
+        if t < best_ms:
+            best, best_ms = c, t
+    if not best["persistent"]:
+        c = dict(best, xcd=0)
+        t = _time(eng, runs, c, steps)
+        timings.append((c, t))
+        if t < best_ms:
+            best, best_ms = c, t
+        for dim in ("flux", "update", "c2p"):
+            for v in range(1, 12 if dim == "flux" else 6):
+                c = dict(best, **{dim: v})
+                t = _time(eng, runs, c, steps)
+                timings.append((c, t))
+                if t < best_ms:
+                    best, best_ms = c, t
+    apply(eng, best)
+    out = dict(config=best, cached=False, timings=[dict(cfg=c, ms=t) for c, t in timings])
+    if use_cache:
+        try:
+            path.parent.mkdir(parents=True, exist_ok=True)
+            cache[key] = dict(config=best, timings=out["timings"])
+            path.write_text(json.dumps(cache, indent=1))
+        except OSError:
+            pass
+    if log is not None:
+        log.extend(out["timings"])
+    return out

@@ -93,9 +93,26 @@ struct KPS {
     int nflf;
     const int* owf;       // own faces
     int nowf;
+    const int* flp;       // theta / phi face lines (bit 26 set), then r faces
+    int nflp;
     // persistent schedule
     unsigned int* qctr;   // [0] stage-A queue, [1] barrier arrivals, [2] stage-B queue, [3] timeout flag
+    int xcd;              // 1: renumber workgroups so that each XCD takes a contiguous range
 };
+
+// Workgroups are dispatched round-robin over the eight XCDs (workgroup x to
+// XCD x mod 8), each with its own L2.  With xcd = 1 the kernels that spread a
+// block over several workgroups use the logical index this returns, so that
+// the workgroups of one block run on one XCD and share its L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int wg_index(const KPS& P) {
+    const int x = blockIdx.x;
+    if (!P.xcd) return x;
+    const int n = gridDim.x;
+    const int q = n / NXCD, r = n % NXCD;
+    const int k = x % NXCD, loc = x / NXCD;
+    return k * q + (k < r ? k : r) + loc;
+}
 
 __device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
 
@@ -449,6 +466,105 @@ __device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, i
     else flux_face_d<Rt, 2>(P, Bf, b, i, j, k);
 }
 
+// All faces of direction D (1: theta, 2: phi) along one line of the block at
+// fixed transverse position (i, jk): faces 0 .. n_D of the line.  The
+// reconstruction is carried along the line -- per variable the two previous
+// cell values and the previous limited slope -- so that each cell is loaded
+// once; the arithmetic is that of flux_face_d face by face.
+template <class Rt, int D>
+__device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk) {
+    constexpr int d = D;
+    constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
+    constexpr int NV = 7;  // rho, p, u~^1..3, B^t1, B^t2
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const int n = d == 1 ? BY : P.bz;
+    const int j0 = d == 1 ? 0 : jk, k0 = d == 1 ? jk : 0;
+    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    const Rt gm1 = (Rt)(R.gam - 1.0);
+    const Rt gam = (Rt)R.gam;
+    auto cell = [&](int c, Rt w[NV]) {
+        const int cc = d == 1 ? cidx(i, c, k0) : cidx(i, j0, c);
+        const Rt rho = (Rt)Wb[(size_t)WRHO * P.ncell + cc];
+        const Rt eps = (Rt)Wb[(size_t)WEPS * P.ncell + cc];
+        w[0] = rho;
+        w[1] = gm1 * rho * eps;
+        w[2] = (Rt)Wb[(size_t)WU1 * P.ncell + cc];
+        w[3] = (Rt)Wb[(size_t)WU2 * P.ncell + cc];
+        w[4] = (Rt)Wb[(size_t)WU3 * P.ncell + cc];
+        w[5] = (Rt)Wb[(size_t)(WB1 + t1) * P.ncell + cc];
+        w[6] = (Rt)Wb[(size_t)(WB1 + t2) * P.ncell + cc];
+    };
+    const int gi = B.bi * BX + i;
+    const long long np = v.npt;
+    auto load_m = [&](const double* tab, int p, Met<Rt>& m) {
+        m.alpha = (Rt)tab[M_ALPHA * np + p];
+        m.beta[0] = (Rt)tab[M_B1 * np + p];
+        m.beta[1] = (Rt)tab[M_B2 * np + p];
+        m.beta[2] = (Rt)tab[M_B3 * np + p];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            m.g[q] = (Rt)tab[(M_G11 + q) * np + p];
+            m.gi[q] = Rt(0);
+        }
+        constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
+        m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
+        m.sg = (Rt)tab[M_SG * np + p];
+    };
+    Met<Rt> m;
+    if (d == 2) load_m(v.gc, (B.bj * BY + j0 + 2) * (v.nr + 5) + (gi + 2), m);
+    // entering face f: prev = cell f-1, cur = cell f, sp = limited slope of cell f-1
+    Rt prev[NV], cur[NV], sp[NV];
+    {
+        Rt a[NV];
+        cell(-2, a);
+        cell(-1, prev);
+        cell(0, cur);
+#pragma unroll
+        for (int q = 0; q < NV; q++) sp[q] = minmod(prev[q] - a[q], cur[q] - prev[q]);
+    }
+    Rt* Fd = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)d * NFW * P.nf;
+    const double* Bfd = Bf + ((size_t)b * 3 + d) * P.nf;
+    for (int f = 0; f <= n; f++) {
+        Rt nx[NV], sc[NV];
+        cell(f + 1, nx);
+        Pt<Rt> PL, PR;
+        Rt L[NV], Rr[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            sc[q] = minmod(cur[q] - prev[q], nx[q] - cur[q]);
+            L[q] = prev[q] + Rt(0.5) * sp[q];
+            Rr[q] = cur[q] - Rt(0.5) * sc[q];
+        }
+        PL.rho = L[0]; PR.rho = Rr[0];
+        PL.p = L[1]; PR.p = Rr[1];
+#pragma unroll
+        for (int e = 0; e < 3; e++) { PL.u[e] = L[2 + e]; PR.u[e] = Rr[2 + e]; }
+        PL.B[t1] = L[5]; PR.B[t1] = Rr[5];
+        PL.B[t2] = L[6]; PR.B[t2] = Rr[6];
+        const int fj = d == 1 ? f : j0, fk = d == 1 ? k0 : f;
+        if (d == 1) load_m(v.gf + (size_t)NMET * v.npt, (B.bj * BY + fj + 2) * (v.nr + 5) + (gi + 2), m);
+        const int fx = fidx(d, i, fj, fk);
+        const Rt bn = m.sg > Rt(0) ? (Rt)Bfd[fx] / m.sg : Rt(0);
+        PL.B[d] = bn;
+        PR.B[d] = bn;
+        Side<Rt> SL, SR;
+        face_side(m, gam, PL, d, SL);
+        face_side(m, gam, PR, d, SR);
+        Rt out[NFW];
+        hll(m, SL, SR, out);
+#pragma unroll
+        for (int q = 0; q < NFW; q++) Fd[(size_t)q * P.nf + fx] = out[q];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            prev[q] = cur[q];
+            cur[q] = nx[q];
+            sp[q] = sc[q];
+        }
+    }
+}
+
 template <class Rt>
 __device__ __forceinline__ Rt Fat(const KPS& P, const Rt* Fb, int d, int q, int i, int j, int k) {
     return Fb[((size_t)d * NFW + q) * P.nf + fidx(d, i, j, k)];
@@ -483,18 +599,18 @@ __device__ void face_update(const KPS& P, int b, int d, int i, int j, int k, dou
     const int gi = B.bi * BX + i;
     double rhs;
     if (d == 0) {
-        const double a1 = (double)(emf_z<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
-        const double a2 = (double)(emf_y<Rt>(P, Fb, i, j, k + 1) - emf_y<Rt>(P, Fb, i, j, k));
+        const double a1 = ((double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - (double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a2 = ((double)emf_y<Rt>(P, Fb, i, j, k + 1) - (double)emf_y<Rt>(P, Fb, i, j, k));
         rhs = -a1 / v.dth + a2 / v.dph;
     } else if (d == 1) {
         const double dri = v.dr[gi + 2];
-        const double a1 = (double)(emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k + 1) - emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
-        const double a2 = (double)(emf_z<Rt>(P, Fb, gj0, v.nth, i + 1, j, k) - emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a1 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k + 1) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a2 = ((double)emf_z<Rt>(P, Fb, gj0, v.nth, i + 1, j, k) - (double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
         rhs = -a1 / v.dph + a2 / dri;
     } else {
         const double dri = v.dr[gi + 2];
-        const double a1 = (double)(emf_y<Rt>(P, Fb, i + 1, j, k) - emf_y<Rt>(P, Fb, i, j, k));
-        const double a2 = (double)(emf_x<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a1 = ((double)emf_y<Rt>(P, Fb, i + 1, j, k) - (double)emf_y<Rt>(P, Fb, i, j, k));
+        const double a2 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
         rhs = -a1 / dri + a2 / v.dth;
     }
     const size_t x = ((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k);
@@ -564,8 +680,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
 
 __global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
     const int per = (P.nghc + TPB - 1) / TPB;
-    const int b = P.list[blockIdx.x / per];
-    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int t = (w % per) * TPB + threadIdx.x;
     if (t >= P.nghc) return;
     int i, j, k;
     unpack(P.ghc[t], i, j, k);
@@ -574,8 +691,9 @@ __global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
 
 __global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
     const int per = (P.nghf + TPB - 1) / TPB;
-    const int b = P.list[blockIdx.x / per];
-    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int t = (w % per) * TPB + threadIdx.x;
     if (t >= P.nghf) return;
     int i, j, k;
     const int x = P.ghf[t];
@@ -583,11 +701,34 @@ __global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
     ghost_face(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
 }
 
+// fluxes with the theta and phi faces by lines (flux_pencil) and the r faces
+// one per thread; P.flp lists the lines first (their threads run longest)
+template <class Rt, int TB, int MW>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_flux_lines(KPS P, int buf) {
+    const int per = (P.nflp + TB - 1) / TB;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int t = (w % per) * TB + threadIdx.x;
+    if (t >= P.nflp) return;
+    int i, j, k;
+    const int x = P.flp[t];
+    unpack(x, i, j, k);
+    const int d = (x >> 24) & 3;
+    const double* Bf = buf ? P.Bf1 : P.Bf0;
+    if (x & (1 << 26)) {
+        if (d == 1) flux_pencil<Rt, 1>(P, Bf, b, i, k);
+        else flux_pencil<Rt, 2>(P, Bf, b, i, j);
+    } else {
+        flux_face_d<Rt, 0>(P, Bf, b, i, j, k);
+    }
+}
+
 template <class Rt, int TB, int MW>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_flux(KPS P, int buf) {
     const int per = (P.nflf + TB - 1) / TB;
-    const int b = P.list[blockIdx.x / per];
-    const int t = (blockIdx.x % per) * TB + threadIdx.x;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int t = (w % per) * TB + threadIdx.x;
     if (t >= P.nflf) return;
     int i, j, k;
     const int x = P.flf[t];
@@ -598,8 +739,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
 template <class Rt>
 __global__ __launch_bounds__(TPB) void k_face(KPS P, int stage) {
     const int per = (P.nowf + TPB - 1) / TPB;
-    const int b = P.list[blockIdx.x / per];
-    const int t = (blockIdx.x % per) * TPB + threadIdx.x;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int t = (w % per) * TPB + threadIdx.x;
     if (t >= P.nowf) return;
     int i, j, k;
     const int x = P.owf[t];
@@ -610,8 +752,9 @@ __global__ __launch_bounds__(TPB) void k_face(KPS P, int stage) {
 template <class Rt, int TB, int MW>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_update(KPS P, int stage) {
     const int per = (P.nint + TB - 1) / TB;
-    const int b = P.list[blockIdx.x / per];
-    const int a = (blockIdx.x % per) * TB + threadIdx.x;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int a = (w % per) * TB + threadIdx.x;
     if (a >= P.nint) return;
     cell_update<Rt>(P, b, a, run_dt(P, b), stage);
 }
@@ -927,11 +1070,18 @@ constexpr int VAR_MW[NVAR] = {1, 4, 1, 4, 1, 4};
 
 #define SPH_VARIANTS(M) M(0, 256, 1) M(1, 256, 4) M(2, 128, 1) M(3, 128, 4) M(4, 64, 1) M(5, 64, 4)
 
+// flux variants 0 .. 5: one face per thread; 6 .. 11: theta and phi faces
+// by lines (same workgroup shapes)
+constexpr int NVAR_FLUX = 2 * NVAR;
 template <class Rt>
-void launch_flux(int v, int nl, int nitems, hipStream_t st, KPS P, int buf) {
+void launch_flux(int v, int nl, int nitems, int nlines, hipStream_t st, KPS P, int buf) {
     switch (v) {
 #define L_(i, tb, mw) \
     case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+        SPH_VARIANTS(L_)
+#undef L_
+#define L_(i, tb, mw) \
+    case NVAR + i: hipLaunchKernelGGL((k_flux_lines<Rt, tb, mw>), dim3(nl * ((nlines + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
         SPH_VARIANTS(L_)
 #undef L_
     }
@@ -1003,7 +1153,8 @@ public:
     DB<RunPar> runp;
     DB<RunCtr> ctr;
     DB<int> ghc, ghf, flf, owf;
-    int nghc = 0, nghf = 0, nflf = 0, nowf = 0;
+    int nghc = 0, nghf = 0, nflf = 0, nowf = 0, nflp = 0;
+    DB<int> flp;
     DB<int> list;                 // active block slots (sorted by run)
     DB<int> runlist;              // active runs
     DB<unsigned int> qctr;
@@ -1020,6 +1171,7 @@ public:
     int mode = 0;                 // 0 multi-kernel, 1 persistent
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
     int var_flux = 0, var_update = 0, var_c2p = 0;  // launch variants (VAR_TB, VAR_MW)
+    bool xcd_remap = true;        // XCD-contiguous workgroup numbering
     bool tables_dirty = true;
     std::string err;
     // graph of the step for the current active set
@@ -1089,6 +1241,17 @@ public:
                         else gf_.push_back(x);
                     }
         }
+        // theta and phi face lines (transverse ranges as the face ranges),
+        // then the r faces
+        std::vector<int> fl_;
+        for (int k = -1; k <= L.bz; k++)
+            for (int i = -1; i <= BX; i++) fl_.push_back((1 << 26) | (1 << 24) | pack_cell(i, 0, k));
+        for (int j = -1; j <= BY; j++)
+            for (int i = -1; i <= BX; i++) fl_.push_back((1 << 26) | (2 << 24) | pack_cell(i, j, 0));
+        for (int x : ff_)
+            if ((x >> 24) == 0) fl_.push_back(x);
+        nflp = (int)fl_.size();
+        flp.upload(fl_.data(), fl_.size(), stream);
         nghc = (int)gc_.size(); nghf = (int)gf_.size(); nflf = (int)ff_.size(); nowf = (int)ow_.size();
         ghc.upload(gc_.data(), gc_.size(), stream);
         ghf.upload(gf_.data(), gf_.size(), stream);
@@ -1105,7 +1268,9 @@ public:
         P.list = lst; P.nlist = nlst;
         P.ghc = ghc.p; P.nghc = nghc; P.ghf = ghf.p; P.nghf = nghf;
         P.flf = flf.p; P.nflf = nflf; P.owf = owf.p; P.nowf = nowf;
+        P.flp = flp.p; P.nflp = nflp;
         P.qctr = qctr.p;
+        P.xcd = xcd_remap ? 1 : 0;
         return P;
     }
 
@@ -1400,7 +1565,7 @@ public:
             tm("k_ghost_faces", [&] {
                 hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
             });
-            tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, stream, P, buf); });
+            tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, nflp, stream, P, buf); });
             tm("k_face", [&] {
                 hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
             });
@@ -1414,7 +1579,7 @@ public:
     }
     int key() const {
         return (mixed ? 1 : 0) | (mode << 1) | (wg_per_cu << 3) | (var_flux << 9) | (var_update << 12) |
-               (var_c2p << 15);
+               (var_c2p << 15) | ((xcd_remap ? 1 : 0) << 18);
     }
     void step() {
         if (act_slots.empty()) throw std::invalid_argument("no active runs");
@@ -1667,7 +1832,8 @@ int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
 }
 int bhs_set_variants(bhs_engine* h, int flux, int update, int c2p) {
     return guard(h, [&] {
-        for (int v : {flux, update, c2p})
+        if (flux < 0 || flux >= NVAR_FLUX) throw std::invalid_argument("flux variant out of range");
+        for (int v : {update, c2p})
             if (v < 0 || v >= NVAR) throw std::invalid_argument("launch variant out of range");
         h->e->var_flux = flux;
         h->e->var_update = update;
@@ -1675,10 +1841,16 @@ int bhs_set_variants(bhs_engine* h, int flux, int update, int c2p) {
         return BHS_OK;
     });
 }
+int bhs_set_xcd_remap(bhs_engine* h, int enable) {
+    return guard(h, [&] {
+        h->e->xcd_remap = enable != 0;
+        return BHS_OK;
+    });
+}
 int bhs_variant_info(int v, int* tpb, int* min_waves) {
-    if (v < 0 || v >= NVAR || !tpb || !min_waves) return BHS_EINVAL;
-    *tpb = VAR_TB[v];
-    *min_waves = VAR_MW[v];
+    if (v < 0 || v >= NVAR_FLUX || !tpb || !min_waves) return BHS_EINVAL;
+    *tpb = VAR_TB[v % NVAR];
+    *min_waves = VAR_MW[v % NVAR];
     return BHS_OK;
 }
 int bhs_use_graph(bhs_engine* h, int enable) {

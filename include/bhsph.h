@@ -120,12 +120,17 @@ int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
 int bhs_set_precision(bhs_engine* e, int mixed);
 int bhs_set_schedule(bhs_engine* e, int persistent, int wg_per_cu);
 int bhs_use_graph(bhs_engine* e, int enable);
-/* Launch variants of the flux, update and con2prim kernels (0 .. 5):
- * workgroup size and minimum wavefronts per SIMD of each are given by
- * bhs_variant_info.  Results do not depend on the variant; the autotuner
- * (bhfluid.tune) times them. */
+/* Launch variants of the flux, update and con2prim kernels (0 .. 5; flux
+ * also 6 .. 11: the theta and phi faces computed by lines of faces, one
+ * thread per line): workgroup size and minimum wavefronts per SIMD of each
+ * are given by bhs_variant_info.  Results do not depend on the variant; the
+ * autotuner (bhfluid.tune) times them. */
 int bhs_set_variants(bhs_engine* e, int flux, int update, int c2p);
 int bhs_variant_info(int variant, int* tpb, int* min_waves);
+/* Number the workgroups of the multi-workgroup kernels so that each of the
+ * eight XCDs (own L2) takes a contiguous range of blocks (1, the default) or
+ * in dispatch order (0). */
+int bhs_set_xcd_remap(bhs_engine* e, int enable);
 int bhs_sync(bhs_engine* e);
 
 #ifdef __cplusplus

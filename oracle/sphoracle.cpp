@@ -399,18 +399,18 @@ void face_update(Run& R, double dt, int stage) {
                         const int gi = R.bi[b] * BX + i;
                         double rhs;
                         if (d == 0) {
-                            const double a1 = (double)(emf_z<Rt>(R, v, b, i, j + 1, k) - emf_z<Rt>(R, v, b, i, j, k));
-                            const double a2 = (double)(emf_y<Rt>(R, v, b, i, j, k + 1) - emf_y<Rt>(R, v, b, i, j, k));
+                            const double a1 = ((double)emf_z<Rt>(R, v, b, i, j + 1, k) - (double)emf_z<Rt>(R, v, b, i, j, k));
+                            const double a2 = ((double)emf_y<Rt>(R, v, b, i, j, k + 1) - (double)emf_y<Rt>(R, v, b, i, j, k));
                             rhs = -a1 / v.dth + a2 / v.dph;
                         } else if (d == 1) {
                             const double dri = v.dr[gi + 2];
-                            const double a1 = (double)(emf_x<Rt>(R, v, b, i, j, k + 1) - emf_x<Rt>(R, v, b, i, j, k));
-                            const double a2 = (double)(emf_z<Rt>(R, v, b, i + 1, j, k) - emf_z<Rt>(R, v, b, i, j, k));
+                            const double a1 = ((double)emf_x<Rt>(R, v, b, i, j, k + 1) - (double)emf_x<Rt>(R, v, b, i, j, k));
+                            const double a2 = ((double)emf_z<Rt>(R, v, b, i + 1, j, k) - (double)emf_z<Rt>(R, v, b, i, j, k));
                             rhs = -a1 / v.dph + a2 / dri;
                         } else {
                             const double dri = v.dr[gi + 2];
-                            const double a1 = (double)(emf_y<Rt>(R, v, b, i + 1, j, k) - emf_y<Rt>(R, v, b, i, j, k));
-                            const double a2 = (double)(emf_x<Rt>(R, v, b, i, j + 1, k) - emf_x<Rt>(R, v, b, i, j, k));
+                            const double a1 = ((double)emf_y<Rt>(R, v, b, i + 1, j, k) - (double)emf_y<Rt>(R, v, b, i, j, k));
+                            const double a2 = ((double)emf_x<Rt>(R, v, b, i, j + 1, k) - (double)emf_x<Rt>(R, v, b, i, j, k));
                             rhs = -a1 / dri + a2 / v.dth;
                         }
                         const size_t x = ((size_t)b * 3 + d) * L.nf + fidx(d, i, j, k);

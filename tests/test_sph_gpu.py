@@ -175,3 +175,27 @@ def test_c2_full_size_stays_finite_div_b_round_off():
     print("C2 diagnostics", d0, d)
     assert d["divb_max"] <= 1e-13 * d["divb_scale"]
     assert abs(d["mass"] - d0["mass"]) <= 1e-4 * d0["mass"]
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_launch_variants_identical(mixed):
+    """Every launch variant (workgroup shapes, occupancy hints, the line
+    schedule of the theta/phi fluxes, XCD numbering) gives the same bits."""
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    ref = None
+    for flux, update, c2p, xcd in ((0, 0, 0, 1), (6, 2, 2, 1), (9, 5, 4, 0), (11, 1, 3, 1)):
+        eng = sph.Engine(16, len(blocks))
+        eng.set_precision(mixed)
+        eng.set_variants(flux, update, c2p)
+        eng.set_xcd_remap(bool(xcd))
+        run = eng.add_run(sph.PHYSICS, g, blocks)
+        run.init_torus(sph.TORUS, 42)
+        dt = eng.step([run], 3)
+        st = run.state()
+        if ref is None:
+            ref = (dt, st)
+        else:
+            assert np.array_equal(dt, ref[0]), (flux, update, c2p, xcd)
+            for x, y in zip(st, ref[1]):
+                assert np.array_equal(x, y), (flux, update, c2p, xcd)
