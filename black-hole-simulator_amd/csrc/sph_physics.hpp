@@ -280,10 +280,20 @@ SPH_HD int c2p(const Met<double>& m, double gam, const double U[8], double w[8])
     if (!(q.D > 0.0) || !isfinite(q.S2 + q.SB + q.B2 + q.E)) return 2;
     int rc = 0;
     double Z = 0.0;
-    // bracket: v^2 < 1 needs Z above the root of v^2(Z) = 1; f grows with Z
+    // bracket: v^2(Z) falls with Z and v^2 < 1 needs Z above its crossing
+    // of 1 (bisection for the crossing when v^2(D) >= 1); f grows with Z
     double lo = q.D, hi = 2.0 * (rmax(q.E, 0.0) + q.B2) + q.D;
-    for (int it = 0; it < 64 && c2p_v2(q, lo) >= 1.0; it++) lo = lo * 2.0;
-    if (!(q.E > 0.0) || c2p_v2(q, lo) >= 1.0) {
+    for (int it = 0; it < 64 && c2p_v2(q, hi) >= 1.0; it++) hi = 2.0 * hi;
+    if (c2p_v2(q, lo) >= 1.0) {
+        double a = lo, b = hi;
+        for (int it = 0; it < 200; it++) {
+            const double mid = 0.5 * (a + b);
+            if (!(mid > a) || !(mid < b)) break;
+            if (c2p_v2(q, mid) >= 1.0) a = mid; else b = mid;
+        }
+        lo = b;
+    }
+    if (!(q.E > 0.0) || !(c2p_v2(q, lo) < 1.0) || !(lo < hi)) {
         rc = 1;
     } else {
         Z = q.D + gam * rmax(q.E - 0.5 * q.B2 - q.D, 0.0);

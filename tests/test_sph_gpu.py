@@ -199,3 +199,57 @@ def test_launch_variants_identical(mixed):
             assert np.array_equal(dt, ref[0]), (flux, update, c2p, xcd)
             for x, y in zip(st, ref[1]):
                 assert np.array_equal(x, y), (flux, update, c2p, xcd)
+
+
+def test_serving_results_same_alone_and_in_the_mix():
+    """Continuous batching: requests of different grids, spins and fields
+    arrive while others run, are admitted as block slots free up (a budget
+    smaller than their sum), advance in shared launches, and leave when done.
+    Each result is bit-identical to the same request run alone."""
+    from bhfluid.serve import Request, Server
+    reqs = []
+    specs = [((64, 64, 64), 1, 0.3, 30.0), ((32, 32, 32), 2, 0.9, 100.0), ((96, 64, 64), 1, 0.6, 300.0),
+             ((48, 32, 32), 2, 0.1, 100.0), ((64, 64, 64), 1, 0.95, 30.0)]
+    for i, (sz, nl, a, beta) in enumerate(specs):
+        g = dict(nr=sz[0], nth=sz[1], nph=sz[2], nlevels=nl, r_in=sph.R_IN, r_out=sph.R_OUT)
+        reqs.append(Request(i, 0.0, g, dict(sph.PHYSICS, a=a), dict(sph.TORUS, beta=beta), seed=100 + i,
+                            nsteps=4 + 2 * i))
+    from bhfluid.serve import request_blocks
+    nbs = [len(request_blocks(r)) for r in reqs]
+    budget = max(nbs) + min(nbs)          # never all at once: arrivals wait for slots
+    srv = Server(budget, mixed=True, keep_state=True)
+    summary = srv.run(reqs, realtime=False)
+    print("served", summary, "blocks", nbs)
+    assert summary["max_batch_runs"] >= 2
+    batch_sizes = {x[1] for x in srv.log}
+    assert len(batch_sizes) >= 2            # the mix changed while running
+    for r in reqs:
+        alone = Server(nbs[r.rid], mixed=True, keep_state=True)
+        solo = Request(r.rid, 0.0, r.grid, r.physics, r.torus, r.seed, r.nsteps)
+        alone.run([solo], realtime=False)
+        assert solo.result["t"] == r.result["t"]
+        for x, y in zip(solo.result["state"], r.result["state"]):
+            assert np.array_equal(x, y), f"request {r.rid} differs alone and in the mix"
+
+
+def test_autotuner_choice_is_applied_and_exact(tmp_path, monkeypatch):
+    from bhfluid import tune
+    monkeypatch.setenv("BHFLUID_TUNE_CACHE", str(tmp_path / "tune.json"))
+    g = dict(sph.c2_grid(), nr=64, nth=32, nph=32)
+    blocks = sph.uniform_blocks(g, 16)
+    eng = sph.Engine(16, len(blocks))
+    run = eng.add_run(sph.PHYSICS, g, blocks)
+    run.init_torus(sph.TORUS, 42)
+    t = tune.tune(eng, [run], False, steps=2)
+    assert not t["cached"] and len(t["timings"]) >= 10
+    t2 = tune.tune(eng, [run], False, steps=2)
+    assert t2["cached"] and t2["config"] == t["config"]
+    # the tuned configuration steps to the same bits as the default one
+    ref = sph.Engine(16, len(blocks))
+    rr = ref.add_run(sph.PHYSICS, g, blocks)
+    rr.init_torus(sph.TORUS, 42)
+    run.init_torus(sph.TORUS, 42)
+    eng.step([run], 3)
+    ref.step([rr], 3)
+    for x, y in zip(run.state(), rr.state()):
+        assert np.array_equal(x, y)
