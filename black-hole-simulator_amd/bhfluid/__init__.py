@@ -118,6 +118,7 @@ def lib():
         _sig(L, "bh_math2_batch", _i, [_i, _i, _dp, _dp, _dp, _i])
         _sig(L, "bh_tree_use_graph", _i, [_v, _i])
         _sig(L, "bh_tree_set_exact_copies", _i, [_v, _i])
+        _sig(L, "bh_tree_maxdivB_device", _i, [_v, _dp, _dp])
         _sig(L, "bh_profile_only", _i, [_v, ctypes.c_char_p])
         _sig(L, "bh_tree_write_dump", _i, [_v, ctypes.c_char_p, _d, ctypes.c_longlong])
         _sig(L, "bh_device_info", _i, [ctypes.c_char_p, _i])
@@ -331,6 +332,15 @@ class AmrTree:
         """Write W of every leaf's active zones to a dump file (format in
         include/bhfluid.h; read it back with ``read_dump``)."""
         self._check(self._L.bh_tree_write_dump(self._h, str(path).encode(), float(time), int(step)), "write_dump")
+
+    def maxdivB_device(self):
+        """constrans::maxdivB of every block on the device: (per-block array,
+        max over the leaves)."""
+        per = np.zeros(self.nblocks)
+        mx = ctypes.c_double(0.0)
+        self._check(self._L.bh_tree_maxdivB_device(self._h, per.ctypes.data_as(_dp), ctypes.byref(mx)),
+                    "maxdivB_device")
+        return per, mx.value
 
     def maxdivB(self, b: int) -> float:
         """``constrans::maxdivB`` of block b (grid/transport.cpp:77-89)."""

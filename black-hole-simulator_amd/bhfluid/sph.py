@@ -248,10 +248,10 @@ class Run:
         self.eng._check(self.eng._L.bhs_c2p(self.eng._h, self.id), "c2p")
 
     def diagnostics(self) -> dict:
-        out = np.zeros(6)
+        out = np.zeros(8)
         self.eng._check(self.eng._L.bhs_run_diagnostics(self.eng._h, self.id, _p(out)), "diagnostics")
         return dict(mass=out[0], energy=out[1], divb_max=out[2], divb_scale=out[3], c2p_fail=int(out[4]),
-                    floors=int(out[5]))
+                    floors=int(out[5]), mdot=out[6], phi_b=out[7])
 
     def active(self, W: np.ndarray) -> np.ndarray:
         """Active zones of a W (or U) array: [nb][8][bz][16][16] (phi, theta, r)."""

@@ -1657,6 +1657,34 @@ __global__ __launch_bounds__(TPB) void k_mark_same(KP P, const int* __restrict__
     if (threadIdx.x == 0) same[b * 6 + side] = differ ? 0 : 1;
 }
 
+// constrans::maxdivB (src/fluid/grid/transport.cpp:77-89) of every listed
+// block: max over cells [-1, 17)^3 of |div B| of the face fields, one
+// workgroup per block; the maximum keeps the reference's std::max semantics
+// (a NaN divergence leaves the running maximum unchanged).
+__global__ __launch_bounds__(TPB) void k_maxdivb(KP P, const int* __restrict__ blocks, double* __restrict__ out) {
+    __shared__ double sm[TPB];
+    const int b = blocks[blockIdx.x];
+    const double dx = P.dxyz[3 * b], dy = P.dxyz[3 * b + 1], dz = P.dxyz[3 * b + 2];
+    const double* bx = P.Bf + ((size_t)b * 3 + 0) * NFACE;
+    const double* by = P.Bf + ((size_t)b * 3 + 1) * NFACE;
+    const double* bz = P.Bf + ((size_t)b * 3 + 2) * NFACE;
+    constexpr int N = BLK + 2 * GH - 2;  // 18 cells per direction
+    double mx = 0.0;
+    for (int t = threadIdx.x; t < N * N * N; t += TPB) {
+        const int i = t / (N * N) - GH + 1, j = (t / N) % N - GH + 1, k = t % N - GH + 1;
+        const double dv = (bx[ixf(i, j, k)] - bx[ixf(i - 1, j, k)]) / dx + (by[iyf(i, j, k)] - by[iyf(i, j - 1, k)]) / dy +
+                          (bz[izf(i, j, k)] - bz[izf(i, j, k - 1)]) / dz;
+        mx = smax(mx, fabs(dv));
+    }
+    sm[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sm[threadIdx.x] = smax(sm[threadIdx.x], sm[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[blockIdx.x] = sm[0];
+}
+
 // Protect mask of the early con2prim pass (KP::pmask): for every leaf side
 // that copies from a same-level block without reusing the source's outcome,
 // mark the source's opposite side.
@@ -2319,6 +2347,27 @@ public:
         out[5] = s[5];
         out[6] = (s[7] > 1e-20) ? (s[6] / s[7]) / rs : 0.0;
     }
+    // constrans::maxdivB of every block (per_block[nb]) and its maximum over
+    // the leaves, reduced on the device
+    void maxdivb(double* per_block, double* max_leaf) {
+        std::vector<int> all(nb);
+        for (int b = 0; b < nb; b++) all[b] = b;
+        DBuf<int> lst;
+        lst.alloc(std::max(1, nb));
+        lst.upload(all.data(), all.size(), stream);
+        DBuf<double> res;
+        res.alloc(std::max(1, nb));
+        KP P = kp();
+        if (nb) hipLaunchKernelGGL(k_maxdivb, dim3(nb), dim3(TPB), 0, stream, P, (const int*)lst.p, res.p);
+        HIPCHK(hipGetLastError());
+        std::vector<double> h(nb);
+        if (nb) HIPCHK(hipMemcpyAsync(h.data(), res.p, nb * sizeof(double), hipMemcpyDeviceToHost, stream));
+        HIPCHK(hipStreamSynchronize(stream));
+        double m = 0.0;
+        for (int L : plan.leaves) m = smax(m, h[L]);
+        if (per_block) std::copy(h.begin(), h.end(), per_block);
+        if (max_leaf) *max_leaf = m;
+    }
     // packed W of leaves [l0, l0 + n) into host memory h ([n][8][4096])
     void pack_w(int l0, int n, double* dbuf, double* h) {
         if (n <= 0) return;
@@ -2771,6 +2820,13 @@ int bh_profile_report(bh_tree* t, char* buf, int len) {
     });
 }
 int bh_device_sync(void) { return hipDeviceSynchronize() == hipSuccess ? BH_OK : BH_EDEVICE; }
+int bh_tree_maxdivB_device(bh_tree* t, double* per_block, double* max_leaf) {
+    return guarded(t, [&] {
+        need_device(t);
+        t->dev->maxdivb(per_block, max_leaf);
+        return BH_OK;
+    });
+}
 int bh_tree_diagnostics(bh_tree* t, double* out) {
     if (!out) return fail(t, BH_EINVAL, "null output");
     return guarded(t, [&] {

@@ -63,6 +63,7 @@ struct DRL {
     int run, level, nr, nth, nph, nbr, nbth, nbph;
     int coarser, finer;
     int npt;
+    int ih;               // r index of the zone row holding the outer horizon (-1: none)
     long long zone0;
     double dth, dph;
     const int* lut;
@@ -1021,13 +1022,13 @@ __global__ __launch_bounds__(TPB) void k_ic_final(KPS P, TorusDev D, const doubl
 // per block: mass, Killing energy (fixed-order tree sums over the block's
 // zones), max |div B| and the divergence stencil's scale
 __global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
-    __shared__ double sm[4][TPB];
+    __shared__ double sm[6][TPB];
     const int b = P.list[blockIdx.x];
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
     const double* Ub = P.U + (size_t)b * 8 * P.ncell;
     const double* F0 = P.Bf0 + (size_t)b * 3 * P.nf;
-    double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0;
+    double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0, md = 0.0, ph = 0.0;
     for (int a = threadIdx.x; a < P.nint; a += TPB) {
         const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
         const int gi = B.bi * BX + i, gj = B.bj * BY + j;
@@ -1048,8 +1049,22 @@ __global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
         const double t0 = (x0 - x1) / dri, t1 = (y0 - y1) / v.dth, t2 = (z0 - z1) / v.dph;
         dm = fmax(dm, fabs(t0 + t1 + t2));
         sc = fmax(sc, (fabs(x0) + fabs(x1)) / dri + (fabs(y0) + fabs(y1)) / v.dth + (fabs(z0) + fabs(z1)) / v.dph);
+        if (gi == v.ih) {
+            // accretion rate through the horizon row, -sqrt(gamma) D (alpha v^r - beta^r)
+            // dtheta dphi, and half the unsigned radial flux sqrt(gamma) |B^r| (faces)
+            const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+            double ut[3], ul[3], g6[6];
+            for (int q = 0; q < 3; q++) ut[q] = Wb[(size_t)(WU1 + q) * P.ncell + c];
+            for (int q = 0; q < 6; q++) g6[q] = v.gc[(M_G11 + q) * np + p];
+            lower3(g6, ut, ul);
+            const double Wl = sqrt(1.0 + dot3(ul, ut));
+            const double vr = ut[0] / Wl;
+            md -= Ub[(size_t)UD * P.ncell + c] * (al * vr - v.gc[M_B1 * np + p]) * v.dth * v.dph;
+            ph += 0.5 * fabs(x1) * v.dth * v.dph;
+        }
     }
     sm[0][threadIdx.x] = ms; sm[1][threadIdx.x] = en; sm[2][threadIdx.x] = dm; sm[3][threadIdx.x] = sc;
+    sm[4][threadIdx.x] = md; sm[5][threadIdx.x] = ph;
     __syncthreads();
     for (int o = TPB / 2; o > 0; o >>= 1) {
         if (threadIdx.x < o) {
@@ -1057,10 +1072,12 @@ __global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
             sm[1][threadIdx.x] += sm[1][threadIdx.x + o];
             sm[2][threadIdx.x] = fmax(sm[2][threadIdx.x], sm[2][threadIdx.x + o]);
             sm[3][threadIdx.x] = fmax(sm[3][threadIdx.x], sm[3][threadIdx.x + o]);
+            sm[4][threadIdx.x] += sm[4][threadIdx.x + o];
+            sm[5][threadIdx.x] += sm[5][threadIdx.x + o];
         }
         __syncthreads();
     }
-    if (threadIdx.x < 4) out[(size_t)blockIdx.x * 4 + threadIdx.x] = sm[threadIdx.x][0];
+    if (threadIdx.x < 6) out[(size_t)blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
 // ---------------------------------------------------- launch variants
@@ -1308,6 +1325,14 @@ public:
             z0 += (long long)v.nr * v.nth * v.nph;
             v.dth = M_PI / v.nth;
             v.dph = 2.0 * M_PI / v.nph;
+            {
+                // the zone row whose r faces bracket the outer horizon
+                const double disc = 1.0 - ph.a * ph.a - ph.Q * ph.Q;
+                const double rh = 1.0 + std::sqrt(disc > 0.0 ? disc : 0.0);
+                const double dl = (std::log(g.r_out) - std::log(g.r_in)) / v.nr;
+                const int ih = (int)std::floor((std::log(rh) - std::log(g.r_in)) / dl);
+                v.ih = (ih >= 0 && ih < v.nr) ? ih : -1;
+            }
             lut[l].assign((size_t)v.nbr * v.nbth * v.nbph, -1);
             lv[l] = v;
         }
@@ -1699,23 +1724,27 @@ public:
     void diagnostics(int r, double* out) {
         set_active({r});
         const int nl = (int)act_slots.size();
-        diagbuf.alloc((size_t)nl * 4);
+        diagbuf.alloc((size_t)nl * 6);
         KPS P = kp(list.p, nl);
         hipLaunchKernelGGL(k_diag, dim3(nl), dim3(TPB), 0, stream, P, diagbuf.p);
-        std::vector<double> h((size_t)nl * 4);
+        std::vector<double> h((size_t)nl * 6);
         SCHK(hipMemcpyAsync(h.data(), diagbuf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
         std::vector<RunCtr> c;
         read_ctr(c);
-        double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0;
+        double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0, md = 0.0, ph = 0.0;
         for (int q = 0; q < nl; q++) {
-            ms += h[4 * q];
-            en += h[4 * q + 1];
-            dm = std::max(dm, h[4 * q + 2]);
-            sc = std::max(sc, h[4 * q + 3]);
+            ms += h[6 * q];
+            en += h[6 * q + 1];
+            dm = std::max(dm, h[6 * q + 2]);
+            sc = std::max(sc, h[6 * q + 3]);
+            md += h[6 * q + 4];
+            ph += h[6 * q + 5];
         }
         out[0] = ms; out[1] = en; out[2] = dm; out[3] = sc;
         out[4] = (double)c[r].c2p_fail;
         out[5] = (double)c[r].floors;
+        out[6] = md;
+        out[7] = ph;
     }
 };
 

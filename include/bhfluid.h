@@ -143,6 +143,11 @@ int bh_tree_diagnostics(bh_tree* t, double* out);
 int bh_tree_write_dump(bh_tree* t, const char* path, double time, long long step);
 /* constrans::maxdivB of block b, src/fluid/grid/transport.cpp:77-89 */
 int bh_tree_maxdivB(bh_tree* t, int b, double* out);
+/* constrans::maxdivB (transport.cpp:77-89) of every block computed on the
+ * device from the device state (per_block[nb], may be NULL) and its maximum
+ * over the leaves (*max_leaf, may be NULL); the reference defines the
+ * function but never calls it. */
+int bh_tree_maxdivB_device(bh_tree* t, double* per_block, double* max_leaf);
 
 /* Pointwise device entry points, for parity tests.  Inputs are host arrays
  * of n points; the metric (M, a, Q) is evaluated at each (r, th).

@@ -106,10 +106,13 @@ int bhs_profile_report(bhs_engine* e, char* buf, int len);
 /* con2prim with floors of a run's active zones (W of the current U;
  * integrator.cpp:201-222). */
 int bhs_c2p(bhs_engine* e, int run);
-/* out[6] = rest mass, Killing energy (sum of alpha (tau + D) - beta^j S_j,
+/* out[8] = rest mass, Killing energy (sum of alpha (tau + D) - beta^j S_j,
  * densitised), max |div B| over the active zones (constrans::maxdivB,
  * src/fluid/grid/transport.cpp:77-89), the divergence stencil's scale,
- * con2prim failures and floored zones since the run was added. */
+ * con2prim failures and floored zones since the run was added, the
+ * accretion rate Mdot and the horizon magnetic flux Phi_B on the zone row
+ * holding the outer horizon (analyser Mdot / Phi_B,
+ * src/fluid/data/quantitatives.cpp:99-118). */
 int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
 
 /* Launch configuration (no reference counterpart): mixed 1 = FP32

@@ -781,14 +781,17 @@ void os_counters(void* r, long long* out) {
     out[1] = R->floors;
 }
 
-// out[4] = total rest mass (sum of U.D dr dth dph), Killing energy
+// out[6] = total rest mass (sum of U.D dr dth dph), Killing energy
 // (sum of alpha (tau + D) - beta^j S_j, densitised), max |div B| over
-// active zones, and the scale of the divergence stencil (max over zones of
-// the sum of |face terms|).
+// active zones, the scale of the divergence stencil (max over zones of the
+// sum of |face terms|), the accretion rate through the zone row holding the
+// outer horizon and half the unsigned radial magnetic flux there.
 void os_diag(void* r, double* out) {
     Run* R = (Run*)r;
     const Lay& L = R->L;
-    double mass = 0.0, en = 0.0, dmax = 0.0, smax = 0.0;
+    double mass = 0.0, en = 0.0, dmax = 0.0, smax = 0.0, mdot = 0.0, phib = 0.0;
+    const double disc = 1.0 - R->a * R->a - R->Q * R->Q;
+    const double rh = 1.0 + std::sqrt(disc > 0.0 ? disc : 0.0);
     for (int b = 0; b < R->nb(); b++) {
         const Level& v = R->lev[R->bl[b]];
         const double* Ub = R->Ub(b);
@@ -824,12 +827,23 @@ void os_diag(void* r, double* out) {
                                       (std::fabs(F1[fidx(1, i, j + 1, k)]) + std::fabs(F1[fidx(1, i, j, k)])) / v.dth +
                                       (std::fabs(F2[fidx(2, i, j, k + 1)]) + std::fabs(F2[fidx(2, i, j, k)])) / v.dph;
                     smax = std::max(smax, sc);
+                    const int ih = (int)std::floor((std::log(rh) - v.lnr0) / v.dlnr);
+                    if (gi == ih) {
+                        double ut[3], ul[3];
+                        for (int q = 0; q < 3; q++) ut[q] = R->W[((size_t)b * 8 + WU1 + q) * L.ncell + c];
+                        lower3(g6, ut, ul);
+                        const double Wl = std::sqrt(1.0 + dot3(ul, ut));
+                        mdot -= Ub[(size_t)UD * L.ncell + c] * (al * (ut[0] / Wl) - bu[0]) * v.dth * v.dph;
+                        phib += 0.5 * std::fabs(F0[fidx(0, i, j, k)]) * v.dth * v.dph;
+                    }
                 }
     }
     out[0] = mass;
     out[1] = en;
     out[2] = dmax;
     out[3] = smax;
+    out[4] = mdot;
+    out[5] = phib;
 }
 
 // pointwise entry points for pinning the shared physics

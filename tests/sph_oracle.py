@@ -123,6 +123,6 @@ class OracleRun:
         return c
 
     def diag(self):
-        d = np.zeros(4)
+        d = np.zeros(6)
         lib().os_diag(self._h, P(d))
-        return dict(mass=d[0], energy=d[1], divb_max=d[2], divb_scale=d[3])
+        return dict(mass=d[0], energy=d[1], divb_max=d[2], divb_scale=d[3], mdot=d[4], phi_b=d[5])

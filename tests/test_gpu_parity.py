@@ -446,3 +446,28 @@ def test_step_against_reference_arithmetic(name, nsteps):
         r = _normwise_rel_linf(g, o)
         print(f"[{name}] step {step}: relative L-inf {r:.3e}", flush=True)
         assert r <= 1e-10, f"{name} step {step}: relative L-inf {r:.3e}"
+
+
+def test_maxdivb_on_device_matches_oracle():
+    """constrans::maxdivB (transport.cpp:77-89) reduced on the device, per
+    block and over the leaves, against the oracle's, bit for bit, on the
+    two-level tree after two steps (where the reference's divergence is far
+    from round-off)."""
+    t, params = build(configs.R3)
+    s0 = snapshot(t)
+    ot = OracleTree.from_state(s0, params, libm=1)
+    rk = bhfluid.RK2Integrator(configs.CFL)
+    for _ in range(2):
+        try:
+            rk.step(t)
+        except bhfluid.DomainError:
+            pass
+        ot.step(configs.CFL)
+    per, mx = t.maxdivB_device()
+    O = oracle_lib()
+    ref = np.array([O.or_maxdivB(ot.h, b) for b in range(len(per))])
+    n, ex = first_diff(per, ref)
+    assert n == 0, f"{n} blocks differ, e.g. {ex}"
+    # the per-block maxima never hold NaN (std::max keeps the running value)
+    leaf = t.blocks()[0][:, 4] == 1
+    assert mx == ref[leaf].max()

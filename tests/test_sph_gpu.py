@@ -253,3 +253,23 @@ def test_autotuner_choice_is_applied_and_exact(tmp_path, monkeypatch):
     ref.step([rr], 3)
     for x, y in zip(run.state(), rr.state()):
         assert np.array_equal(x, y)
+
+
+def test_diagnostics_against_oracle():
+    """Device reductions (rest mass, Killing energy, max |div B|, Mdot and
+    Phi_B on the horizon row) against the oracle's sequential sums after 5
+    steps of the 3-level tree: the maxima bit for bit, the sums within the
+    summation-order bound."""
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    eng, run, orc, _, _ = make_pair(g, 16, blocks)
+    for _ in range(5):
+        eng.step([run])
+        orc.step(False)
+    dg, do = run.diagnostics(), orc.diag()
+    print("device", dg, "oracle", do)
+    assert dg["divb_max"] == do["divb_max"] and dg["divb_scale"] == do["divb_scale"]
+    n = len(blocks) * 4096
+    for k in ("mass", "energy", "mdot", "phi_b"):
+        assert abs(dg[k] - do[k]) <= n * 2.2e-16 * abs(do[k]) + 1e-300, k
+    assert dg["mdot"] > 0 and dg["phi_b"] >= 0
