@@ -85,6 +85,7 @@ def lib():
         sig("bhs_set_variants", _i, [_v, _i, _i, _i])
         sig("bhs_variant_info", _i, [_i, _ip, _ip])
         sig("bhs_set_xcd_remap", _i, [_v, _i])
+        sig("bhs_run_diagnostics_static", _i, [_v, _i, _dp])
         sig("bhs_profile", _i, [_v, _i, ctypes.c_char_p])
         sig("bhs_profile_report", _i, [_v, ctypes.c_char_p, _i])
         _sigs_done = True

@@ -1080,6 +1080,30 @@ __global__ __launch_bounds__(TPB) void k_diag(KPS P, double* out) {
     if (threadIdx.x < 6) out[(size_t)blockIdx.x * 6 + threadIdx.x] = sm[threadIdx.x][0];
 }
 
+// ---------------------------------------------------- allocation-free diagnostics
+// Scratch of bhs_run_diagnostics_static: the per-block partials of k_diag
+// (6 words each) for up to DIAG_MAX_BLOCKS blocks, then the 6 reduced
+// words.  One copy per GPU, held until the process ends; calls that overlap
+// on one GPU share it and race.
+constexpr int DIAG_MAX_BLOCKS = 65536;
+__device__ __attribute__((used)) __align__(256) double scratch[6 * DIAG_MAX_BLOCKS + 8];
+
+// the host loop of Engine::diagnostics (sums in block order, maxima), on
+// the device: one thread, the same additions in the same order
+__global__ void k_diag_seq(const double* __restrict__ part, int nl, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double ms = 0.0, en = 0.0, dm = 0.0, sc = 0.0, md = 0.0, ph = 0.0;
+    for (int q = 0; q < nl; q++) {
+        ms += part[6 * q];
+        en += part[6 * q + 1];
+        dm = dm < part[6 * q + 2] ? part[6 * q + 2] : dm;
+        sc = sc < part[6 * q + 3] ? part[6 * q + 3] : sc;
+        md += part[6 * q + 4];
+        ph += part[6 * q + 5];
+    }
+    out[0] = ms; out[1] = en; out[2] = dm; out[3] = sc; out[4] = md; out[5] = ph;
+}
+
 // ---------------------------------------------------- launch variants
 constexpr int NVAR = 6;
 constexpr int VAR_TB[NVAR] = {256, 256, 128, 128, 64, 64};
@@ -1155,6 +1179,7 @@ struct HostRun {
     std::vector<int> slots;    // its block slots, in the caller's block order
     std::vector<std::unique_ptr<DB<double>>> tabs;  // per level: gc, gf, dr
     std::vector<std::unique_ptr<DB<int>>> luts;
+    std::unique_ptr<DB<int>> slotlist;                // its slots on the device
 };
 
 inline int pack_cell(int i, int j, int k) { return ((k + 2) << 16) | ((j + 2) << 8) | (i + 2); }
@@ -1164,7 +1189,7 @@ public:
     Lay L;
     long long cap = 0;
     hipStream_t stream = nullptr;
-    DB<double> U, W, Us, Bf0, Bf1, F, scratch;
+    DB<double> U, W, Us, Bf0, Bf1, F, icwork;  // icwork: initial-data work array
     DB<Blk> blk;
     DB<DRL> rl;
     DB<RunPar> runp;
@@ -1197,10 +1222,14 @@ public:
     bool use_graph = true;
     int graph_key = -1;
 
+    double* scratch_dev = nullptr;  // this device's copy of the static scratch array
+
     explicit Engine(int bz, long long capacity) {
         L = make_lay(bz);
         cap = capacity;
         SCHK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        // the device address of the static scratch array on the current GPU
+        SCHK(hipGetSymbolAddress((void**)&scratch_dev, HIP_SYMBOL(scratch)));
         U.alloc((size_t)cap * 8 * L.ncell);
         W.alloc((size_t)cap * 8 * L.ncell);
         Us.alloc((size_t)cap * 5 * L.nint);
@@ -1356,6 +1385,8 @@ public:
         for (int l = 0; l < g.nlevels; l++)
             for (int& e : lut[l])
                 if (e >= 0) e = R.slots[e];
+        R.slotlist = std::make_unique<DB<int>>();
+        R.slotlist->upload(R.slots.data(), R.slots.size(), stream);
         const double lnr0 = std::log(g.r_in);
         for (int l = 0; l < g.nlevels; l++) {
             DRL& v = lv[l];
@@ -1659,7 +1690,7 @@ public:
         D.st = U128{t.pcg_state_hi, t.pcg_state_lo};
         D.inc = U128{t.pcg_inc_hi, t.pcg_inc_lo};
         const int nl = (int)act_slots.size();
-        scratch.alloc((size_t)cap * L.ncell);
+        icwork.alloc((size_t)cap * L.ncell);
         KPS P = kp(list.p, nl);
         // clear the run's state: face fields (incl. phi faces), W
         for (int s : R.slots) {
@@ -1669,15 +1700,15 @@ public:
             SCHK(hipMemsetAsync(U.p + (size_t)s * 8 * L.ncell, 0, 8 * L.ncell * sizeof(double), stream));
         }
         SCHK(hipMemsetAsync(icmax.p, 0, 4 * sizeof(unsigned long long), stream));
-        hipLaunchKernelGGL(k_ic_torus, dim3(nl * ((L.ncell + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+        hipLaunchKernelGGL(k_ic_torus, dim3(nl * ((L.ncell + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, icwork.p,
                            icmax.p);
-        hipLaunchKernelGGL(k_ic_faces, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+        hipLaunchKernelGGL(k_ic_faces, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, icwork.p,
                            icmax.p);
-        hipLaunchKernelGGL(k_ic_cells, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p,
+        hipLaunchKernelGGL(k_ic_cells, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, icwork.p,
                            icmax.p);
         hipLaunchKernelGGL(k_ic_scale, dim3(nl * ((3 * L.nf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, icmax.p,
                            t.beta, icscale.p);
-        hipLaunchKernelGGL(k_ic_final, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, scratch.p);
+        hipLaunchKernelGGL(k_ic_final, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, D, icwork.p);
         SCHK(hipGetLastError());
         unsigned long long mx[4];
         double sc = 0.0;
@@ -1719,6 +1750,33 @@ public:
             if (hB) SCHK(hipMemcpyAsync(Bf0.p + (size_t)s * 3 * L.nf, hB + q * 3 * L.nf, 3 * L.nf * sizeof(double), hipMemcpyHostToDevice, stream));
         }
         sync();
+    }
+
+    // diagnostics without any allocation: the run's persistent slot list,
+    // the static scratch array, one stream-ordered copy out
+    void diagnostics_static(int r, double* out) {
+        if (r < 0 || r >= (int)runs.size() || !runs[r].live) throw std::invalid_argument("no such run");
+        if (tables_dirty) throw std::domain_error("the engine's block tables are not on the device yet (step or "
+                                                  "initialise a run first)");
+        const HostRun& R = runs[r];
+        const int nl = (int)R.slots.size();
+        if (nl > DIAG_MAX_BLOCKS) throw std::invalid_argument("run has more blocks than the static scratch holds");
+        KPS P = kp(R.slotlist->p, nl);
+        double* part = scratch_dev;
+        double* fin = scratch_dev + (size_t)6 * DIAG_MAX_BLOCKS;
+        hipLaunchKernelGGL(k_diag, dim3(nl), dim3(TPB), 0, stream, P, part);
+        hipLaunchKernelGGL(k_diag_seq, dim3(1), dim3(64), 0, stream, (const double*)part, nl, fin);
+        SCHK(hipGetLastError());
+        double h[6];
+        RunCtr c;
+        SCHK(hipMemcpyAsync(h, fin, sizeof h, hipMemcpyDeviceToHost, stream));
+        SCHK(hipMemcpyAsync(&c, ctr.p + r, sizeof c, hipMemcpyDeviceToHost, stream));
+        SCHK(hipStreamSynchronize(stream));
+        out[0] = h[0]; out[1] = h[1]; out[2] = h[2]; out[3] = h[3];
+        out[4] = (double)c.c2p_fail;
+        out[5] = (double)c.floors;
+        out[6] = h[4];
+        out[7] = h[5];
     }
 
     void diagnostics(int r, double* out) {
@@ -1970,6 +2028,13 @@ int bhs_c2p(bhs_engine* h, int run) {
 int bhs_run_diagnostics(bhs_engine* h, int run, double* out) {
     return guard(h, [&] {
         h->e->diagnostics(run, out);
+        return BHS_OK;
+    });
+}
+int bhs_run_diagnostics_static(bhs_engine* h, int run, double* out) {
+    return guard(h, [&] {
+        if (!out) throw std::invalid_argument("null output");
+        h->e->diagnostics_static(run, out);
         return BHS_OK;
     });
 }

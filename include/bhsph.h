@@ -114,6 +114,16 @@ int bhs_c2p(bhs_engine* e, int run);
  * holding the outer horizon (analyser Mdot / Phi_B,
  * src/fluid/data/quantitatives.cpp:99-118). */
 int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
+/* bhs_run_diagnostics for callers that cannot allocate (code under graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  Same out[8], computed without any device or
+ * host allocation: the per-block partials and their reduction go to a
+ * static device array of 65536 blocks x 6 words (3 MB per GPU, held until
+ * the process ends); runs of more blocks are rejected (BHS_EINVAL).  The
+ * engine's block tables must already be on the device (after any step or
+ * initial data of a run since the last run was added), else BHS_ESTATE.
+ * Calls that overlap on one GPU share the array and race. */
+int bhs_run_diagnostics_static(bhs_engine* e, int run, double* out);
 
 /* Launch configuration (no reference counterpart): mixed 1 = FP32
  * reconstruction, Riemann fluxes and EMFs (con2prim, sources, update and
