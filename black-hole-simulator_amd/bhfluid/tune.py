@@ -21,7 +21,7 @@ import numpy as np
 
 from . import sph
 
-DEFAULT = dict(persistent=0, wg_per_cu=0, flux=0, update=0, c2p=0, xcd=1)
+DEFAULT = dict(persistent=0, wg_per_cu=0, flux=0, update=0, c2p=1, xcd=1)
 
 
 def _cache_path() -> Path:

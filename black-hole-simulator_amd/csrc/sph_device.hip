@@ -72,6 +72,16 @@ struct DRL {
     const double* dr;
 };
 
+// Table pointers reached through device memory (DRL records) are generic
+// pointers to the compiler, which then emits flat loads (counted against
+// both the vector-memory and the LDS/scalar wait counters).  The tables
+// live in device global memory, so their loads are issued as global loads.
+#define GLB __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const GLB T* gl(const T* p) {
+    return (const GLB T*)p;
+}
+
 struct KPS {
     int bz, sz, ncell, nint, nf;
     const Blk* blk;       // [slot]
@@ -118,94 +128,113 @@ __device__ __forceinline__ int wg_index(const KPS& P) {
 __device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 __device__ __forceinline__ int leaf_at(const KPS& P, const DRL& v, int gi, int gj, int gk) {
-    const int b1 = gi / BX, b2 = gj / BY, b3 = gk / P.bz;
-    return v.lut[((size_t)b3 * v.nbth + b2) * v.nbr + b1];
+    const int b1 = gi / BX, b2 = gj / BY, b3 = P.bz == 1 ? gk : gk / BX;  // bz is 16 or 1
+    return gl(v.lut)[((size_t)b3 * v.nbth + b2) * v.nbr + b1];
 }
 __device__ __forceinline__ void bc_cell(const DRL& v, int& gi, int& gj, int& gk, int& flip) {
     if (gj < 0) { gj = -1 - gj; flip ^= 1; }
     if (gj >= v.nth) { gj = 2 * v.nth - 1 - gj; flip ^= 1; }
     if (gi < 0) gi = 0;
     if (gi >= v.nr) gi = v.nr - 1;
-    gk = ((gk % v.nph) + v.nph) % v.nph;
+    if (gk < 0 || gk >= v.nph) gk = ((gk % v.nph) + v.nph) % v.nph;  // periodic in phi
 }
-__device__ __forceinline__ void fetch_w(const KPS& P, int s, int gi, int gj, int gk, int flip, double w[8]) {
+// a source cell of a ghost fill: its slot, its cell index there, and
+// whether u~^theta and B^theta change sign (reflection at a pole)
+struct Src {
+    int s, c, flip;
+};
+__device__ __forceinline__ void src_at(const KPS& P, int s, int gi, int gj, int gk, int flip, Src& o) {
     const Blk B = P.blk[s];
-    const int c = cidx(gi - B.bi * BX, gj - B.bj * BY, gk - B.bk * P.bz);
-    const double* Ws = P.W + (size_t)s * 8 * P.ncell;
-#pragma unroll
-    for (int q = 0; q < 8; q++) w[q] = Ws[(size_t)q * P.ncell + c];
-    if (flip) { w[WU2] = -w[WU2]; w[WB2] = -w[WB2]; }
+    o.s = s;
+    o.c = cidx(gi - B.bi * BX, gj - B.bj * BY, gk - B.bk * P.bz);
+    o.flip = flip;
 }
-__device__ __forceinline__ bool leaf_w(const KPS& P, const DRL& v, int gi, int gj, int gk, double w[8]) {
+__device__ __forceinline__ bool leaf_src(const KPS& P, const DRL& v, int gi, int gj, int gk, Src& o) {
     int flip = 0;
     bc_cell(v, gi, gj, gk, flip);
     const int s = leaf_at(P, v, gi, gj, gk);
     if (s < 0) return false;
-    fetch_w(P, s, gi, gj, gk, flip, w);
+    src_at(P, s, gi, gj, gk, flip, o);
     return true;
 }
+__device__ __forceinline__ double w_of(const KPS& P, const double* __restrict__ Wr, const Src& x, int q) {
+    const double w = Wr[((size_t)x.s * 8 + q) * P.ncell + x.c];
+    return (x.flip && (q == WU2 || q == WB2)) ? -w : w;
+}
 
-// ghost cell (i, j, k) of block b
-__device__ void ghost_cell(const KPS& P, int b, int i, int j, int k) {
+// ghost cell (i, j, k) of block b.  The source cells are located first,
+// then the eight words are formed and stored one at a time.  Sources are
+// active zones and the destination a ghost cell, so the reads (Wr) and the
+// writes (Ww) of one call never meet, although both point into W: declared
+// as distinct, the loads need not wait for earlier stores.
+__device__ __forceinline__ void ghost_cell_rw(const KPS& P, const double* __restrict__ Wr, double* __restrict__ Ww,
+                                              int b, int i, int j, int k) {
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
     int gi = B.bi * BX + i, gj = B.bj * BY + j, gk = B.bk * P.bz + k, flip = 0;
     bc_cell(v, gi, gj, gk, flip);
-    double out[8];
+    double* Wc = Ww + (size_t)b * 8 * P.ncell + cidx(i, j, k);
+    auto put = [&](int q, double x) { Wc[(size_t)q * P.ncell] = (flip && (q == WU2 || q == WB2)) ? -x : x; };
     const int s = leaf_at(P, v, gi, gj, gk);
     if (s >= 0) {
-        fetch_w(P, s, gi, gj, gk, flip, out);
+        // same level: copy
+        Src x;
+        src_at(P, s, gi, gj, gk, 0, x);
+#pragma unroll
+        for (int q = 0; q < 8; q++) put(q, w_of(P, Wr, x, q));
     } else {
         bool done = false;
         if (v.coarser >= 0) {
+            // coarser leaf: its value plus minmod-limited slopes in each
+            // direction (zero where a neighbour is not a leaf of that level)
             const DRL& vc = P.rl[v.coarser];
             const int c1 = gi >> 1, c2 = gj >> 1, c3 = gk >> 1;
-            double wc[8];
-            if (leaf_w(P, vc, c1, c2, c3, wc)) {
-                double acc[8];
+            Src xc;
+            if (leaf_src(P, vc, c1, c2, c3, xc)) {
+                Src xm[3], xp[3];
+                bool h[3];
 #pragma unroll
-                for (int q = 0; q < 8; q++) acc[q] = wc[q];
-                const int par[3] = {gi & 1, gj & 1, gk & 1};
                 for (int d = 0; d < 3; d++) {
-                    double wm[8], wp[8];
                     const int o0 = d == 0, o1 = d == 1, o2 = d == 2;
-                    const bool hm = leaf_w(P, vc, c1 - o0, c2 - o1, c3 - o2, wm);
-                    const bool hp = leaf_w(P, vc, c1 + o0, c2 + o1, c3 + o2, wp);
-                    const double off = par[d] ? 0.25 : -0.25;
-#pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const double sl = (hm && hp) ? minmod(wc[q] - wm[q], wp[q] - wc[q]) : 0.0;
-                        acc[q] = acc[q] + sl * off;
-                    }
+                    const bool hm = leaf_src(P, vc, c1 - o0, c2 - o1, c3 - o2, xm[d]);
+                    const bool hp = leaf_src(P, vc, c1 + o0, c2 + o1, c3 + o2, xp[d]);
+                    h[d] = hm && hp;
                 }
+                const double off[3] = {(gi & 1) ? 0.25 : -0.25, (gj & 1) ? 0.25 : -0.25, (gk & 1) ? 0.25 : -0.25};
 #pragma unroll
-                for (int q = 0; q < 8; q++) out[q] = acc[q];
-                if (flip) { out[WU2] = -out[WU2]; out[WB2] = -out[WB2]; }
+                for (int q = 0; q < 8; q++) {
+                    const double wc = w_of(P, Wr, xc, q);
+                    double acc = wc;
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        const double sl = h[d] ? minmod(wc - w_of(P, Wr, xm[d], q), w_of(P, Wr, xp[d], q) - wc) : 0.0;
+                        acc = acc + sl * off[d];
+                    }
+                    put(q, acc);
+                }
                 done = true;
             }
         }
         if (!done) {
+            // finer leaves: the mean of the eight cells covering this one
             const DRL& vf = P.rl[v.finer];
-            double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-            for (int c = 0; c < 2; c++)
-                for (int bb = 0; bb < 2; bb++)
-                    for (int a = 0; a < 2; a++) {
-                        double wf[8];
-                        const int fi = 2 * gi + a, fj = 2 * gj + bb, fk = 2 * gk + c;
-                        fetch_w(P, leaf_at(P, vf, fi, fj, fk), fi, fj, fk, 0, wf);
+            Src xf[8];
 #pragma unroll
-                        for (int q = 0; q < 8; q++) acc[q] += wf[q];
-                    }
+            for (int n = 0; n < 8; n++) {
+                const int fi = 2 * gi + (n & 1), fj = 2 * gj + ((n >> 1) & 1), fk = 2 * gk + (n >> 2);
+                src_at(P, leaf_at(P, vf, fi, fj, fk), fi, fj, fk, 0, xf[n]);
+            }
 #pragma unroll
-            for (int q = 0; q < 8; q++) out[q] = acc[q] * 0.125;
-            if (flip) { out[WU2] = -out[WU2]; out[WB2] = -out[WB2]; }
+            for (int q = 0; q < 8; q++) {
+                double acc = 0.0;
+#pragma unroll
+                for (int n = 0; n < 8; n++) acc += w_of(P, Wr, xf[n], q);
+                put(q, acc * 0.125);
+            }
         }
     }
-    double* Wb = P.W + (size_t)b * 8 * P.ncell;
-    const int c = cidx(i, j, k);
-#pragma unroll
-    for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + c] = out[q];
 }
+__device__ void ghost_cell(const KPS& P, int b, int i, int j, int k) { ghost_cell_rw(P, P.W, P.W, b, i, j, k); }
 
 __device__ __forceinline__ void bc_face(const DRL& v, int d, int& gi, int& gj, int& gk, double& sgn) {
     if (d == 1) {
@@ -219,7 +248,7 @@ __device__ __forceinline__ void bc_face(const DRL& v, int d, int& gi, int& gj, i
         if (gi < 0) gi = 0;
         if (gi >= v.nr) gi = v.nr - 1;
     }
-    gk = ((gk % v.nph) + v.nph) % v.nph;
+    if (gk < 0 || gk >= v.nph) gk = ((gk % v.nph) + v.nph) % v.nph;  // periodic in phi
 }
 // the stored field of face (d; gi, gj, gk) at level v from a leaf that has
 // it as an own face: the leaf of the cell on the face's high side, else the
@@ -312,13 +341,13 @@ __device__ double c2p_zone(const KPS& P, int b, int a, bool want_dt, int& nfail,
     const int gi = B.bi * BX + i, gj = B.bj * BY + j;
     const int c = cidx(i, j, k);
     const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
-    const Met<double> m = load_met<double>(v.gc, v.npt, p);
+    const Met<double> m = load_met<double>(gl(v.gc), v.npt, p);
     double* Ub = P.U + (size_t)b * 8 * P.ncell;
     double* Wb = P.W + (size_t)b * 8 * P.ncell;
     double u[8], w[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) u[q] = Ub[(size_t)q * P.ncell + c];
-    const double rfl = v.gc[(size_t)C_RFL * v.npt + p], efl = v.gc[(size_t)C_EFL * v.npt + p];
+    const double rfl = gl(v.gc)[(size_t)C_RFL * v.npt + p], efl = gl(v.gc)[(size_t)C_EFL * v.npt + p];
     // con2prim; a failed inversion (cold inversion or, for an unrecoverable
     // state, the floor state) always rewrites U, floors where they change W
     const int rc = c2p(m, R.gam, u, w);
@@ -336,7 +365,7 @@ __device__ double c2p_zone(const KPS& P, int b, int a, bool want_dt, int& nfail,
 #pragma unroll
     for (int q = 0; q < 8; q++) Wb[(size_t)q * P.ncell + c] = w[q];
     if (!want_dt) return 1e300;
-    const double dx[3] = {v.dr[gi + 2], v.dth, v.dph};
+    const double dx[3] = {gl(v.dr)[gi + 2], v.dth, v.dph};
     return cell_dt(m, R.gam, w, dx, P.bz == 1 ? 2 : 3);
 }
 
@@ -395,18 +424,17 @@ __device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int 
     const RunPar& R = P.run[v.run];
     const int gi = B.bi * BX + i, gj = B.bj * BY + j;
     const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
-    const double* tab = d == 0 ? v.gf : d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc;
+    const GLB double* tab = gl(d == 0 ? v.gf : d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc);
     // the face metric: of gamma^ij only gamma^nn enters the flux
     Met<Rt> m;
     {
         const long long np = v.npt;
         m.alpha = (Rt)tab[M_ALPHA * np + p];
         m.beta[0] = (Rt)tab[M_B1 * np + p];
-        m.beta[1] = (Rt)tab[M_B2 * np + p];
-        m.beta[2] = (Rt)tab[M_B3 * np + p];
+        m.beta[1] = m.beta[2] = Rt(0);
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            m.g[q] = (Rt)tab[(M_G11 + q) * np + p];
+            m.g[q] = (q == 1 || q == 4) ? Rt(0) : (Rt)tab[(M_G11 + q) * np + p];
             m.gi[q] = Rt(0);
         }
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
@@ -499,14 +527,13 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
     };
     const int gi = B.bi * BX + i;
     const long long np = v.npt;
-    auto load_m = [&](const double* tab, int p, Met<Rt>& m) {
+    auto load_m = [&](const GLB double* tab, int p, Met<Rt>& m) {
         m.alpha = (Rt)tab[M_ALPHA * np + p];
         m.beta[0] = (Rt)tab[M_B1 * np + p];
-        m.beta[1] = (Rt)tab[M_B2 * np + p];
-        m.beta[2] = (Rt)tab[M_B3 * np + p];
+        m.beta[1] = m.beta[2] = Rt(0);
 #pragma unroll
         for (int q = 0; q < 6; q++) {
-            m.g[q] = (Rt)tab[(M_G11 + q) * np + p];
+            m.g[q] = (q == 1 || q == 4) ? Rt(0) : (Rt)tab[(M_G11 + q) * np + p];
             m.gi[q] = Rt(0);
         }
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
@@ -514,7 +541,7 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         m.sg = (Rt)tab[M_SG * np + p];
     };
     Met<Rt> m;
-    if (d == 2) load_m(v.gc, (B.bj * BY + j0 + 2) * (v.nr + 5) + (gi + 2), m);
+    if (d == 2) load_m(gl(v.gc), (B.bj * BY + j0 + 2) * (v.nr + 5) + (gi + 2), m);
     // entering face f: prev = cell f-1, cur = cell f, sp = limited slope of cell f-1
     Rt prev[NV], cur[NV], sp[NV];
     {
@@ -545,7 +572,7 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         PL.B[t1] = L[5]; PR.B[t1] = Rr[5];
         PL.B[t2] = L[6]; PR.B[t2] = Rr[6];
         const int fj = d == 1 ? f : j0, fk = d == 1 ? k0 : f;
-        if (d == 1) load_m(v.gf + (size_t)NMET * v.npt, (B.bj * BY + fj + 2) * (v.nr + 5) + (gi + 2), m);
+        if (d == 1) load_m(gl(v.gf + (size_t)NMET * v.npt), (B.bj * BY + fj + 2) * (v.nr + 5) + (gi + 2), m);
         const int fx = fidx(d, i, fj, fk);
         const Rt bn = m.sg > Rt(0) ? (Rt)Bfd[fx] / m.sg : Rt(0);
         PL.B[d] = bn;
@@ -604,12 +631,12 @@ __device__ void face_update(const KPS& P, int b, int d, int i, int j, int k, dou
         const double a2 = ((double)emf_y<Rt>(P, Fb, i, j, k + 1) - (double)emf_y<Rt>(P, Fb, i, j, k));
         rhs = -a1 / v.dth + a2 / v.dph;
     } else if (d == 1) {
-        const double dri = v.dr[gi + 2];
+        const double dri = gl(v.dr)[gi + 2];
         const double a1 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k + 1) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
         const double a2 = ((double)emf_z<Rt>(P, Fb, gj0, v.nth, i + 1, j, k) - (double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
         rhs = -a1 / v.dph + a2 / dri;
     } else {
-        const double dri = v.dr[gi + 2];
+        const double dri = gl(v.dr)[gi + 2];
         const double a1 = ((double)emf_y<Rt>(P, Fb, i + 1, j, k) - (double)emf_y<Rt>(P, Fb, i, j, k));
         const double a2 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
         rhs = -a1 / dri + a2 / v.dth;
@@ -628,7 +655,7 @@ __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
     const int gi = B.bi * BX + i, gj = B.bj * BY + j;
     const int c = cidx(i, j, k);
     const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
-    const Met<double> m = load_met<double>(v.gc, v.npt, p);
+    const Met<double> m = load_met<double>(gl(v.gc), v.npt, p);
     double* Ub = P.U + (size_t)b * 8 * P.ncell;
     const double* Wb = P.W + (size_t)b * 8 * P.ncell;
     double* Usb = P.Us + (size_t)b * 5 * P.nint;
@@ -637,8 +664,8 @@ __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
 #pragma unroll
     for (int q = 0; q < 8; q++) w[q] = Wb[(size_t)q * P.ncell + c];
     double src[5];
-    source(v.gc, v.npt, p, m, R.gam, w, src);
-    const double dri = v.dr[gi + 2];
+    source(gl(v.gc), v.npt, p, m, R.gam, w, src);
+    const double dri = gl(v.dr)[gi + 2];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
         const double dx = ((double)Fat(P, Fb, 0, q, i + 1, j, k) - (double)Fat(P, Fb, 0, q, i, j, k)) / dri;
@@ -1106,10 +1133,10 @@ __global__ void k_diag_seq(const double* __restrict__ part, int nl, double* __re
 
 // ---------------------------------------------------- launch variants
 constexpr int NVAR = 6;
-constexpr int VAR_TB[NVAR] = {256, 256, 128, 128, 64, 64};
-constexpr int VAR_MW[NVAR] = {1, 4, 1, 4, 1, 4};
+constexpr int VAR_TB[NVAR] = {256, 256, 256, 128, 128, 128};
+constexpr int VAR_MW[NVAR] = {1, 3, 4, 1, 3, 4};
 
-#define SPH_VARIANTS(M) M(0, 256, 1) M(1, 256, 4) M(2, 128, 1) M(3, 128, 4) M(4, 64, 1) M(5, 64, 4)
+#define SPH_VARIANTS(M) M(0, 256, 1) M(1, 256, 3) M(2, 256, 4) M(3, 128, 1) M(4, 128, 3) M(5, 128, 4)
 
 // flux variants 0 .. 5: one face per thread; 6 .. 11: theta and phi faces
 // by lines (same workgroup shapes)
@@ -1212,7 +1239,7 @@ public:
     bool mixed = false;
     int mode = 0;                 // 0 multi-kernel, 1 persistent
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
-    int var_flux = 0, var_update = 0, var_c2p = 0;  // launch variants (VAR_TB, VAR_MW)
+    int var_flux = 0, var_update = 0, var_c2p = 1;  // launch variants (VAR_TB, VAR_MW)
     bool xcd_remap = true;        // XCD-contiguous workgroup numbering
     bool tables_dirty = true;
     std::string err;
@@ -1275,9 +1302,15 @@ public:
             for (int k = lo[2]; k <= hi[2]; k++)
                 for (int j = lo[1]; j <= hi[1]; j++)
                     for (int i = lo[0]; i <= hi[0]; i++) {
+                        const int c[3] = {i, j, k};
+                        // a face outside the block in both transverse
+                        // directions enters no EMF of an own face
+                        int out = 0;
+                        for (int e = 0; e < 3; e++)
+                            if (e != d) out += c[e] < 0 || c[e] >= n[e];
+                        if (out == 2) continue;
                         const int x = (d << 24) | pack_cell(i, j, k);
                         ff_.push_back(x);
-                        const int c[3] = {i, j, k};
                         bool own = true;
                         for (int e = 0; e < 3; e++) {
                             if (e == d) own = own && c[e] >= 0 && c[e] <= n[e];
@@ -1291,9 +1324,13 @@ public:
         // then the r faces
         std::vector<int> fl_;
         for (int k = -1; k <= L.bz; k++)
-            for (int i = -1; i <= BX; i++) fl_.push_back((1 << 26) | (1 << 24) | pack_cell(i, 0, k));
+            for (int i = -1; i <= BX; i++)
+                if (!((i < 0 || i >= BX) && (k < 0 || k >= L.bz)))
+                    fl_.push_back((1 << 26) | (1 << 24) | pack_cell(i, 0, k));
         for (int j = -1; j <= BY; j++)
-            for (int i = -1; i <= BX; i++) fl_.push_back((1 << 26) | (2 << 24) | pack_cell(i, j, 0));
+            for (int i = -1; i <= BX; i++)
+                if (!((i < 0 || i >= BX) && (j < 0 || j >= BY)))
+                    fl_.push_back((1 << 26) | (2 << 24) | pack_cell(i, j, 0));
         for (int x : ff_)
             if ((x >> 24) == 0) fl_.push_back(x);
         nflp = (int)fl_.size();

@@ -16,6 +16,7 @@
 #pragma once
 
 #include <cmath>
+#include <stdexcept>
 #include <vector>
 
 #include "sph_common.hpp"
@@ -111,6 +112,11 @@ inline KS ks_metric(double a, double Q, double r, double s, double c, bool pole)
 }
 
 inline void put_met(const KS& k, double* tab, long long npt, int p) {
+    // the physics skips these words (a block-diagonal spatial metric and a
+    // radial shift); they must be exact zeros
+    if (k.beta[1] != 0.0 || k.beta[2] != 0.0 || k.gam[1] != 0.0 || k.gam[4] != 0.0 || k.igam[1] != 0.0 ||
+        k.igam[4] != 0.0)
+        throw std::logic_error("metric table: gamma_12, gamma_23 or beta^theta, beta^phi not zero");
     tab[M_ALPHA * npt + p] = k.alpha;
     for (int d = 0; d < 3; d++) tab[(M_B1 + d) * npt + p] = k.beta[d];
     for (int q = 0; q < 6; q++) {
@@ -148,6 +154,9 @@ inline void build_level_tables(double a, double Q, double rho_fl, double eps_fl,
                 const double s = std::sin(th), c = std::cos(th);
                 const KS k = ks_metric(a, Q, rc, s, c, false);
                 put_met(k, gc.data(), npt, p);
+                for (int e = 0; e < 2; e++)
+                    if (k.dg[e][2] != 0.0 || k.dg[e][5] != 0.0 || k.dg[e][8] != 0.0)
+                        throw std::logic_error("metric table: derivative of g_t theta, g_r theta or g_theta phi not zero");
                 for (int q = 0; q < 10; q++) {
                     gc[(size_t)(C_DG1 + q) * npt + p] = k.dg[0][q];
                     gc[(size_t)(C_DG2 + q) * npt + p] = k.dg[1][q];

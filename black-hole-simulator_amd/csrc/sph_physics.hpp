@@ -40,29 +40,36 @@ struct Met {
     R alpha, beta[3], g[6], gi[6], sg;
 };
 
+// The Kerr-Schild (r, theta, phi) chart has a block-diagonal spatial metric:
+// theta decouples, so gamma_12 = gamma_23 = 0, gamma^12 = gamma^23 = 0, and
+// the shift is radial, beta^theta = beta^phi = 0.  The tables hold those words
+// (exact zeros, checked where the tables are built); the physics below
+// neither loads nor multiplies them.
+
 // metric words at point p of a table with npt points per word
 template <class R, class T>
 SPH_HD Met<R> load_met(const T* tab, long long npt, int p) {
     Met<R> m;
     m.alpha = (R)tab[M_ALPHA * npt + p];
     m.beta[0] = (R)tab[M_B1 * npt + p];
-    m.beta[1] = (R)tab[M_B2 * npt + p];
-    m.beta[2] = (R)tab[M_B3 * npt + p];
+    m.beta[1] = m.beta[2] = R(0);
 #pragma unroll
     for (int q = 0; q < 6; q++) {
-        m.g[q] = (R)tab[(M_G11 + q) * npt + p];
-        m.gi[q] = (R)tab[(M_I11 + q) * npt + p];
+        const bool zero = q == 1 || q == 4;
+        m.g[q] = zero ? R(0) : (R)tab[(M_G11 + q) * npt + p];
+        m.gi[q] = zero ? R(0) : (R)tab[(M_I11 + q) * npt + p];
     }
     m.sg = (R)tab[M_SG * npt + p];
     return m;
 }
 
-// lower an index with a symmetric 3x3 stored as 11 12 13 22 23 33
+// lower an index with a symmetric 3x3 stored as 11 12 13 22 23 33 whose 12
+// and 23 words are zero
 template <class R>
 SPH_HD void lower3(const R s[6], const R x[3], R y[3]) {
-    y[0] = s[0] * x[0] + s[1] * x[1] + s[2] * x[2];
-    y[1] = s[1] * x[0] + s[3] * x[1] + s[4] * x[2];
-    y[2] = s[2] * x[0] + s[4] * x[1] + s[5] * x[2];
+    y[0] = s[0] * x[0] + s[2] * x[2];
+    y[1] = s[3] * x[1];
+    y[2] = s[2] * x[0] + s[5] * x[2];
 }
 template <class R>
 SPH_HD R dot3(const R a[3], const R b[3]) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -123,9 +130,10 @@ SPH_HD void pt_cons(const Pt<R>& s, R c[5]) {
 // B^n word is zero.  c = pt_cons(s).
 template <class R>
 SPH_HD void pt_flux(const Met<R>& m, const Pt<R>& s, const R c[5], int n, R f[8]) {
-    R V[3];
-#pragma unroll
-    for (int d = 0; d < 3; d++) V[d] = m.alpha * s.v[d] - m.beta[d];
+    R V[3];  // alpha v^d - beta^d (radial shift)
+    V[0] = m.alpha * s.v[0] - m.beta[0];
+    V[1] = m.alpha * s.v[1];
+    V[2] = m.alpha * s.v[2];
     const R Vn = V[n];
     const R ab0 = s.W * s.Bv;  // alpha b^0
     const R Bn = s.B[n];
@@ -161,8 +169,13 @@ SPH_HD void pt_speeds(const Met<R>& m, const Pt<R>& s, R gam, int n, R& lm, R& l
     const R sq = sqrt(cf2 * disc);
     const R a = vn * (R(1) - cf2);
     const R ad = m.alpha / den;
-    lp = (a + sq) * ad - m.beta[n];
-    lm = (a - sq) * ad - m.beta[n];
+    if (n == 0) {
+        lp = (a + sq) * ad - m.beta[0];
+        lm = (a - sq) * ad - m.beta[0];
+    } else {
+        lp = (a + sq) * ad;
+        lm = (a - sq) * ad;
+    }
 }
 
 // One side of a face: the seven conserved words the HLL flux mixes (D,
@@ -407,37 +420,47 @@ SPH_HD void source(const Tb* gc, long long npt, int p, const Met<double>& m, dou
     Pt<double> s;
     s.rho = w[WRHO];
     s.p = (gam - 1.0) * w[WRHO] * w[WEPS];
+#pragma unroll
     for (int d = 0; d < 3; d++) { s.u[d] = w[WU1 + d]; s.B[d] = w[WB1 + d]; }
     pt_fill(m, gam, s);
     const double ia = 1.0 / m.alpha;
-    // four-velocity and magnetic four-vector, contravariant
+    // four-velocity and magnetic four-vector, contravariant (radial shift)
     double u[4], b[4];
     u[0] = s.W * ia;
-    for (int d = 0; d < 3; d++) u[1 + d] = s.W * (s.v[d] - m.beta[d] * ia);
+    u[1] = s.W * (s.v[0] - m.beta[0] * ia);
+    u[2] = s.W * s.v[1];
+    u[3] = s.W * s.v[2];
     b[0] = s.W * s.Bv * ia;
+#pragma unroll
     for (int d = 0; d < 3; d++) b[1 + d] = (s.B[d] + m.alpha * b[0] * u[1 + d]) / s.W;
-    // inverse four-metric
+    // inverse four-metric g^00 = -1/alpha^2, g^0i = beta^i/alpha^2,
+    // g^ij = gamma^ij - beta^i beta^j/alpha^2: the pairs 02, 03, 12, 23 vanish
     const double ia2 = ia * ia;
-    double gu[4][4];
-    gu[0][0] = -ia2;
-    for (int d = 0; d < 3; d++) gu[0][1 + d] = gu[1 + d][0] = m.beta[d] * ia2;
-    const int si[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
-    for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) gu[1 + i][1 + j] = m.gi[si[i][j]] - m.beta[i] * m.beta[j] * ia2;
-    const double wt = s.rhoh + s.b2;
-    // T^mn over the 10 independent pairs, off-diagonal ones doubled
-    const int pm[10] = {0, 0, 0, 0, 1, 1, 1, 2, 2, 3};
-    const int pn[10] = {0, 1, 2, 3, 1, 2, 3, 2, 3, 3};
+    const double g00 = -ia2, g01 = m.beta[0] * ia2, g11 = m.gi[0] - m.beta[0] * m.beta[0] * ia2;
+    const double g13 = m.gi[2], g22 = m.gi[3], g33 = m.gi[5];
+    const double wt = s.rhoh + s.b2, ps = s.ps;
+    // T^mn over the 10 independent pairs 00 01 02 03 11 12 13 22 23 33
     double Tm[10];
-    for (int q = 0; q < 10; q++) {
-        const int a = pm[q], c = pn[q];
-        Tm[q] = wt * u[a] * u[c] + s.ps * gu[a][c] - b[a] * b[c];
-    }
+    Tm[0] = wt * u[0] * u[0] + ps * g00 - b[0] * b[0];
+    Tm[1] = wt * u[0] * u[1] + ps * g01 - b[0] * b[1];
+    Tm[2] = wt * u[0] * u[2] - b[0] * b[2];
+    Tm[3] = wt * u[0] * u[3] - b[0] * b[3];
+    Tm[4] = wt * u[1] * u[1] + ps * g11 - b[1] * b[1];
+    Tm[5] = wt * u[1] * u[2] - b[1] * b[2];
+    Tm[6] = wt * u[1] * u[3] + ps * g13 - b[1] * b[3];
+    Tm[7] = wt * u[2] * u[2] + ps * g22 - b[2] * b[2];
+    Tm[8] = wt * u[2] * u[3] - b[2] * b[3];
+    Tm[9] = wt * u[3] * u[3] + ps * g33 - b[3] * b[3];
+    // contractions, off-diagonal pairs twice; d_j g_mn vanishes for the pairs
+    // 02, 12, 23 (g_t theta = g_r theta = g_theta phi = 0)
     double s1 = 0.0, s2 = 0.0, sg0 = 0.0;
+#pragma unroll
     for (int q = 0; q < 10; q++) {
-        const double wq = pm[q] == pn[q] ? 0.5 : 1.0;
-        s1 += wq * Tm[q] * (double)gc[(C_DG1 + q) * npt + p];
-        s2 += wq * Tm[q] * (double)gc[(C_DG2 + q) * npt + p];
+        const double wq = (q == 0 || q == 4 || q == 7 || q == 9) ? 0.5 : 1.0;
+        if (q != 2 && q != 5 && q != 8) {
+            s1 += wq * Tm[q] * (double)gc[(C_DG1 + q) * npt + p];
+            s2 += wq * Tm[q] * (double)gc[(C_DG2 + q) * npt + p];
+        }
         sg0 += (2.0 * wq) * Tm[q] * (double)gc[(C_GAM0 + q) * npt + p];
     }
     const double sqg = m.alpha * m.sg;
@@ -457,7 +480,11 @@ SPH_HD double cell_dt(const Met<double>& m, double gam, const double w[8], const
     for (int d = 0; d < 3; d++) { s.u[d] = w[WU1 + d]; s.B[d] = w[WB1 + d]; }
     pt_fill(m, gam, s);
     double dt = 1e300;
-    for (int d = 0; d < ndir; d++) {
+    // unrolled so that the direction index is a constant in each copy (a
+    // runtime index into the point's arrays would place them in memory)
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= ndir) break;
         double lm, lp;
         pt_speeds(m, s, gam, d, lm, lp);
         const double c = rmax(rabs(lm), rabs(lp));

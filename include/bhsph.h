@@ -137,7 +137,8 @@ int bhs_use_graph(bhs_engine* e, int enable);
  * also 6 .. 11: the theta and phi faces computed by lines of faces, one
  * thread per line): workgroup size and minimum wavefronts per SIMD of each
  * are given by bhs_variant_info.  Results do not depend on the variant; the
- * autotuner (bhfluid.tune) times them. */
+ * autotuner (bhfluid.tune) times them.  Defaults: flux 0, update 0,
+ * con2prim 1. */
 int bhs_set_variants(bhs_engine* e, int flux, int update, int c2p);
 int bhs_variant_info(int variant, int* tpb, int* min_waves);
 /* Number the workgroups of the multi-workgroup kernels so that each of the

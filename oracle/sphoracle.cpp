@@ -247,13 +247,21 @@ inline bool own_face(const Lay& L, int d, int i, int j, int k) {
 }
 
 // the face range a block computes fluxes on: along d [0, n_d], across
-// [-1, n_e]
+// [-1, n_e], without the faces outside the block in both transverse
+// directions (no EMF of an own face reads them; see corner())
 inline void flux_range(const Lay& L, int d, int lo[3], int hi[3]) {
     const int n[3] = {BX, BY, L.bz};
     for (int e = 0; e < 3; e++) {
         lo[e] = e == d ? 0 : -1;
         hi[e] = n[e];  // inclusive
     }
+}
+inline bool corner(const Lay& L, int d, int i, int j, int k) {
+    const int n[3] = {BX, BY, L.bz}, c[3] = {i, j, k};
+    int out = 0;
+    for (int e = 0; e < 3; e++)
+        if (e != d) out += c[e] < 0 || c[e] >= n[e];
+    return out == 2;
 }
 
 void ghosts(Run& R, const std::vector<double>& Bfr, std::vector<double>& Bfw) {
@@ -279,7 +287,7 @@ void ghosts(Run& R, const std::vector<double>& Bfr, std::vector<double>& Bfw) {
             for (int k = lo[2]; k <= hi[2]; k++)
                 for (int j = lo[1]; j <= hi[1]; j++)
                     for (int i = lo[0]; i <= hi[0]; i++) {
-                        if (own_face(L, d, i, j, k)) continue;
+                        if (own_face(L, d, i, j, k) || corner(L, d, i, j, k)) continue;
                         ftmp[((size_t)b * 3 + d) * L.nf + fidx(d, i, j, k)] = ghost_face(R, Bfr, b, d, i, j, k);
                     }
         }
@@ -298,7 +306,7 @@ void ghosts(Run& R, const std::vector<double>& Bfr, std::vector<double>& Bfw) {
             for (int k = lo[2]; k <= hi[2]; k++)
                 for (int j = lo[1]; j <= hi[1]; j++)
                     for (int i = lo[0]; i <= hi[0]; i++) {
-                        if (own_face(L, d, i, j, k)) continue;
+                        if (own_face(L, d, i, j, k) || corner(L, d, i, j, k)) continue;
                         const size_t x = ((size_t)b * 3 + d) * L.nf + fidx(d, i, j, k);
                         Bfw[x] = ftmp[x];
                     }
@@ -321,6 +329,7 @@ void fluxes(Run& R, const std::vector<double>& Bf) {
             for (int k = lo[2]; k <= hi[2]; k++)
                 for (int j = lo[1]; j <= hi[1]; j++)
                     for (int i = lo[0]; i <= hi[0]; i++) {
+                        if (corner(L, d, i, j, k)) continue;
                         const int gi = R.bi[b] * BX + i, gj = R.bj[b] * BY + j;
                         const double* tab = d == 0 ? v.gf.data() : d == 1 ? v.gf.data() + (size_t)NMET * v.npt()
                                                                             : v.gc.data();
