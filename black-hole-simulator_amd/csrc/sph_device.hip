@@ -109,6 +109,13 @@ struct KPS {
     // persistent schedule
     unsigned int* qctr;   // [0] stage-A queue, [1] barrier arrivals, [2] stage-B queue, [3] timeout flag
     int xcd;              // 1: renumber workgroups so that each XCD takes a contiguous range
+    // Side faces: a block computes fluxes on the faces one layer outside it
+    // (for the EMFs of its boundary edges).  The flux of a face depends only
+    // on its level and global position, so where the block across that
+    // side is a leaf of the same level, which computes the same face as its
+    // own, the block skips it and the EMFs read the neighbour's word.
+    const int* side;      // [slot][6]: same-level leaf across r-, r+, theta-, theta+, phi-, phi+ (-1: none)
+    int reuse;            // 1: skip and share side faces (the sub-phase schedule)
 };
 
 // Workgroups are dispatched round-robin over the eight XCDs (workgroup x to
@@ -123,6 +130,20 @@ __device__ __forceinline__ int wg_index(const KPS& P) {
     const int q = n / NXCD, r = n % NXCD;
     const int k = x % NXCD, loc = x / NXCD;
     return k * q + (k < r ? k : r) + loc;
+}
+
+// the side (0..5) a flux face of direction d lies outside the block on, or -1
+__device__ __forceinline__ int face_side_of(const KPS& P, int d, int i, int j, int k) {
+    if (d != 0) { if (i < 0) return 0; if (i >= BX) return 1; }
+    if (d != 1) { if (j < 0) return 2; if (j >= BY) return 3; }
+    if (d != 2) { if (k < 0) return 4; if (k >= P.bz) return 5; }
+    return -1;
+}
+// true where block b leaves face (d; i, j, k) to its same-level neighbour
+__device__ __forceinline__ bool face_shared(const KPS& P, int b, int d, int i, int j, int k) {
+    if (!P.reuse) return false;
+    const int sd = face_side_of(P, d, i, j, k);
+    return sd >= 0 && P.side[(size_t)b * 6 + sd] >= 0;
 }
 
 __device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -597,24 +618,56 @@ template <class Rt>
 __device__ __forceinline__ Rt Fat(const KPS& P, const Rt* Fb, int d, int q, int i, int j, int k) {
     return Fb[((size_t)d * NFW + q) * P.nf + fidx(d, i, j, k)];
 }
+// The same-level leaves across the six sides of a block, where the block
+// shares its side faces with them (else -1).
+struct SideNb {
+    int s[6];
+};
+__device__ __forceinline__ SideNb side_nb(const KPS& P, int b) {
+    SideNb n;
+#pragma unroll
+    for (int e = 0; e < 6; e++) n.s[e] = P.reuse ? P.side[(size_t)b * 6 + e] : -1;
+    return n;
+}
+// flux word of face (d; i, j, k) of block b, from the same-level neighbour
+// where b left that side face to it (at most one transverse coordinate is
+// outside the block: corner faces are never read)
 template <class Rt>
-__device__ __forceinline__ Rt emf_z(const KPS& P, const Rt* Fb, int gj0, int nth, int i, int j, int k) {
+__device__ __forceinline__ Rt Fsh(const KPS& P, int b, const SideNb& n, int d, int q, int i, int j, int k) {
+    int s = b;
+    if (d != 0) {
+        if (i < 0 && n.s[0] >= 0) { s = n.s[0]; i += BX; }
+        else if (i >= BX && n.s[1] >= 0) { s = n.s[1]; i -= BX; }
+    }
+    if (d != 1) {
+        if (j < 0 && n.s[2] >= 0) { s = n.s[2]; j += BY; }
+        else if (j >= BY && n.s[3] >= 0) { s = n.s[3]; j -= BY; }
+    }
+    if (d != 2) {
+        if (k < 0 && n.s[4] >= 0) { s = n.s[4]; k += P.bz; }
+        else if (k >= P.bz && n.s[5] >= 0) { s = n.s[5]; k -= P.bz; }
+    }
+    return ((const Rt*)P.F)[(((size_t)s * 3 + d) * NFW + q) * P.nf + fidx(d, i, j, k)];
+}
+
+template <class Rt>
+__device__ __forceinline__ Rt emf_z(const KPS& P, int b, const SideNb& n, int gj0, int nth, int i, int j, int k) {
     const int gj = gj0 + j;
     if (gj == 0 || gj == nth) return Rt(0);
-    return Rt(0.25) * (Fat(P, Fb, 1, 6, i, j, k) + Fat(P, Fb, 1, 6, i - 1, j, k) - Fat(P, Fb, 0, 5, i, j, k) -
-                       Fat(P, Fb, 0, 5, i, j - 1, k));
+    return Rt(0.25) * (Fsh<Rt>(P, b, n, 1, 6, i, j, k) + Fsh<Rt>(P, b, n, 1, 6, i - 1, j, k) - Fsh<Rt>(P, b, n, 0, 5, i, j, k) -
+                       Fsh<Rt>(P, b, n, 0, 5, i, j - 1, k));
 }
 template <class Rt>
-__device__ __forceinline__ Rt emf_x(const KPS& P, const Rt* Fb, int gj0, int nth, int i, int j, int k) {
+__device__ __forceinline__ Rt emf_x(const KPS& P, int b, const SideNb& n, int gj0, int nth, int i, int j, int k) {
     const int gj = gj0 + j;
     if (gj == 0 || gj == nth) return Rt(0);
-    return Rt(0.25) * (Fat(P, Fb, 2, 6, i, j, k) + Fat(P, Fb, 2, 6, i, j - 1, k) - Fat(P, Fb, 1, 5, i, j, k) -
-                       Fat(P, Fb, 1, 5, i, j, k - 1));
+    return Rt(0.25) * (Fsh<Rt>(P, b, n, 2, 6, i, j, k) + Fsh<Rt>(P, b, n, 2, 6, i, j - 1, k) - Fsh<Rt>(P, b, n, 1, 5, i, j, k) -
+                       Fsh<Rt>(P, b, n, 1, 5, i, j, k - 1));
 }
 template <class Rt>
-__device__ __forceinline__ Rt emf_y(const KPS& P, const Rt* Fb, int i, int j, int k) {
-    return Rt(0.25) * (Fat(P, Fb, 0, 6, i, j, k) + Fat(P, Fb, 0, 6, i, j, k - 1) - Fat(P, Fb, 2, 5, i, j, k) -
-                       Fat(P, Fb, 2, 5, i - 1, j, k));
+__device__ __forceinline__ Rt emf_y(const KPS& P, int b, const SideNb& n, int i, int j, int k) {
+    return Rt(0.25) * (Fsh<Rt>(P, b, n, 0, 6, i, j, k) + Fsh<Rt>(P, b, n, 0, 6, i, j, k - 1) - Fsh<Rt>(P, b, n, 2, 5, i, j, k) -
+                       Fsh<Rt>(P, b, n, 2, 5, i - 1, j, k));
 }
 
 // own face (d; i, j, k) of block b: face field of the stage
@@ -622,23 +675,23 @@ template <class Rt>
 __device__ void face_update(const KPS& P, int b, int d, int i, int j, int k, double dt, int stage) {
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
-    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    const SideNb n = side_nb(P, b);
     const int gj0 = B.bj * BY;
     const int gi = B.bi * BX + i;
     double rhs;
     if (d == 0) {
-        const double a1 = ((double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - (double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
-        const double a2 = ((double)emf_y<Rt>(P, Fb, i, j, k + 1) - (double)emf_y<Rt>(P, Fb, i, j, k));
+        const double a1 = ((double)emf_z<Rt>(P, b, n, gj0, v.nth, i, j + 1, k) - (double)emf_z<Rt>(P, b, n, gj0, v.nth, i, j, k));
+        const double a2 = ((double)emf_y<Rt>(P, b, n, i, j, k + 1) - (double)emf_y<Rt>(P, b, n, i, j, k));
         rhs = -a1 / v.dth + a2 / v.dph;
     } else if (d == 1) {
         const double dri = gl(v.dr)[gi + 2];
-        const double a1 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k + 1) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
-        const double a2 = ((double)emf_z<Rt>(P, Fb, gj0, v.nth, i + 1, j, k) - (double)emf_z<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a1 = ((double)emf_x<Rt>(P, b, n, gj0, v.nth, i, j, k + 1) - (double)emf_x<Rt>(P, b, n, gj0, v.nth, i, j, k));
+        const double a2 = ((double)emf_z<Rt>(P, b, n, gj0, v.nth, i + 1, j, k) - (double)emf_z<Rt>(P, b, n, gj0, v.nth, i, j, k));
         rhs = -a1 / v.dph + a2 / dri;
     } else {
         const double dri = gl(v.dr)[gi + 2];
-        const double a1 = ((double)emf_y<Rt>(P, Fb, i + 1, j, k) - (double)emf_y<Rt>(P, Fb, i, j, k));
-        const double a2 = ((double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j + 1, k) - (double)emf_x<Rt>(P, Fb, gj0, v.nth, i, j, k));
+        const double a1 = ((double)emf_y<Rt>(P, b, n, i + 1, j, k) - (double)emf_y<Rt>(P, b, n, i, j, k));
+        const double a2 = ((double)emf_x<Rt>(P, b, n, gj0, v.nth, i, j + 1, k) - (double)emf_x<Rt>(P, b, n, gj0, v.nth, i, j, k));
         rhs = -a1 / dri + a2 / v.dth;
     }
     const size_t x = ((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k);
@@ -726,6 +779,8 @@ __global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
     int i, j, k;
     const int x = P.ghf[t];
     unpack(x, i, j, k);
+    // a shared side face's field is read only by the flux the neighbour computes
+    if (face_shared(P, b, x >> 24, i, j, k)) return;
     ghost_face(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
 }
 
@@ -743,6 +798,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     unpack(x, i, j, k);
     const int d = (x >> 24) & 3;
     const double* Bf = buf ? P.Bf1 : P.Bf0;
+    if (face_shared(P, b, d, i, j, k)) return;  // a line lies on one side
     if (x & (1 << 26)) {
         if (d == 1) flux_pencil<Rt, 1>(P, Bf, b, i, k);
         else flux_pencil<Rt, 2>(P, Bf, b, i, j);
@@ -761,6 +817,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     int i, j, k;
     const int x = P.flf[t];
     unpack(x, i, j, k);
+    if (face_shared(P, b, x >> 24, i, j, k)) return;
     flux_face<Rt>(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
 }
 
@@ -1218,12 +1275,15 @@ public:
     hipStream_t stream = nullptr;
     DB<double> U, W, Us, Bf0, Bf1, F, icwork;  // icwork: initial-data work array
     DB<Blk> blk;
+    DB<int> side;                 // [slot][6] same-level leaf across each side, -1
+    std::vector<int> hside;
     DB<DRL> rl;
     DB<RunPar> runp;
     DB<RunCtr> ctr;
     DB<int> ghc, ghf, flf, owf;
     int nghc = 0, nghf = 0, nflf = 0, nowf = 0, nflp = 0;
     DB<int> flp;
+    std::vector<char> own_mask;   // [3][nf]: own faces (the evolved face field)
     DB<int> list;                 // active block slots (sorted by run)
     DB<int> runlist;              // active runs
     DB<unsigned int> qctr;
@@ -1270,6 +1330,8 @@ public:
         SCHK(hipMemsetAsync(U.p, 0, U.n * sizeof(double), stream));
         blk.alloc(cap);
         hblk.assign(cap, Blk{-1, 0, 0, 0});
+        side.alloc((size_t)cap * 6);
+        hside.assign((size_t)cap * 6, -1);
         for (long long s = cap - 1; s >= 0; s--) free_slots.push_back((int)s);
         qctr.alloc(4);
         SCHK(hipMemsetAsync(qctr.p, 0, 4 * sizeof(unsigned), stream));
@@ -1296,6 +1358,10 @@ public:
             for (int j = -G; j < BY + G; j++)
                 for (int i = -G; i < BX + G; i++)
                     if (!(i >= 0 && i < BX && j >= 0 && j < BY && k >= 0 && k < L.bz)) gc_.push_back(pack_cell(i, j, k));
+        // flux faces: the own faces of the three directions, then the side
+        // faces grouped by direction and side (a block that shares a side
+        // skips whole wavefronts of it)
+        std::vector<int> sides[3][6];
         for (int d = 0; d < 3; d++) {
             int lo[3], hi[3];
             for (int e = 0; e < 3; e++) { lo[e] = e == d ? 0 : -1; hi[e] = n[e]; }
@@ -1305,32 +1371,50 @@ public:
                         const int c[3] = {i, j, k};
                         // a face outside the block in both transverse
                         // directions enters no EMF of an own face
-                        int out = 0;
+                        int out = 0, sd = -1;
                         for (int e = 0; e < 3; e++)
-                            if (e != d) out += c[e] < 0 || c[e] >= n[e];
+                            if (e != d && (c[e] < 0 || c[e] >= n[e])) {
+                                out++;
+                                sd = 2 * e + (c[e] < 0 ? 0 : 1);
+                            }
                         if (out == 2) continue;
                         const int x = (d << 24) | pack_cell(i, j, k);
-                        ff_.push_back(x);
-                        bool own = true;
-                        for (int e = 0; e < 3; e++) {
-                            if (e == d) own = own && c[e] >= 0 && c[e] <= n[e];
-                            else own = own && c[e] >= 0 && c[e] < n[e];
+                        if (out == 0) {
+                            ow_.push_back(x);
+                            ff_.push_back(x);
+                        } else {
+                            sides[d][sd].push_back(x);
                         }
-                        if (own) ow_.push_back(x);
-                        else gf_.push_back(x);
                     }
         }
-        // theta and phi face lines (transverse ranges as the face ranges),
-        // then the r faces
-        std::vector<int> fl_;
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 6; e++) {
+                ff_.insert(ff_.end(), sides[d][e].begin(), sides[d][e].end());
+                gf_.insert(gf_.end(), sides[d][e].begin(), sides[d][e].end());
+            }
+        own_mask.assign((size_t)3 * L.nf, 0);
+        for (int x : ow_) {
+            int i, j, k;
+            const int d = x >> 24;
+            i = (x & 0xff) - 2; j = ((x >> 8) & 0xff) - 2; k = ((x >> 16) & 0xff) - 2;
+            own_mask[(size_t)d * L.nf + fidx(d, i, j, k)] = 1;
+        }
+        // theta and phi face lines (transverse ranges as the face ranges;
+        // those inside the block first), then the r faces in the order above
+        std::vector<int> fl_, fls_;
         for (int k = -1; k <= L.bz; k++)
-            for (int i = -1; i <= BX; i++)
-                if (!((i < 0 || i >= BX) && (k < 0 || k >= L.bz)))
-                    fl_.push_back((1 << 26) | (1 << 24) | pack_cell(i, 0, k));
+            for (int i = -1; i <= BX; i++) {
+                const int oi = i < 0 || i >= BX, ok = k < 0 || k >= L.bz;
+                if (oi && ok) continue;
+                ((oi || ok) ? fls_ : fl_).push_back((1 << 26) | (1 << 24) | pack_cell(i, 0, k));
+            }
         for (int j = -1; j <= BY; j++)
-            for (int i = -1; i <= BX; i++)
-                if (!((i < 0 || i >= BX) && (j < 0 || j >= BY)))
-                    fl_.push_back((1 << 26) | (2 << 24) | pack_cell(i, j, 0));
+            for (int i = -1; i <= BX; i++) {
+                const int oi = i < 0 || i >= BX, oj = j < 0 || j >= BY;
+                if (oi && oj) continue;
+                ((oi || oj) ? fls_ : fl_).push_back((1 << 26) | (2 << 24) | pack_cell(i, j, 0));
+            }
+        fl_.insert(fl_.end(), fls_.begin(), fls_.end());
         for (int x : ff_)
             if ((x >> 24) == 0) fl_.push_back(x);
         nflp = (int)fl_.size();
@@ -1354,6 +1438,8 @@ public:
         P.flp = flp.p; P.nflp = nflp;
         P.qctr = qctr.p;
         P.xcd = xcd_remap ? 1 : 0;
+        P.side = side.p;
+        P.reuse = mode == 0 ? 1 : 0;
         return P;
     }
 
@@ -1463,6 +1549,20 @@ public:
         R.par.rl0 = R.rls[0];
         for (int q = 0; q < nb; q++)
             hblk[R.slots[q]] = Blk{R.rls[blocks[4 * q]], blocks[4 * q + 1], blocks[4 * q + 2], blocks[4 * q + 3]};
+        // the same-level leaf across each side (r and theta: none beyond the
+        // domain or the axis; phi: periodic)
+        for (int q = 0; q < nb; q++) {
+            const int l = blocks[4 * q], bi = blocks[4 * q + 1], bj = blocks[4 * q + 2], bk = blocks[4 * q + 3];
+            const DRL& v = lv[l];
+            auto at = [&](int i, int j, int k) { return lut[l][((size_t)k * v.nbth + j) * v.nbr + i]; };
+            int* sd = &hside[(size_t)R.slots[q] * 6];
+            sd[0] = bi > 0 ? at(bi - 1, bj, bk) : -1;
+            sd[1] = bi + 1 < v.nbr ? at(bi + 1, bj, bk) : -1;
+            sd[2] = bj > 0 ? at(bi, bj - 1, bk) : -1;
+            sd[3] = bj + 1 < v.nbth ? at(bi, bj + 1, bk) : -1;
+            sd[4] = at(bi, bj, (bk + v.nbph - 1) % v.nbph);
+            sd[5] = at(bi, bj, (bk + 1) % v.nbph);
+        }
         tables_dirty = true;
         return id;
     }
@@ -1529,6 +1629,7 @@ public:
         SCHK(hipStreamSynchronize(stream));
         for (int s : R.slots) {
             hblk[s] = Blk{-1, 0, 0, 0};
+            for (int e = 0; e < 6; e++) hside[(size_t)s * 6 + e] = -1;
             free_slots.push_back(s);
         }
         for (int x : R.rls) hrl[x].run = -1;
@@ -1541,6 +1642,7 @@ public:
         if (!tables_dirty) return;
         SCHK(hipStreamSynchronize(stream));
         blk.upload(hblk.data(), hblk.size(), stream);
+        side.upload(hside.data(), hside.size(), stream);
         rl.upload(hrl.data(), std::max<size_t>(1, hrl.size()), stream);
         std::vector<RunPar> rp(std::max<size_t>(1, runs.size()));
         for (size_t r = 0; r < runs.size(); r++) rp[r] = runs[r].par;
@@ -1776,6 +1878,12 @@ public:
             if (hB) SCHK(hipMemcpyAsync(hB + q * 3 * L.nf, Bf0.p + (size_t)s * 3 * L.nf, 3 * L.nf * sizeof(double), hipMemcpyDeviceToHost, stream));
         }
         sync();
+        // the face field of a block is its own faces; the others are ghost
+        // faces, refilled from their owners every stage, and are exported as 0
+        if (hB)
+            for (size_t q = 0; q < R.slots.size(); q++)
+                for (size_t x = 0; x < (size_t)3 * L.nf; x++)
+                    if (!own_mask[x]) hB[q * 3 * L.nf + x] = 0.0;
     }
     void set_state(int r, const double* hU, const double* hW, const double* hB) {
         const HostRun& R = runs.at(r);

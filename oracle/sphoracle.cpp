@@ -766,7 +766,20 @@ void os_get_state(void* r, double* U, double* W, double* Bf) {
     Run* R = (Run*)r;
     if (U) std::memcpy(U, R->U.data(), R->U.size() * sizeof(double));
     if (W) std::memcpy(W, R->W.data(), R->W.size() * sizeof(double));
-    if (Bf) std::memcpy(Bf, R->Bf0.data(), R->Bf0.size() * sizeof(double));
+    if (Bf) {
+        // own faces only (ghost faces are refilled every stage), as the engine exports them
+        const Lay& L = R->L;
+        std::memcpy(Bf, R->Bf0.data(), R->Bf0.size() * sizeof(double));
+        std::vector<char> own((size_t)3 * L.nf, 0);
+        const int n[3] = {BX, BY, L.bz};
+        for (int d = 0; d < 3; d++)
+            for (int k = 0; k < n[2] + (d == 2); k++)
+                for (int j = 0; j < n[1] + (d == 1); j++)
+                    for (int i = 0; i < n[0] + (d == 0); i++) own[(size_t)d * L.nf + fidx(d, i, j, k)] = 1;
+        for (int b = 0; b < R->nb(); b++)
+            for (size_t x = 0; x < own.size(); x++)
+                if (!own[x]) Bf[(size_t)b * 3 * L.nf + x] = 0.0;
+    }
 }
 void os_set_state(void* r, const double* U, const double* W, const double* Bf) {
     Run* R = (Run*)r;
