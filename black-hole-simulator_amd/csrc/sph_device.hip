@@ -146,6 +146,38 @@ __device__ __forceinline__ bool face_shared(const KPS& P, int b, int d, int i, i
     return sd >= 0 && P.side[(size_t)b * 6 + sd] >= 0;
 }
 
+// The same-level leaves across the six sides of a block, where the block
+// shares its side faces with them (else -1).
+struct SideNb {
+    int s[6];
+};
+__device__ __forceinline__ SideNb side_nb(const KPS& P, int b) {
+    SideNb n;
+#pragma unroll
+    for (int e = 0; e < 6; e++) n.s[e] = P.reuse ? P.side[(size_t)b * 6 + e] : -1;
+    return n;
+}
+// whether any flux stencil reads ghost cell (i, j, k) of a block with
+// sides n: corner cells (three coordinates outside) never; a slab cell
+// only where its side is not shared; an edge cell (two outside) only where
+// one of its two sides is not shared (the side faces computed there read it)
+__device__ __forceinline__ bool ghost_read(const KPS& P, const SideNb& n, int i, int j, int k) {
+    const int si = i < 0 ? 0 : (i >= BX ? 1 : -1);
+    const int sj = j < 0 ? 2 : (j >= BY ? 3 : -1);
+    const int sk = k < 0 ? 4 : (k >= P.bz ? 5 : -1);
+    const int nout = (si >= 0) + (sj >= 0) + (sk >= 0);
+    if (nout == 3) return false;
+    auto shared = [&](int e) {
+        return e == 0 ? n.s[0] >= 0 : e == 1 ? n.s[1] >= 0 : e == 2 ? n.s[2] >= 0
+             : e == 3 ? n.s[3] >= 0 : e == 4 ? n.s[4] >= 0 : n.s[5] >= 0;
+    };
+    bool all = true;
+    if (si >= 0) all = all && shared(si);
+    if (sj >= 0) all = all && shared(sj);
+    if (sk >= 0) all = all && shared(sk);
+    return !all;
+}
+
 __device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
 
 __device__ __forceinline__ int leaf_at(const KPS& P, const DRL& v, int gi, int gj, int gk) {
@@ -464,14 +496,31 @@ __device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int 
     }
     constexpr int o0 = d == 0, o1 = d == 1, o2 = d == 2;
     constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
-    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
-    int cc[4];
+    // each stencil cell's word 0, in this block or, for an own face whose
+    // stencil crosses a shared side, in the same-level neighbour (a side
+    // face computed here lies on an unshared side: all its cells are local)
+    const double* Wc[4];
+    {
+        const SideNb n = side_nb(P, b);
+        const int N = d == 0 ? BX : d == 1 ? BY : P.bz;
+        const bool own = (d == 0 || (i >= 0 && i < BX)) && (d == 1 || (j >= 0 && j < BY)) &&
+                         (d == 2 || (k >= 0 && k < P.bz));
+        const int lo = n.s[2 * d], hi = n.s[2 * d + 1];
+        const int a0 = d == 0 ? i : d == 1 ? j : k;
 #pragma unroll
-    for (int c = 0; c < 4; c++) cc[c] = cidx(i + (c - 2) * o0, j + (c - 2) * o1, k + (c - 2) * o2);
+        for (int c = 0; c < 4; c++) {
+            int s = b, a = a0 + c - 2;
+            if (own) {
+                if (a < 0 && lo >= 0) { s = lo; a += N; }
+                else if (a >= N && hi >= 0) { s = hi; a -= N; }
+            }
+            Wc[c] = P.W + (size_t)s * 8 * P.ncell + (d == 0 ? cidx(a, j, k) : d == 1 ? cidx(i, a, k) : cidx(i, j, a));
+        }
+    }
     // one variable at a time: four stencil loads, then its two face values
     auto ld = [&](int q, Rt x[4]) {
 #pragma unroll
-        for (int c = 0; c < 4; c++) x[c] = (Rt)Wb[(size_t)q * P.ncell + cc[c]];
+        for (int c = 0; c < 4; c++) x[c] = (Rt)Wc[c][(size_t)q * P.ncell];
     };
     Pt<Rt> PL, PR;
     {
@@ -531,20 +580,27 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
     const RunPar& R = P.run[v.run];
     const int n = d == 1 ? BY : P.bz;
     const int j0 = d == 1 ? 0 : jk, k0 = d == 1 ? jk : 0;
-    const double* Wb = P.W + (size_t)b * 8 * P.ncell;
+    // an own line (transverse position inside the block) reads the cells
+    // beyond a shared side in the same-level neighbour; a side line is local
+    const SideNb nbs = side_nb(P, b);
+    const bool own = i >= 0 && i < BX && (d == 1 ? (k0 >= 0 && k0 < P.bz) : (j0 >= 0 && j0 < BY));
+    const int slo = own ? nbs.s[2 * d] : -1, shi = own ? nbs.s[2 * d + 1] : -1;
     const Rt gm1 = (Rt)(R.gam - 1.0);
     const Rt gam = (Rt)R.gam;
     auto cell = [&](int c, Rt w[NV]) {
-        const int cc = d == 1 ? cidx(i, c, k0) : cidx(i, j0, c);
-        const Rt rho = (Rt)Wb[(size_t)WRHO * P.ncell + cc];
-        const Rt eps = (Rt)Wb[(size_t)WEPS * P.ncell + cc];
+        int s = b;
+        if (c < 0 && slo >= 0) { s = slo; c += n; }
+        else if (c >= n && shi >= 0) { s = shi; c -= n; }
+        const double* Wb = P.W + (size_t)s * 8 * P.ncell + (d == 1 ? cidx(i, c, k0) : cidx(i, j0, c));
+        const Rt rho = (Rt)Wb[(size_t)WRHO * P.ncell];
+        const Rt eps = (Rt)Wb[(size_t)WEPS * P.ncell];
         w[0] = rho;
         w[1] = gm1 * rho * eps;
-        w[2] = (Rt)Wb[(size_t)WU1 * P.ncell + cc];
-        w[3] = (Rt)Wb[(size_t)WU2 * P.ncell + cc];
-        w[4] = (Rt)Wb[(size_t)WU3 * P.ncell + cc];
-        w[5] = (Rt)Wb[(size_t)(WB1 + t1) * P.ncell + cc];
-        w[6] = (Rt)Wb[(size_t)(WB1 + t2) * P.ncell + cc];
+        w[2] = (Rt)Wb[(size_t)WU1 * P.ncell];
+        w[3] = (Rt)Wb[(size_t)WU2 * P.ncell];
+        w[4] = (Rt)Wb[(size_t)WU3 * P.ncell];
+        w[5] = (Rt)Wb[(size_t)(WB1 + t1) * P.ncell];
+        w[6] = (Rt)Wb[(size_t)(WB1 + t2) * P.ncell];
     };
     const int gi = B.bi * BX + i;
     const long long np = v.npt;
@@ -617,17 +673,6 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
 template <class Rt>
 __device__ __forceinline__ Rt Fat(const KPS& P, const Rt* Fb, int d, int q, int i, int j, int k) {
     return Fb[((size_t)d * NFW + q) * P.nf + fidx(d, i, j, k)];
-}
-// The same-level leaves across the six sides of a block, where the block
-// shares its side faces with them (else -1).
-struct SideNb {
-    int s[6];
-};
-__device__ __forceinline__ SideNb side_nb(const KPS& P, int b) {
-    SideNb n;
-#pragma unroll
-    for (int e = 0; e < 6; e++) n.s[e] = P.reuse ? P.side[(size_t)b * 6 + e] : -1;
-    return n;
 }
 // flux word of face (d; i, j, k) of block b, from the same-level neighbour
 // where b left that side face to it (at most one transverse coordinate is
@@ -767,6 +812,7 @@ __global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
     if (t >= P.nghc) return;
     int i, j, k;
     unpack(P.ghc[t], i, j, k);
+    if (!ghost_read(P, side_nb(P, b), i, j, k)) return;
     ghost_cell(P, b, i, j, k);
 }
 
@@ -897,7 +943,7 @@ __global__ __launch_bounds__(TPB) void k_stage(KPS P, int stage) {
         for (int t = threadIdx.x; t < P.nghc; t += TPB) {
             int i, j, k;
             unpack(P.ghc[t], i, j, k);
-            ghost_cell(P, b, i, j, k);
+            if (ghost_read(P, side_nb(P, b), i, j, k)) ghost_cell(P, b, i, j, k);
         }
         for (int t = threadIdx.x; t < P.nghf; t += TPB) {
             int i, j, k;
@@ -1560,8 +1606,10 @@ public:
             sd[1] = bi + 1 < v.nbr ? at(bi + 1, bj, bk) : -1;
             sd[2] = bj > 0 ? at(bi, bj - 1, bk) : -1;
             sd[3] = bj + 1 < v.nbth ? at(bi, bj + 1, bk) : -1;
-            sd[4] = at(bi, bj, (bk + v.nbph - 1) % v.nbph);
-            sd[5] = at(bi, bj, (bk + 1) % v.nbph);
+            // phi sides of an axisymmetric block (one cell deep) are its own
+            // single layer; a stencil two cells deep cannot be mapped into it
+            sd[4] = L.bz > 2 ? at(bi, bj, (bk + v.nbph - 1) % v.nbph) : -1;
+            sd[5] = L.bz > 2 ? at(bi, bj, (bk + 1) % v.nbph) : -1;
         }
         tables_dirty = true;
         return id;
@@ -1878,12 +1926,24 @@ public:
             if (hB) SCHK(hipMemcpyAsync(hB + q * 3 * L.nf, Bf0.p + (size_t)s * 3 * L.nf, 3 * L.nf * sizeof(double), hipMemcpyDeviceToHost, stream));
         }
         sync();
-        // the face field of a block is its own faces; the others are ghost
-        // faces, refilled from their owners every stage, and are exported as 0
-        if (hB)
-            for (size_t q = 0; q < R.slots.size(); q++)
+        // the state of a block is its active zones and its own faces; ghost
+        // cells and ghost faces are refilled from their owners every stage
+        // (as far as the step reads them) and are exported as 0
+        for (size_t q = 0; q < R.slots.size(); q++) {
+            for (int k = -G; k < L.bz + G; k++)
+                for (int j = -G; j < BY + G; j++)
+                    for (int i = -G; i < BX + G; i++) {
+                        if (i >= 0 && i < BX && j >= 0 && j < BY && k >= 0 && k < L.bz) continue;
+                        for (int w = 0; w < 8; w++) {
+                            const size_t x = (q * 8 + w) * L.ncell + cidx(i, j, k);
+                            if (hU) hU[x] = 0.0;
+                            if (hW) hW[x] = 0.0;
+                        }
+                    }
+            if (hB)
                 for (size_t x = 0; x < (size_t)3 * L.nf; x++)
                     if (!own_mask[x]) hB[q * 3 * L.nf + x] = 0.0;
+        }
     }
     void set_state(int r, const double* hU, const double* hW, const double* hB) {
         const HostRun& R = runs.at(r);

@@ -766,6 +766,22 @@ void os_get_state(void* r, double* U, double* W, double* Bf) {
     Run* R = (Run*)r;
     if (U) std::memcpy(U, R->U.data(), R->U.size() * sizeof(double));
     if (W) std::memcpy(W, R->W.data(), R->W.size() * sizeof(double));
+    // active zones and own faces only (ghosts are refilled every stage), as
+    // the engine exports them
+    {
+        const Lay& L = R->L;
+        for (int b = 0; b < R->nb(); b++)
+            for (int k = -G; k < L.bz + G; k++)
+                for (int j = -G; j < BY + G; j++)
+                    for (int i = -G; i < BX + G; i++) {
+                        if (i >= 0 && i < BX && j >= 0 && j < BY && k >= 0 && k < L.bz) continue;
+                        for (int w = 0; w < 8; w++) {
+                            const size_t x = ((size_t)b * 8 + w) * L.ncell + cidx(i, j, k);
+                            if (U) U[x] = 0.0;
+                            if (W) W[x] = 0.0;
+                        }
+                    }
+    }
     if (Bf) {
         // own faces only (ghost faces are refilled every stage), as the engine exports them
         const Lay& L = R->L;
