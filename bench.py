@@ -210,6 +210,11 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--kernels-only", choices=("fp64", "mixed"),
+                    help="for counter collection: C3 initial data, then --steps eager steps of the step's kernels "
+                         "in the given precision with the default launch configuration; prints one JSON line")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="for counter collection: three plain 1 GiB device copies (reads and writes of known size)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the step runs on a single MI355X (every box here has one); "
@@ -217,6 +222,23 @@ def main():
         sys.exit(2)
 
     from bhfluid import sph, tune, device_info
+
+    if args.copy_calibration:
+        from bhfluid import copy_probe
+        ms = copy_probe(1 << 30, 3)
+        print(json.dumps({"mode": "copy-calibration", "bytes": 1 << 30, "reps": 3, "ms_per_copy": ms}))
+        return
+    if args.kernels_only:
+        g, blocks = c3_setup(sph)
+        eng, run = make_engine(sph, g, blocks, args.kernels_only == "mixed")
+        tune.apply(eng, tune.DEFAULT)
+        eng.use_graph(False)
+        eng.step([run], args.steps, want_dt=False)
+        eng.sync()
+        print(json.dumps({"mode": "kernels-only", "precision": args.kernels_only, "steps": args.steps,
+                          "blocks": int(len(blocks)), "zones": int(len(blocks)) * 4096,
+                          "launch": tune.DEFAULT}))
+        return
 
     g, blocks = c3_setup(sph)
     zones = len(blocks) * 4096
