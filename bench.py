@@ -90,9 +90,10 @@ def measured_flops(precision):
         return None
     for line in f.read_text().splitlines():
         cols = [c.strip() for c in line.strip().strip("|").split("|")]
-        if len(cols) >= 3 and cols[0] == "step" and cols[1] == precision:
+        # | step | precision | ms / step | FP64 flops / step | ...
+        if len(cols) >= 4 and cols[0] == "step" and cols[1] == precision:
             try:
-                return float(cols[2])
+                return float(cols[3])
             except ValueError:
                 return None
     return None
