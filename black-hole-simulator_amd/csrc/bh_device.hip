@@ -1464,28 +1464,30 @@ __global__ __launch_bounds__(TPB) void k_regrid_flags(KP P, const int* __restric
                  zc = P.org[3 * b + 2] + (k + 0.5) * dz;
     double r = sqrt(xc * xc + yc * yc + zc * zc);
     if (r < 1e-10) r = 1e-10;
+    // centred difference of word w along an axis, over twice the spacing
+    const double h[3] = {dx, dy, dz};
+    auto centred = [&](int w, int axis) {
+        const int up = axis == 0 ? lindx(i + 1, j, k) : axis == 1 ? lindx(i, j + 1, k) : lindx(i, j, k + 1);
+        const int dn = axis == 0 ? lindx(i - 1, j, k) : axis == 1 ? lindx(i, j - 1, k) : lindx(i, j, k - 1);
+        return fabs(Wa(P, b, w, up) - Wa(P, b, w, dn)) / (2 * h[axis]);
+    };
+    const bool deeper = level < 4;
     unsigned int f = 0u;
-    if (r < 3.0 && level < 4) {
+    if (r < 3.0 && deeper) {
         f = 3u;
     } else {
-        const double chro = Wa(P, b, WRHO, c);
-        if (chro > 0.0) {
-            double drho_dx = fabs(Wa(P, b, WRHO, lindx(i + 1, j, k)) - Wa(P, b, WRHO, lindx(i - 1, j, k))) / (2 * dx);
-            double drho_dy = fabs(Wa(P, b, WRHO, lindx(i, j + 1, k)) - Wa(P, b, WRHO, lindx(i, j - 1, k))) / (2 * dy);
-            double drho_dz = fabs(Wa(P, b, WRHO, lindx(i, j, k + 1)) - Wa(P, b, WRHO, lindx(i, j, k - 1))) / (2 * dz);
-            double drho = sqrt(drho_dx * drho_dx + drho_dy * drho_dy + drho_dz * drho_dz);
-            double rgrh = drho * dx / chro;
-            if (rgrh > 0.5 && level < 4) f |= 3u;
-            if (rgrh > 0.1) f |= 2u;
+        const double rho = Wa(P, b, WRHO, c);
+        if (rho > 0.0) {
+            const double gx = centred(WRHO, 0), gy = centred(WRHO, 1), gz = centred(WRHO, 2);
+            const double contrast = sqrt(gx * gx + gy * gy + gz * gz) * dx / rho;
+            if (contrast > 0.5 && deeper) f |= 3u;
+            if (contrast > 0.1) f |= 2u;
         }
-        const double B0 = Wa(P, b, WB0, c), B1 = Wa(P, b, WB1, c), B2 = Wa(P, b, WB2, c);
-        double Bmag = sqrt(B0 * B0 + B1 * B1 + B2 * B2);
-        if (Bmag > 1e-14) {
-            double dB_dx = fabs(Wa(P, b, WB0, lindx(i + 1, j, k)) - Wa(P, b, WB0, lindx(i - 1, j, k))) / (2 * dx);
-            double dB_dy = fabs(Wa(P, b, WB1, lindx(i, j + 1, k)) - Wa(P, b, WB1, lindx(i, j - 1, k))) / (2 * dy);
-            double dB_dz = fabs(Wa(P, b, WB2, lindx(i, j, k + 1)) - Wa(P, b, WB2, lindx(i, j, k - 1))) / (2 * dz);
-            double rdB = (dB_dx * dx + dB_dy * dy + dB_dz * dz) / Bmag;
-            if (rdB > 0.5 && level < 4) f |= 3u;
+        const double b0 = Wa(P, b, WB0, c), b1 = Wa(P, b, WB1, c), b2 = Wa(P, b, WB2, c);
+        const double bmag = sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+        if (bmag > 1e-14) {
+            const double var = (centred(WB0, 0) * dx + centred(WB1, 1) * dy + centred(WB2, 2) * dz) / bmag;
+            if (var > 0.5 && deeper) f |= 3u;
         }
     }
     if (f) atomicOr(&wg_flags, f);

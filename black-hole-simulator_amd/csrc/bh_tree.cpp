@@ -481,19 +481,30 @@ void HostTree::floor_or_keep(Patch& p, int c, const Cons& q) {
 }
 
 // ------------------------------------------------------------- regrid
+// The eight children of a patch, appended to the quilt in octant order
+// h = 4 ix + 2 iy + iz (the order amrtree::refine creates them in,
+// grid.cpp:574-600), each with the half-open box of its octant.
+void HostTree::append_children(Patch* p) {
+    const double lo[3] = {p->xe[0], p->ye[0], p->ze[0]}, hi[3] = {p->xe[BLK], p->ye[BLK], p->ze[BLK]};
+    const double mid[3] = {(lo[0] + hi[0]) / 2, (lo[1] + hi[1]) / 2, (lo[2] + hi[2]) / 2};
+    for (int h = 0; h < 8; h++) {
+        const int oct[3] = {(h >> 2) & 1, (h >> 1) & 1, h & 1};
+        std::array<double, 3> a, b;
+        std::array<int, 3> ijk;
+        for (int e = 0; e < 3; e++) {
+            a[e] = oct[e] ? mid[e] : lo[e];
+            b[e] = oct[e] ? hi[e] : mid[e];
+            ijk[e] = 2 * p->ijk[e] + oct[e];
+        }
+        quilt.push_back(std::make_unique<Patch>(p->lvl + 1, ijk, a, b, p));
+        p->children[h] = quilt.back().get();
+    }
+}
+
 // amrtree::refine, grid.cpp:574-655 (quirk Q24: +/- neighbours swapped)
 void HostTree::refine(Patch* p) {
     if (!p->leaf) return;
-    double xm = (p->xe[0] + p->xe[BLK]) / 2, ym = (p->ye[0] + p->ye[BLK]) / 2, zm = (p->ze[0] + p->ze[BLK]) / 2;
-    double xl = p->xe[0], xh = p->xe[BLK], yl = p->ye[0], yh = p->ye[BLK], zl = p->ze[0], zh = p->ze[BLK];
-    for (int h = 0; h < 8; h++) {
-        int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
-        std::array<double, 3> c1 = {ix ? xm : xl, iy ? ym : yl, iz ? zm : zl};
-        std::array<double, 3> c2 = {ix ? xh : xm, iy ? yh : ym, iz ? zh : zm};
-        std::array<int, 3> cijk = {2 * p->ijk[0] + ix, 2 * p->ijk[1] + iy, 2 * p->ijk[2] + iz};
-        quilt.push_back(std::make_unique<Patch>(p->lvl + 1, cijk, c1, c2, p));
-        p->children[h] = quilt.back().get();
-    }
+    append_children(p);
     for (int h = 0; h < 8; h++) {
         Patch* ch = p->children[h];
         int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
@@ -577,70 +588,93 @@ void HostTree::flatten(Patch* p) {
     p->leaf = true;
 }
 
-// amrtree::regrid, grid.cpp:739-835.  A patch's refine/erase flags are only
-// ever set to true there; they start false.  A patch flattened away while
-// still queued is skipped (the reference reads it after release).
-void HostTree::regrid() {
-    constexpr double lim_refine = 0.5, lim_flatten = 0.1, horizon_refine_r0 = 3.0;
-    constexpr int max_amr_lvl = 4;
-    for (auto& pp : quilt) {
-        Patch* p = pp.get();
-        if (!p->leaf) continue;
-        bool anyref = false, allflt = true;
-        for (int i = 0; i < BLK; i++)
-            for (int j = 0; j < BLK; j++)
-                for (int k = 0; k < BLK; k++) {
-                    int c = lindx(i, j, k);
-                    bool cref = false;
-                    if (p->r[c] < horizon_refine_r0 && p->lvl < max_amr_lvl) {
-                        anyref = true; allflt = false;
-                        continue;
-                    }
-                    double dx = p->dx(), dy = p->dy(), dz = p->dz();
-                    double chro = p->w(WRHO, c);
-                    if (chro > 0.0) {
-                        double drho_dx = std::abs(p->w(WRHO, lindx(i + 1, j, k)) - p->w(WRHO, lindx(i - 1, j, k))) / (2 * dx);
-                        double drho_dy = std::abs(p->w(WRHO, lindx(i, j + 1, k)) - p->w(WRHO, lindx(i, j - 1, k))) / (2 * dy);
-                        double drho_dz = std::abs(p->w(WRHO, lindx(i, j, k + 1)) - p->w(WRHO, lindx(i, j, k - 1))) / (2 * dz);
-                        double drho = std::sqrt(drho_dx * drho_dx + drho_dy * drho_dy + drho_dz * drho_dz);
-                        double rgrh = drho * dx / chro;
-                        if (rgrh > lim_refine && p->lvl < max_amr_lvl) { cref = true; anyref = true; allflt = false; }
-                        if (rgrh > lim_flatten) allflt = false;
-                    }
-                    double B0 = p->w(WB0, c), B1 = p->w(WB1, c), B2 = p->w(WB2, c);
-                    double Bmag = std::sqrt(B0 * B0 + B1 * B1 + B2 * B2);
-                    if (Bmag > 1e-14) {
-                        double dB_dx = std::abs(p->w(WB0, lindx(i + 1, j, k)) - p->w(WB0, lindx(i - 1, j, k))) / (2 * dx);
-                        double dB_dy = std::abs(p->w(WB1, lindx(i, j + 1, k)) - p->w(WB1, lindx(i, j - 1, k))) / (2 * dy);
-                        double dB_dz = std::abs(p->w(WB2, lindx(i, j, k + 1)) - p->w(WB2, lindx(i, j, k - 1))) / (2 * dz);
-                        double rdB = (dB_dx * dx + dB_dy * dy + dB_dz * dz) / Bmag;
-                        if (rdB > lim_refine && p->lvl < max_amr_lvl) { cref = true; anyref = true; allflt = false; }
-                    }
-                    (void)cref;
+// The refinement verdict of one leaf (the flag pass of amrtree::regrid,
+// grid.cpp:744-795): it asks to be refined where a zone lies inside r < 3
+// or its density or field varies by more than half its value across a zone,
+// and may be erased only if no zone's density varies by more than a tenth.
+// The operations are the reference's, so that the verdicts, and with them
+// the block tables, agree bit for bit: the density contrast is the length
+// of the centred-difference gradient times dx (also for dy, dz) over rho;
+// the field's is the sum over the axes of each component's own centred
+// difference times the spacing, over |B|.
+HostTree::LeafVerdict HostTree::leaf_verdict(Patch* p) const {
+    constexpr double kRefine = 0.5, kKeep = 0.1, kHoleZone = 3.0;
+    constexpr int kDeepest = 4;
+    const bool deeper = p->lvl < kDeepest;
+    const double h[3] = {p->dx(), p->dy(), p->dz()};
+    LeafVerdict v{false, true};
+    auto centred = [&](int var, int i, int j, int k, int axis) {
+        const int up = axis == 0 ? lindx(i + 1, j, k) : axis == 1 ? lindx(i, j + 1, k) : lindx(i, j, k + 1);
+        const int dn = axis == 0 ? lindx(i - 1, j, k) : axis == 1 ? lindx(i, j - 1, k) : lindx(i, j, k - 1);
+        return std::abs(p->w(var, up) - p->w(var, dn)) / (2 * h[axis]);
+    };
+    for (int i = 0; i < BLK; i++)
+        for (int j = 0; j < BLK; j++)
+            for (int k = 0; k < BLK; k++) {
+                const int c = lindx(i, j, k);
+                if (p->r[c] < kHoleZone && deeper) {
+                    v.refine = true;
+                    v.erasable = false;
+                    continue;
                 }
-        if (anyref && p->lvl < max_amr_lvl) p->refine = true;
-        if (allflt && p->lvl > 0) p->erase = true;
+                const double rho = p->w(WRHO, c);
+                if (rho > 0.0) {
+                    const double gx = centred(WRHO, i, j, k, 0), gy = centred(WRHO, i, j, k, 1),
+                                 gz = centred(WRHO, i, j, k, 2);
+                    const double contrast = std::sqrt(gx * gx + gy * gy + gz * gz) * h[0] / rho;
+                    if (contrast > kRefine && deeper) { v.refine = true; v.erasable = false; }
+                    if (contrast > kKeep) v.erasable = false;
+                }
+                const double b0 = p->w(WB0, c), b1 = p->w(WB1, c), b2 = p->w(WB2, c);
+                const double bmag = std::sqrt(b0 * b0 + b1 * b1 + b2 * b2);
+                if (bmag > 1e-14) {
+                    const double var = (centred(WB0, i, j, k, 0) * h[0] + centred(WB1, i, j, k, 1) * h[1] +
+                                        centred(WB2, i, j, k, 2) * h[2]) / bmag;
+                    if (var > kRefine && deeper) { v.refine = true; v.erasable = false; }
+                }
+            }
+    if (!deeper) v.refine = false;
+    if (p->lvl == 0) v.erasable = false;
+    return v;
+}
+
+// whether a parent's children can be merged back into it: every child is
+// a leaf and none is about to be refined (grid.cpp:817-830)
+static bool children_mergeable(const Patch* parent) {
+    for (int h = 0; h < 8; h++) {
+        const Patch* c = parent->children[h];
+        if (c && (!c->leaf || c->refine)) return false;
     }
-    std::vector<Patch*> rfnQ, fltQ;
+    return true;
+}
+
+// amrtree::regrid, grid.cpp:739-835: verdicts for every leaf, then the
+// leaves that asked are refined in quilt order, then for each leaf that may
+// be erased (in the quilt order of before the refinement) its parent is
+// flattened when all its children can be merged.  The refine/erase marks
+// start false and are cleared when acted on; a leaf removed by an earlier
+// flatten of the same pass is skipped (the reference reads it after release).
+void HostTree::regrid() {
     for (auto& pp : quilt) {
-        if (pp->leaf && pp->refine) rfnQ.push_back(pp.get());
-        if (pp->leaf && pp->erase) fltQ.push_back(pp.get());
+        if (!pp->leaf) continue;
+        const LeafVerdict v = leaf_verdict(pp.get());
+        if (v.refine) pp->refine = true;
+        if (v.erasable) pp->erase = true;
     }
-    for (Patch* p : rfnQ) {
+    std::vector<Patch*> grow, shrink;
+    for (auto& pp : quilt) {
+        if (pp->leaf && pp->refine) grow.push_back(pp.get());
+        if (pp->leaf && pp->erase) shrink.push_back(pp.get());
+    }
+    for (Patch* p : grow) {
         refine(p);
         p->refine = false;
     }
-    for (Patch* p : fltQ) {
-        if (std::find(erased_.begin(), erased_.end(), p) != erased_.end()) continue;
-        if (p->parent && p->parent->leaf == false) {
-            bool dhl = true;
-            for (int h = 0; h < 8; h++) {
-                Patch* sib = p->parent->children[h];
-                if (sib && (!sib->leaf || sib->refine)) { dhl = false; break; }
-            }
-            if (dhl) flatten(p->parent);
-        }
-        if (std::find(erased_.begin(), erased_.end(), p) == erased_.end()) p->erase = false;
+    auto gone = [&](const Patch* p) { return std::find(erased_.begin(), erased_.end(), p) != erased_.end(); };
+    for (Patch* p : shrink) {
+        if (gone(p)) continue;
+        if (p->parent && !p->parent->leaf && children_mergeable(p->parent)) flatten(p->parent);
+        if (!gone(p)) p->erase = false;
     }
     erased_.clear();
 }
@@ -659,55 +693,37 @@ void HostTree::regrid_structure(const std::vector<unsigned char>& leaf_flags, Re
         if (f & 2) pp->erase = true;
     }
     if (n != leaf_flags.size()) throw std::runtime_error("regrid flags: more flags than leaves");
-    std::vector<Patch*> rfnQ, fltQ;
+    std::vector<Patch*> grow, shrink;
     for (auto& pp : quilt) {
-        if (pp->leaf && pp->refine) rfnQ.push_back(pp.get());
-        if (pp->leaf && pp->erase) fltQ.push_back(pp.get());
+        if (pp->leaf && pp->refine) grow.push_back(pp.get());
+        if (pp->leaf && pp->erase) shrink.push_back(pp.get());
     }
-    for (Patch* p : rfnQ) {
-        double xm = (p->xe[0] + p->xe[BLK]) / 2, ym = (p->ye[0] + p->ye[BLK]) / 2, zm = (p->ze[0] + p->ze[BLK]) / 2;
-        double xl = p->xe[0], xh = p->xe[BLK], yl = p->ye[0], yh = p->ye[BLK], zl = p->ze[0], zh = p->ze[BLK];
-        for (int h = 0; h < 8; h++) {
-            int ix = (h >> 2) & 1, iy = (h >> 1) & 1, iz = h & 1;
-            std::array<double, 3> c1 = {ix ? xm : xl, iy ? ym : yl, iz ? zm : zl};
-            std::array<double, 3> c2 = {ix ? xh : xm, iy ? yh : ym, iz ? zh : zm};
-            std::array<int, 3> cijk = {2 * p->ijk[0] + ix, 2 * p->ijk[1] + iy, 2 * p->ijk[2] + iz};
-            quilt.push_back(std::make_unique<Patch>(p->lvl + 1, cijk, c1, c2, p));
-            p->children[h] = quilt.back().get();
-        }
+    for (Patch* p : grow) {
+        append_children(p);
         p->leaf = false;
         p->refine = false;
         log.refined.push_back(p);
     }
-    for (Patch* p : fltQ) {
-        if (std::find(erased_.begin(), erased_.end(), p) != erased_.end()) continue;
+    auto gone = [&](const Patch* p) { return std::find(erased_.begin(), erased_.end(), p) != erased_.end(); };
+    for (Patch* p : shrink) {
+        if (gone(p)) continue;
         Patch* par = p->parent;
-        if (par && par->leaf == false) {
-            bool dhl = true;
+        if (par && !par->leaf && children_mergeable(par)) {
+            std::array<int, 8> slots;
             for (int h = 0; h < 8; h++) {
-                Patch* sib = par->children[h];
-                if (sib && (!sib->leaf || sib->refine)) { dhl = false; break; }
+                Patch* ch = par->children[h];
+                slots[h] = ch ? ch->dev : -1;
+                if (!ch) continue;
+                erased_.push_back(ch);
+                quilt.erase(std::remove_if(quilt.begin(), quilt.end(),
+                                           [&](const std::unique_ptr<Patch>& q) { return q.get() == ch; }),
+                            quilt.end());
+                par->children[h] = nullptr;
             }
-            bool all_leaves = true;
-            for (int h = 0; h < 8; h++)
-                if (par->children[h] && !par->children[h]->leaf) all_leaves = false;
-            if (dhl && all_leaves) {
-                std::array<int, 8> slots;
-                for (int h = 0; h < 8; h++) {
-                    Patch* ch = par->children[h];
-                    slots[h] = ch ? ch->dev : -1;
-                    if (!ch) continue;
-                    erased_.push_back(ch);
-                    quilt.erase(std::remove_if(quilt.begin(), quilt.end(),
-                                               [&](const std::unique_ptr<Patch>& q) { return q.get() == ch; }),
-                                quilt.end());
-                    par->children[h] = nullptr;
-                }
-                par->leaf = true;
-                log.flattened.push_back({par, slots});
-            }
+            par->leaf = true;
+            log.flattened.push_back({par, slots});
         }
-        if (std::find(erased_.begin(), erased_.end(), p) == erased_.end()) p->erase = false;
+        if (!gone(p)) p->erase = false;
     }
     erased_.clear();
 }

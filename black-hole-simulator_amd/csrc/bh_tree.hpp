@@ -80,6 +80,12 @@ public:
     void ptoc_interior();
     // mesh adaptation, grid.cpp:574-835
     void regrid();
+    struct LeafVerdict {
+        bool refine;    // asks to be refined
+        bool erasable;  // may be merged back into its parent
+    };
+    LeafVerdict leaf_verdict(Patch* p) const;
+    void append_children(Patch* p);
     void refine(Patch* p);
     void ccrstr(Patch* p);
     void flatten(Patch* p);

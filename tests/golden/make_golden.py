@@ -6,6 +6,8 @@ the reference sources); no GPU run executes it.  Writes:
 
 * pointwise.npz -- inputs and outputs of the reference's metric::comp,
   conserved::ptoc / ctop and hlld on seeded random points;
+* regrid_flatten.json -- for R2 and R3, digests of the block table and
+  state after a second regrid that flattens (tests/regrid_cases.py);
 * trees.json    -- for each parity configuration, the SHA-256 of the initial
   state (block table and U, W, Bf after src/fluid/main.cpp:94-168) and of the
   state after every phase of every step (dtcomp, rkstg 0, rkstg 1, floors),
@@ -15,7 +17,7 @@ the reference sources); no GPU run executes it.  Writes:
   initially and after every completed step.  NaNs are canonicalised before
   hashing.
 
-Usage: python tests/golden/make_golden.py
+Usage: python tests/golden/make_golden.py [--flatten-only]
 """
 import ctypes
 import hashlib
@@ -148,7 +150,34 @@ def tree_record(name, cfg, nsteps):
     return rec
 
 
+def flatten_record(name, cfg):
+    """Digests of the block table and state after a second regrid that
+    flattens (tests/regrid_cases.py), with the counts before and after."""
+    from regrid_cases import flat_state
+    lo, hi, nqlt, regrid, a, q = cfg
+    t = RefTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    t.L.ref_set_threads(1)
+    t.torus_init(regrid=True)
+    s = t.state()
+    nb1, leaves1 = s.nb, int(s.meta[:, 4].sum())
+    s.W[:] = flat_state(s.W.shape)
+    t.set_state(s)
+    if t.L.ref_ptoc_interior(t.h) or t.L.ref_regrid(t.h):
+        raise RuntimeError(t.error())
+    s2 = t.state()
+    rec = {"config": name, "nblocks_before": nb1, "leaves_before": leaves1, "nblocks": s2.nb,
+           "leaves": int(s2.meta[:, 4].sum()), "blocks": digest(s2.meta, s2.edges), "state": state_digest(s2)}
+    print(name, rec, flush=True)
+    return rec
+
+
 def main():
+    if "--flatten-only" in sys.argv:
+        (HERE / "regrid_flatten.json").write_text(
+            json.dumps({n: flatten_record(n, getattr(configs, n)) for n in ("R2", "R3")}, indent=1))
+        return
+    (HERE / "regrid_flatten.json").write_text(
+        json.dumps({n: flatten_record(n, getattr(configs, n)) for n in ("R2", "R3")}, indent=1))
     pointwise()
     trees = {}
     for name, cfg, n in (("R0", configs.R0, 6), ("R1", configs.R1, 10), ("R2", configs.R2, 2),

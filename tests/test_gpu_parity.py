@@ -276,6 +276,41 @@ def test_regrid_on_device_matches_host(name):
     _same_state(host, dev)
 
 
+@pytest.mark.parametrize("name", ["R2", "R3"])
+def test_regrid_device_that_flattens_matches_reference_digests(name):
+    """The device regrid (flags, prolongation, restriction, con2prim of the
+    restricted zones, B_init on the GPU) on the flatten case of
+    tests/regrid_cases.py against the compiled reference's digests."""
+    import hashlib
+    import json
+    from pathlib import Path
+    from regrid_cases import flat_state
+    rec = json.loads((Path(__file__).resolve().parent / "golden" / "regrid_flatten.json").read_text())[name]
+    lo, hi, nqlt, regrid, a, Q = getattr(configs, name)
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    bhfluid.torus_setup(t, regrid=True)
+    U, W, Bf = t.state()
+    t.set_state(U, flat_state(W.shape), Bf)
+    t.ptoc_interior()
+    t.prepare()
+    assert t.regrid_device() > 0
+    meta, edges = t.blocks()
+    U, W, Bf = t.state()
+
+    def digest(*arrays):
+        h = hashlib.sha256()
+        for x in arrays:
+            x = np.ascontiguousarray(x)
+            if x.dtype == np.float64:
+                x = x.copy()
+                x[np.isnan(x)] = np.nan
+            h.update(x.tobytes())
+        return h.hexdigest()
+    assert (len(meta), int(meta[:, 4].sum())) == (rec["nblocks"], rec["leaves"])
+    assert digest(meta, edges) == rec["blocks"]
+    assert digest(U, W, Bf) == rec["state"]
+
+
 # ------------------------------------------------------------ initial data
 def test_initial_data_math_on_device():
     """The device's sin, cos, acos, pow and atan2 equal the oracle's

@@ -137,6 +137,31 @@ def test_oracle_phases_match_reference_digests(trees, name):
             assert bits_equal(ot.diagnostics()[0], _golden_diag(st["diag"])), (name, n)
 
 
+@pytest.mark.parametrize("name", ["R2", "R3"])
+def test_regrid_that_flattens_matches_reference_digests(name):
+    """amrtree::regrid's flatten path (ccrstr: volume-weighted restriction
+    of U, con2prim or the floor state, B_init; grid.cpp:657-736) together
+    with a second refinement, from the product's host regrid, against the
+    compiled reference's block table and state digests for the same input
+    (tests/regrid_cases.py).  R3 flattens 227 patches to 99."""
+    import bhfluid
+    from regrid_cases import flat_state
+    rec = json.loads((GOLD / "regrid_flatten.json").read_text())[name]
+    lo, hi, nqlt, regrid, a, q = getattr(configs, name)
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    bhfluid.torus_setup(t, regrid=True)
+    assert t.nblocks == rec["nblocks_before"]
+    U, W, Bf = t.state()
+    t.set_state(U, flat_state(W.shape), Bf)
+    t.ptoc_interior()
+    t.regrid()
+    meta, edges = t.blocks()
+    U, W, Bf = t.state()
+    assert (len(meta), int(meta[:, 4].sum())) == (rec["nblocks"], rec["leaves"])
+    assert digest(meta, edges) == rec["blocks"]
+    assert digest(U, W, Bf) == rec["state"]
+
+
 # ------------------------------------------------- against the compiled reference
 needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not built (no reference sources here)")
 
