@@ -91,6 +91,7 @@ def lib():
         _sig(L, "bh_tree_B_pot_init", _i, [_v, _d])
         _sig(L, "bh_tree_ptoc_interior", _i, [_v])
         _sig(L, "bh_tree_regrid", _i, [_v])
+        _sig(L, "bh_tree_set_blocks", _i, [_v, _i, _ip])
         _sig(L, "bh_tree_nblocks", _i, [_v, _ip])
         _sig(L, "bh_tree_get_blocks", _i, [_v, _ip, _dp])
         _sig(L, "bh_tree_get_state", _i, [_v, _dp, _dp, _dp])
@@ -290,6 +291,14 @@ class AmrTree:
     def regrid(self):
         """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""
         self._check(self._L.bh_tree_regrid(self._h), "regrid")
+
+    def set_blocks(self, meta: np.ndarray):
+        """Load a caller-owned block table (``meta[nb][14]`` as ``blocks()``
+        returns it, quilt order); every zone is reset to the floor state."""
+        meta = np.ascontiguousarray(meta, dtype=np.int32)
+        if meta.ndim != 2 or meta.shape[1] != 14:
+            raise ValueError("meta must be [nb, 14]")
+        self._check(self._L.bh_tree_set_blocks(self._h, int(meta.shape[0]), _pi(meta)), "set_blocks")
 
     def floors(self):
         """``patch::floors`` on every patch (grid.cpp:331-358)."""

@@ -2616,6 +2616,16 @@ int bh_tree_regrid(bh_tree* t) {
         return finish_faults(t, t->host->faults - f0);
     });
 }
+int bh_tree_set_blocks(bh_tree* t, int nb, const int* meta) {
+    if (!t || nb <= 0 || !meta) return fail(t, BH_EINVAL, "invalid arguments");
+    return guarded(t, [&] {
+        need_host(t);
+        t->host->set_blocks(nb, meta);
+        host_changed(t);
+        t->layout_dirty = true;
+        return BH_OK;
+    });
+}
 int bh_tree_regrid_device(bh_tree* t, int* changed) {
     return guarded(t, [&] {
         need_device(t);

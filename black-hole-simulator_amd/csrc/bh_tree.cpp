@@ -5,6 +5,10 @@
 // reference writes them so the initial data and geometry round identically.
 #include "bh_tree.hpp"
 
+#include <map>
+#include <stdexcept>
+#include <string>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -204,6 +208,78 @@ HostTree::HostTree(const double lo[3], const double hi[3], int nqlt, double M, d
             p->setU(c, ptoc(tree_point(p->r[c], p->th[c]), prm.gamma, w, nf));
         }
     faults += nf;
+}
+
+// Replace the block table by a caller's, meta[nb][14] in the layout of
+// export_blocks (level, i, j, k, leaf, parent, children[8]) and in quilt
+// order.  Roots are patches of the constructor's root grid (their boxes are
+// taken from it); every other block is a child of an earlier block, in the
+// octant its ijk names, with the box refine() would give it, so the
+// geometry is the reference's bit for bit.  Every zone starts in the floor
+// state; the caller sets the state next.
+void HostTree::set_blocks(int nb, const int* meta) {
+    if (nb <= 0 || !meta) throw std::invalid_argument("set_blocks: empty block table");
+    std::map<std::array<int, 3>, const Patch*> roots;
+    for (auto& pp : quilt)
+        if (pp->lvl == 0) roots[pp->ijk] = pp.get();
+    auto bad = [](int b, const char* what) {
+        throw std::invalid_argument("set_blocks: block " + std::to_string(b) + ": " + what);
+    };
+    std::vector<std::unique_ptr<Patch>> nq(nb);
+    for (int b = 0; b < nb; b++) {
+        const int* m = meta + 14 * (size_t)b;
+        const std::array<int, 3> ijk = {m[1], m[2], m[3]};
+        const int par = m[5];
+        if (par < 0) {
+            auto it = roots.find(ijk);
+            if (m[0] != 0 || it == roots.end()) bad(b, "a block without parent must be a root of the constructor grid");
+            const Patch* r = it->second;
+            nq[b] = std::make_unique<Patch>(0, ijk, std::array<double, 3>{r->xe[0], r->ye[0], r->ze[0]},
+                                            std::array<double, 3>{r->xe[BLK], r->ye[BLK], r->ze[BLK]}, nullptr);
+        } else {
+            if (par >= b) bad(b, "its parent must come earlier in the table");
+            const Patch* P = nq[par].get();
+            const int* pm = meta + 14 * (size_t)par;
+            int oct[3];
+            for (int e = 0; e < 3; e++) {
+                oct[e] = ijk[e] - 2 * P->ijk[e];
+                if (oct[e] != 0 && oct[e] != 1) bad(b, "ijk is not inside its parent");
+            }
+            const int h = 4 * oct[0] + 2 * oct[1] + oct[2];
+            if (m[0] != P->lvl + 1 || pm[6 + h] != b) bad(b, "parent and child entries disagree");
+            const double lo[3] = {P->xe[0], P->ye[0], P->ze[0]}, hi[3] = {P->xe[BLK], P->ye[BLK], P->ze[BLK]};
+            std::array<double, 3> a, c;
+            for (int e = 0; e < 3; e++) {
+                const double mid = (lo[e] + hi[e]) / 2;
+                a[e] = oct[e] ? mid : lo[e];
+                c[e] = oct[e] ? hi[e] : mid;
+            }
+            nq[b] = std::make_unique<Patch>(m[0], ijk, a, c, nq[par].get());
+        }
+    }
+    for (int b = 0; b < nb; b++) {
+        const int* m = meta + 14 * (size_t)b;
+        bool any = false;
+        for (int h = 0; h < 8; h++) {
+            const int c = m[6 + h];
+            if (c < 0) continue;
+            if (c >= nb || meta[14 * (size_t)c + 5] != b) bad(b, "child entry does not point back");
+            nq[b]->children[h] = nq[c].get();
+            any = true;
+        }
+        if ((m[4] != 0) == any) bad(b, "leaf flag and children disagree");
+        nq[b]->leaf = !any;
+    }
+    int nf = 0;
+    for (auto& p : nq)
+        for (int c = 0; c < NCELL; c++) {
+            Prim w = pvfs(p->r[c]);
+            p->setW(c, w);
+            p->setU(c, ptoc(tree_point(p->r[c], p->th[c]), prm.gamma, w, nf));
+        }
+    faults += nf;
+    quilt = std::move(nq);
+    erased_.clear();
 }
 
 int HostTree::index_of(const Patch* p) const {

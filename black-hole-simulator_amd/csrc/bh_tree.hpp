@@ -80,6 +80,7 @@ public:
     void ptoc_interior();
     // mesh adaptation, grid.cpp:574-835
     void regrid();
+    void set_blocks(int nb, const int* meta);
     struct LeafVerdict {
         bool refine;    // asks to be refined
         bool erasable;  // may be merged back into its parent

@@ -53,6 +53,17 @@ int bh_tree_B_pot_init(bh_tree* t, double rho_max);
 int bh_tree_ptoc_interior(bh_tree* t);
 /* grid::amrtree::regrid(), src/fluid/grid/grid.cpp:739-835. */
 int bh_tree_regrid(bh_tree* t);
+/* Load a caller-owned block table in place of the tree's: the quilt of a
+ * grid::amrtree (src/fluid/grid.hpp:111-120), meta[nb][14] in the layout of
+ * bh_tree_get_blocks (level, i, j, k, leaf, parent, children[8]) and in quilt
+ * order.  Blocks without parent must be roots of the grid the tree was
+ * created with; every other block must follow its parent and sit in the
+ * octant its ijk names.  Patch boxes are derived as amrtree::refine derives
+ * them (grid.cpp:574-600), so edges and cell geometry equal the caller's
+ * tree bit for bit.  Every zone is reset to the floor state: set the state
+ * next (bh_tree_set_state).  BH_EINVAL (with the offending block in
+ * bh_last_error) for an inconsistent table. */
+int bh_tree_set_blocks(bh_tree* t, int nb, const int* meta);
 /* The same regrid (grid.cpp:739-835) with the state on the GPU: the
  * refinement flags are computed on the device, the block table changes on
  * the host, and prolongation (refine), restriction (ccrstr), their con2prim

@@ -38,6 +38,7 @@
 #include "recon.hpp"
 #include "hlld.hpp"
 #include "analysis.hpp"
+#include "ref_harness.h"
 
 // Zero-filled heap for the reference's objects.  grid::patch leaves its
 // refine/erase flags uninitialised (src/fluid/grid.hpp:98-99, constructor at
@@ -59,11 +60,6 @@ void operator delete[](void* p, std::size_t) noexcept { std::free(p); }
 namespace {
 constexpr int NCELL = cpn * cpn * cpn;
 constexpr int NFACE = (cpn + 1) * cpn * cpn;
-
-struct RefTree {
-    std::unique_ptr<grid::amrtree> tree;
-    char err[256] = {0};
-};
 
 void put_prim(const prim& w, double* o) {
     o[0] = w.rho; o[1] = w.eps;

@@ -93,6 +93,26 @@ def ref_available() -> bool:
     return REF_SO.exists()
 
 
+ADAPTER_SO = REF_SO.parent / "libbhadapter.so"
+_adapter = None
+
+
+def adapter_lib():
+    """oracle/_ref/libbhadapter.so: the mode-B drop-in (oracle/gpu_rk2_adapter.hpp)
+    compiled against the reference's headers, stepping reference trees on the GPU."""
+    global _adapter
+    if _adapter is None:
+        if not ADAPTER_SO.exists():
+            raise FileNotFoundError(f"{ADAPTER_SO} not built (make -C oracle ref)")
+        L = ctypes.CDLL(str(ADAPTER_SO))
+        _sig(L, "adapter_new", _v, [_d, _dp, _dp, _i])
+        _sig(L, "adapter_free", None, [_v])
+        _sig(L, "adapter_error", ctypes.c_char_p, [])
+        _sig(L, "adapter_step", _i, [_v, _v, _dp])
+        _adapter = L
+    return _adapter
+
+
 def ref_lib():
     global _ref
     if _ref is None:

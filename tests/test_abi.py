@@ -123,3 +123,51 @@ def test_dump_roundtrip_from_host_state(tmp_path):
     assert np.array_equal(d["lo"], e[:, :, 0]) and np.array_equal(d["hi"], e[:, :, 16])
     assert np.array_equal(d["W"].view(np.int64), Wl.view(np.int64))
     assert path.stat().st_size == 128 + 64 * leaf.sum() + leaf.sum() * 8 * 4096 * 8
+
+
+def test_set_blocks_reproduces_a_regridded_tree():
+    """bh_tree_set_blocks: a fresh tree given the block table of a regridded
+    one (R3: 227 patches on three levels) has the same blocks, edges and cell
+    geometry bit for bit, takes its state, and regrids to the same tree."""
+    import configs
+    lo, hi, nqlt, _, a, q = configs.R3
+    src = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    bhfluid.torus_setup(src, regrid=True)
+    meta, edges = src.blocks()
+    dst = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    dst.set_blocks(meta)
+    m2, e2 = dst.blocks()
+    assert np.array_equal(m2, meta) and np.array_equal(e2.view(np.int64), edges.view(np.int64))
+    dst.set_state(*src.state())
+    for x, y in zip(src.state(), dst.state()):
+        assert np.array_equal(x.view(np.int64), y.view(np.int64))
+    src.regrid()
+    dst.regrid()
+    assert np.array_equal(src.blocks()[0], dst.blocks()[0])
+    for x, y in zip(src.state(), dst.state()):
+        assert np.array_equal(x.view(np.int64), y.view(np.int64))
+
+
+def test_set_blocks_rejects_inconsistent_tables():
+    import configs
+    lo, hi, nqlt, _, a, q = configs.R2
+    src = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    bhfluid.torus_setup(src, regrid=True)
+    meta, _ = src.blocks()
+    dst = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    child = int(np.flatnonzero(meta[:, 5] >= 0)[0])
+    bad = meta.copy()
+    bad[child, 1] += 2                      # ijk outside its parent
+    with pytest.raises(bhfluid.BhError, match="block"):
+        dst.set_blocks(bad)
+    bad = meta.copy()
+    bad[int(meta[child, 5]), 4] = 1          # a parent marked leaf
+    with pytest.raises(bhfluid.BhError, match="leaf flag"):
+        dst.set_blocks(bad)
+    bad = meta.copy()
+    bad[0, 5] = -1
+    bad[0, 1] = 99                          # not a root of the grid
+    with pytest.raises(bhfluid.BhError, match="root"):
+        dst.set_blocks(bad)
+    dst.set_blocks(meta)                    # the tree is still usable
+    assert dst.nblocks == len(meta)

@@ -311,6 +311,50 @@ def test_regrid_device_that_flattens_matches_reference_digests(name):
     assert digest(U, W, Bf) == rec["state"]
 
 
+@pytest.mark.parametrize("name,nsteps", [("R4", 3), ("R3", 1)])
+def test_mode_b_adapter_steps_the_reference_tree(name, nsteps):
+    """The reference-side drop-in (oracle/gpu_rk2_adapter.hpp, compiled
+    against the reference's headers): a compiled-reference tree -- its own
+    initial data, and for R3 its own regrid to two levels -- stepped by
+    integ::gpu_rk2integrator (block table loaded with bh_tree_set_blocks,
+    state copied to the GPU and back every step) equals the oracle stepping
+    the same state with the GPU's arithmetic, bit for bit, dt included; where
+    the reference's equation of state throws (R3, step 1), the adapter throws
+    std::domain_error after copying the state back."""
+    from oracle_libs import RefTree, adapter_lib, ref_available
+    if not ref_available():
+        pytest.skip("oracle/_ref not built")
+    import ctypes
+    lo, hi, nqlt, regrid, a, Q = getattr(configs, name)
+    rt = RefTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
+    rt.L.ref_set_threads(1)
+    rt.torus_init(regrid=regrid)
+    s0 = rt.state()
+    ot = OracleTree.from_state(s0, (1.0, a, Q, configs.GAMMA), libm=1)
+    A = adapter_lib()
+    lo_a, hi_a = np.array(lo, np.float64), np.array(hi, np.float64)
+    dp = ctypes.POINTER(ctypes.c_double)
+    ad = A.adapter_new(configs.CFL, lo_a.ctypes.data_as(dp), hi_a.ctypes.data_as(dp), nqlt)
+    try:
+        for step in range(nsteps):
+            dt = ctypes.c_double(0.0)
+            rc = A.adapter_step(ad, rt.h, ctypes.byref(dt))
+            assert rc >= 0, A.adapter_error().decode()
+            dt_o, faults = ot.step(configs.CFL)
+            print(f"[{name}] step {step}: rc {rc} dt {dt.value!r} oracle {dt_o!r} faults {faults}", flush=True)
+            assert (rc == 1) == (faults > 0), (rc, faults)
+            if rc == 0:                      # a throwing step returns no dt (as the reference's)
+                assert dt.value == dt_o
+            if rc == 1:
+                assert "density is not positive" in A.adapter_error().decode()
+            s, o = rt.state(), ot.state()
+            for nm, x, y in (("U", s.U, o.U), ("W", s.W, o.W), ("Bf", s.Bf, o.Bf)):
+                n, ex = first_diff(x, y)
+                assert n == 0, f"{name} step {step}: {nm} differs in {n} values, first {ex}"
+    finally:
+        A.adapter_free(ad)
+
+
 # ------------------------------------------------------------ initial data
 def test_initial_data_math_on_device():
     """The device's sin, cos, acos, pow and atan2 equal the oracle's

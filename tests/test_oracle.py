@@ -167,6 +167,28 @@ needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not buil
 
 
 @needs_ref
+@pytest.mark.parametrize("name", ["R2", "R3"])
+def test_set_blocks_takes_the_reference_quilt(name):
+    """bh_tree_set_blocks on the compiled reference's own quilt after its
+    regrid: the product's tree has the reference's block table and patch
+    edges bit for bit, and takes its state unchanged."""
+    import bhfluid
+    lo, hi, nqlt, regrid, a, q = getattr(configs, name)
+    rt = RefTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    rt.L.ref_set_threads(1)
+    rt.torus_init(regrid=True)
+    s = rt.state()
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    t.set_blocks(s.meta)
+    meta, edges = t.blocks()
+    assert np.array_equal(meta, s.meta)
+    assert np.array_equal(edges.view(np.int64), s.edges.view(np.int64))
+    t.set_state(s.U, s.W, s.Bf)
+    U, W, Bf = t.state()
+    assert digest(U, W, Bf) == digest(s.U, s.W, s.Bf)
+
+
+@needs_ref
 def test_oracle_vs_reference_random_pointwise():
     from oracle_libs import ref_lib
 
