@@ -87,6 +87,11 @@ int bhs_run_nblocks(bhs_engine* e, int run, int* nblocks);
  * info[4] = rho_max (unnormalised), max p_gas, max p_mag before scaling, the
  * field scale factor. */
 int bhs_run_init_torus(bhs_engine* e, int run, const bhs_torus* torus, double* info);
+/* A run's state: the active zones of U and W and the own faces of Bf (a
+ * block owns the faces of its zones, both ends along each direction).
+ * Ghost cells and ghost faces are derived data, refilled every stage as far
+ * as the step reads them; get returns them as 0 and set may pass anything
+ * there.  Any pointer may be NULL. */
 int bhs_run_get_state(bhs_engine* e, int run, double* U, double* W, double* Bf);
 int bhs_run_set_state(bhs_engine* e, int run, const double* U, const double* W, const double* Bf);
 
