@@ -42,6 +42,11 @@ _lib = None
 
 _dp = ctypes.POINTER(ctypes.c_double)
 _ip = ctypes.POINTER(ctypes.c_int)
+
+
+class _Params(ctypes.Structure):
+    """bh_params (include/bhfluid.h)."""
+    _fields_ = [("limiter", ctypes.c_int), ("reserved", ctypes.c_int * 7)]
 _d = ctypes.c_double
 _i = ctypes.c_int
 _v = ctypes.c_void_p
@@ -92,6 +97,8 @@ def lib():
         _sig(L, "bh_tree_ptoc_interior", _i, [_v])
         _sig(L, "bh_tree_regrid", _i, [_v])
         _sig(L, "bh_tree_set_blocks", _i, [_v, _i, _ip])
+        _sig(L, "bh_tree_set_params", _i, [_v, ctypes.POINTER(_Params)])
+        _sig(L, "bh_tree_get_params", _i, [_v, ctypes.POINTER(_Params)])
         _sig(L, "bh_tree_nblocks", _i, [_v, _ip])
         _sig(L, "bh_tree_get_blocks", _i, [_v, _ip, _dp])
         _sig(L, "bh_tree_get_state", _i, [_v, _dp, _dp, _dp])
@@ -291,6 +298,22 @@ class AmrTree:
     def regrid(self):
         """``amrtree::regrid`` (src/fluid/grid/grid.cpp:739-835)."""
         self._check(self._L.bh_tree_regrid(self._h), "regrid")
+
+    LIMITERS = {"minmod": 0, "vanleer": 1, "mc": 2}
+
+    def set_limiter(self, name: str):
+        """reconfp's slope limiter in the step: "minmod", "vanleer" (the
+        reference's, the default) or "mc" (src/fluid/recon.hpp:16-34)."""
+        p = _Params()
+        self._check(self._L.bh_tree_get_params(self._h, ctypes.byref(p)), "get_params")
+        p.limiter = self.LIMITERS[name]
+        self._check(self._L.bh_tree_set_params(self._h, ctypes.byref(p)), "set_params")
+
+    @property
+    def limiter(self) -> str:
+        p = _Params()
+        self._check(self._L.bh_tree_get_params(self._h, ctypes.byref(p)), "get_params")
+        return {v: k for k, v in self.LIMITERS.items()}[p.limiter]
 
     def set_blocks(self, meta: np.ndarray):
         """Load a caller-owned block table (``meta[nb][14]`` as ``blocks()``

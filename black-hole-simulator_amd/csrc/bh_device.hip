@@ -2469,6 +2469,7 @@ using namespace bh;
 struct bh_tree {
     std::unique_ptr<HostTree> host;
     std::unique_ptr<DeviceTree> dev;
+    bh_params params{BH_LIM_VANLEER, {0, 0, 0, 0, 0, 0, 0}};
     bool host_valid = true;   // host copy holds the current state
     bool dev_valid = false;   // device copy holds the current state
     bool layout_dirty = true; // block structure changed since the device build
@@ -2496,6 +2497,10 @@ void host_changed(bh_tree* t) {
 }
 void need_device(bh_tree* t) {
     if (!t->dev) t->dev = std::make_unique<DeviceTree>();
+    if (t->dev->limiter != t->params.limiter) {
+        t->dev->limiter = t->params.limiter;
+        t->dev->graph_release();  // the captured step holds the old kernel arguments
+    }
     if (t->layout_dirty) {
         need_host(t);
         t->dev->build(*t->host);
@@ -2615,6 +2620,19 @@ int bh_tree_regrid(bh_tree* t) {
         t->layout_dirty = true;
         return finish_faults(t, t->host->faults - f0);
     });
+}
+int bh_tree_set_params(bh_tree* t, const bh_params* p) {
+    if (!t || !p) return fail(t, BH_EINVAL, "invalid arguments");
+    if (p->limiter < BH_LIM_MINMOD || p->limiter > BH_LIM_MC) return fail(t, BH_EINVAL, "limiter must be 0, 1 or 2");
+    for (int q = 0; q < 7; q++)
+        if (p->reserved[q]) return fail(t, BH_EINVAL, "reserved parameter words must be zero");
+    t->params = *p;
+    return BH_OK;
+}
+int bh_tree_get_params(const bh_tree* t, bh_params* p) {
+    if (!t || !p) return BH_EINVAL;
+    *p = t->params;
+    return BH_OK;
 }
 int bh_tree_set_blocks(bh_tree* t, int nb, const int* meta) {
     if (!t || nb <= 0 || !meta) return fail(t, BH_EINVAL, "invalid arguments");

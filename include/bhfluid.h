@@ -53,6 +53,20 @@ int bh_tree_B_pot_init(bh_tree* t, double rho_max);
 int bh_tree_ptoc_interior(bh_tree* t);
 /* grid::amrtree::regrid(), src/fluid/grid/grid.cpp:739-835. */
 int bh_tree_regrid(bh_tree* t);
+/* Numerical options of the step, which the reference fixes in its source.
+ * limiter: the PLM slope limiter of reconfp (src/fluid/recon.hpp:16-34):
+ * BH_LIM_MINMOD, BH_LIM_VANLEER (the one the reference passes,
+ * src/fluid/grid/grid.cpp:261,285,309; the default) or BH_LIM_MC.  The
+ * reserved words must be zero.  Takes effect from the next step. */
+#define BH_LIM_MINMOD 0
+#define BH_LIM_VANLEER 1
+#define BH_LIM_MC 2
+typedef struct {
+    int limiter;
+    int reserved[7];
+} bh_params;
+int bh_tree_set_params(bh_tree* t, const bh_params* p);
+int bh_tree_get_params(const bh_tree* t, bh_params* p);
 /* Load a caller-owned block table in place of the tree's: the quilt of a
  * grid::amrtree (src/fluid/grid.hpp:111-120), meta[nb][14] in the layout of
  * bh_tree_get_blocks (level, i, j, k, leaf, parent, children[8]) and in quilt

@@ -598,6 +598,7 @@ struct Tree {
     std::vector<double> F;           // [nb][3][8400][8] fluxes (zero outside computed faces, quirk Q19)
     std::vector<double> E;           // [nb][3][8820] EMFs
     std::atomic<long long> faults{0};
+    int limiter = 1;  // reconfp's limiter: the reference passes van Leer (grid.cpp:261,285,309)
 
     EOS eos() { return EOS{gamma, &faults}; }
     double& u(int b, int v, int c) { return U[((size_t)b * 8 + v) * NCELL + c]; }
@@ -890,7 +891,7 @@ void fluxcomp(Tree& t, int p) {
             for (int j = lo[1]; j < BLK; j++)
                 for (int k = lo[2]; k < BLK; k++) {
                     Prim L, R;
-                    reconfp(t, p, i, j, k, dim, 1, L, R);
+                    reconfp(t, p, i, j, k, dim, t.limiter, L, R);
                     int bidx = dim == 0 ? ixf(i + 1, j, k) : dim == 1 ? iyf(i, j + 1, k) : izf(i, j, k + 1);
                     L.B[dim] = Bfs[dim][bidx];
                     R.B[dim] = Bfs[dim][bidx];
@@ -1655,6 +1656,8 @@ void or_cell_geom(void* h, int b, double* out) {
     }
 }
 void or_set_threads(int n) { omp_set_num_threads(n); }
+// the limiter reconfp applies in the step: 0 minmod, 1 van Leer, 2 MC (recon.hpp:16-34)
+void or_set_limiter(void* h, int lim) { static_cast<Tree*>(h)->limiter = lim; }
 
 void or_metric(double M, double a, double Q, double r, double th, double* o) {
     MC mc = metric_comp(KS{M, a, Q}, r, th);

@@ -50,6 +50,7 @@ void or_cell_geom(void* t, int b, double* out);
  * abs[8] = sum of |term| of each integral (Hscal: numerator [6], denominator [7]). */
 void or_diagnostics(void* t, double* out, double* abs);
 void or_set_threads(int n);
+void or_set_limiter(void* t, int lim);
 
 /* pointwise */
 void or_metric(double M, double a, double Q, double r, double th, double* out120);

@@ -77,6 +77,7 @@ def oracle_lib():
         _sig(L, "or_cell_geom", None, [_v, _i, _dp])
         _sig(L, "or_diagnostics", None, [_v, _dp, _dp])
         _sig(L, "or_set_threads", None, [_i])
+        _sig(L, "or_set_limiter", None, [_v, _i])
         _sig(L, "or_metric", None, [_d, _d, _d, _d, _d, _dp])
         _sig(L, "or_ptoc", _i, [_d, _d, _d, _d, _dp, _d, _d, _dp, _dp])
         _sig(L, "or_ctop", _i, [_d, _d, _d, _d, _dp, _d, _d, _dp, _dp, _i])
@@ -276,6 +277,10 @@ class OracleTree:
 
     def set_state(self, s: State):
         self.L.or_set_state(self.h, _p(s.U), _p(s.W), _p(s.Bf), _p(s.dU))
+
+    def set_limiter(self, lim: int):
+        """reconfp's limiter in the step: 0 minmod, 1 van Leer (default), 2 MC."""
+        self.L.or_set_limiter(self.h, int(lim))
 
     def state(self, with_dU=False) -> State:
         nb = self.nb

@@ -171,3 +171,20 @@ def test_set_blocks_rejects_inconsistent_tables():
         dst.set_blocks(bad)
     dst.set_blocks(meta)                    # the tree is still usable
     assert dst.nblocks == len(meta)
+
+
+def test_params_limiter_roundtrip_and_validation():
+    import configs
+    lo, hi, nqlt, _, a, q = configs.R1
+    t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
+    assert t.limiter == "vanleer"               # the reference's
+    for name in ("minmod", "mc", "vanleer"):
+        t.set_limiter(name)
+        assert t.limiter == name
+    P = bhfluid._Params()
+    P.limiter = 3
+    assert t._L.bh_tree_set_params(t._h, ctypes.byref(P)) < 0
+    P.limiter = 0
+    P.reserved[3] = 1
+    assert t._L.bh_tree_set_params(t._h, ctypes.byref(P)) < 0
+    assert t.limiter == "vanleer"               # rejected calls change nothing
