@@ -77,6 +77,7 @@ def lib():
         sig("bhs_steps", _i, [_v, _i, _ip, _i, _dp])
         sig("bhs_c2p", _i, [_v, _i])
         sig("bhs_run_diagnostics", _i, [_v, _i, _dp])
+        sig("bhs_run_analysis", _i, [_v, _i, _d, _dp])
         sig("bhs_set_precision", _i, [_v, _i])
         sig("bhs_set_schedule", _i, [_v, _i, _i])
         sig("bhs_use_graph", _i, [_v, _i])
@@ -253,6 +254,14 @@ class Run:
         self.eng._check(self.eng._L.bhs_run_diagnostics(self.eng._h, self.id, _p(out)), "diagnostics")
         return dict(mass=out[0], energy=out[1], divb_max=out[2], divb_scale=out[3], c2p_fail=int(out[4]),
                     floors=int(out[5]), mdot=out[6], phi_b=out[7])
+
+    def analysis(self, rho_min: float = 0.0) -> dict:
+        """The analyser quantities that read derived primitives
+        (src/fluid/data/quantitatives.cpp:109-193) in covariant form; the
+        stress ratios over zones with rho >= rho_min (include/bhsph.h)."""
+        d = np.zeros(10)
+        self.eng._check(self.eng._L.bhs_run_analysis(self.eng._h, self.id, float(rho_min), _p(d)), "analysis")
+        return dict(thermE=d[0], magE=d[1], Ldot=d[2], L_BZ=d[3], eta_BZ=d[4], phi_BH=d[5], P_beta=d[6], alpha_SS=d[7], maxwell_reynolds=d[8], mdot=d[9])
 
     def active(self, W: np.ndarray) -> np.ndarray:
         """Active zones of a W (or U) array: [nb][8][bz][16][16] (phi, theta, r)."""

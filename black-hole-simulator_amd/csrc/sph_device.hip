@@ -1234,6 +1234,53 @@ __global__ void k_diag_seq(const double* __restrict__ part, int nl, double* __re
     out[0] = ms; out[1] = en; out[2] = dm; out[3] = sc; out[4] = md; out[5] = ph;
 }
 
+// Analyser quantities: per block the sums of analysis_terms times the
+// volume element sqrt(-g) dr dtheta dphi (words 0-5) or, on the horizon row,
+// the surface element sqrt(-g) dtheta dphi (words 6-8), and half the
+// unsigned radial face flux there (word 9, as Phi_B in k_diag); fixed-order
+// workgroup sums.
+constexpr int NANA_W = NANA + 1;
+__global__ __launch_bounds__(TPB) void k_analysis(KPS P, double rho_min, double* out) {
+    __shared__ double sm[NANA_W][TPB];
+    const int b = P.list[blockIdx.x];
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const RunPar& R = P.run[v.run];
+    const double* F0 = P.Bf0 + (size_t)b * 3 * P.nf;
+    double acc[NANA_W];
+#pragma unroll
+    for (int q = 0; q < NANA_W; q++) acc[q] = 0.0;
+    for (int a = threadIdx.x; a < P.nint; a += TPB) {
+        const int i = a % BX, j = (a / BX) % BY, k = a / (BX * BY);
+        const int gi = B.bi * BX + i, gj = B.bj * BY + j;
+        const int c = cidx(i, j, k);
+        const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
+        const Met<double> m = load_met<double>(gl(v.gc), v.npt, p);
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) w[q] = P.W[((size_t)b * 8 + q) * P.ncell + c];
+        const bool hz = gi == v.ih;
+        double t[NANA];
+        analysis_terms(m, R.gam, w, rho_min, hz, t);
+        const double sa = m.alpha * m.sg * v.dth * v.dph, sv = sa * gl(v.dr)[gi + 2];
+#pragma unroll
+        for (int q = 0; q < 6; q++) acc[q] += t[q] * sv;
+#pragma unroll
+        for (int q = 6; q < NANA; q++) acc[q] += t[q] * sa;
+        if (hz) acc[NANA] += 0.5 * fabs(F0[fidx(0, i, j, k)]) * v.dth * v.dph;
+    }
+#pragma unroll
+    for (int q = 0; q < NANA_W; q++) sm[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = TPB / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o)
+#pragma unroll
+            for (int q = 0; q < NANA_W; q++) sm[q][threadIdx.x] += sm[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < NANA_W) out[(size_t)blockIdx.x * NANA_W + threadIdx.x] = sm[threadIdx.x][0];
+}
+
 // ---------------------------------------------------- launch variants
 constexpr int NVAR = 6;
 constexpr int VAR_TB[NVAR] = {256, 256, 256, 128, 128, 128};
@@ -2009,6 +2056,26 @@ public:
         out[6] = md;
         out[7] = ph;
     }
+
+    // the analyser quantities of a run: out[10] = thermal energy, magnetic
+    // energy, Ldot, L_BZ, eta_BZ = L_BZ/|Mdot|, phi_BH = Phi_B/sqrt(|Mdot|),
+    // P_beta = p/p_mag, alpha_SS = (Reynolds + Maxwell)/p_tot and
+    // Maxwell/Reynolds (both over rho >= rho_min), Mdot
+    void analysis(int r, double rho_min, double* out) {
+        set_active({r});
+        const int nl = (int)act_slots.size();
+        diagbuf.alloc((size_t)nl * NANA_W);
+        KPS P = kp(list.p, nl);
+        hipLaunchKernelGGL(k_analysis, dim3(nl), dim3(TPB), 0, stream, P, rho_min, diagbuf.p);
+        SCHK(hipGetLastError());
+        std::vector<double> h((size_t)nl * NANA_W);
+        SCHK(hipMemcpyAsync(h.data(), diagbuf.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, stream));
+        SCHK(hipStreamSynchronize(stream));
+        double S[NANA_W] = {};
+        for (int q = 0; q < nl; q++)
+            for (int w = 0; w < NANA_W; w++) S[w] += h[(size_t)q * NANA_W + w];
+        analysis_combine(S, out);
+    }
 };
 
 }  // namespace sph
@@ -2233,6 +2300,13 @@ int bhs_c2p(bhs_engine* h, int run) {
 int bhs_run_diagnostics(bhs_engine* h, int run, double* out) {
     return guard(h, [&] {
         h->e->diagnostics(run, out);
+        return BHS_OK;
+    });
+}
+int bhs_run_analysis(bhs_engine* h, int run, double rho_min, double* out) {
+    return guard(h, [&] {
+        if (!out) throw std::invalid_argument("null output");
+        h->e->analysis(run, rho_min, out);
         return BHS_OK;
     });
 }

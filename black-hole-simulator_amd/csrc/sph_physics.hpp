@@ -493,4 +493,74 @@ SPH_HD double cell_dt(const Met<double>& m, double gam, const double w[8], const
     return dt;
 }
 
+// Per-zone terms of the analyser quantities (analysis::analyser,
+// src/fluid/data/quantitatives.cpp:109-193) in the consistent preset, from
+// the four-velocity u and the field four-vector b (as in source()):
+//   out[0] rho eps              thermal energy density      (thermE)
+//   out[1] b^2 / 2              magnetic energy density     (magE)
+//   out[2] p                    gas pressure                (P_beta's numerator)
+//   out[3] p + b^2 / 2          total pressure, rho >= rho_min   (alpha_SS)
+//   out[4] rho h u^r u_phi      Reynolds stress, rho >= rho_min  (alpha_SS, maxreyn)
+//   out[5] -b^r b_phi           Maxwell stress, rho >= rho_min   (alpha_SS, maxreyn)
+// and, on the zone row at the horizon (horizon = true; else 0):
+//   out[6] -rho u^r             rest-mass inflow            (Mdot)
+//   out[7] -T^r_phi             angular-momentum inflow     (Ldot)
+//   out[8] -(b^2 u^r u_t - b^r b_t)   electromagnetic energy outflow (L_BZ)
+// The caller multiplies by sqrt(-g) dr dtheta dphi (volume terms) or
+// sqrt(-g) dtheta dphi (horizon terms).  The reference's versions combine
+// coordinate components of v and B instead (Ldot subtracts B^r and B^phi
+// separately, quantitatives.cpp:109-113); these are the covariant ones.
+constexpr int NANA = 9;
+SPH_HD void analysis_terms(const Met<double>& m, double gam, const double w[8], double rho_min, bool horizon,
+                           double out[NANA]) {
+    Pt<double> s;
+    s.rho = w[WRHO];
+    s.p = (gam - 1.0) * w[WRHO] * w[WEPS];
+#pragma unroll
+    for (int d = 0; d < 3; d++) { s.u[d] = w[WU1 + d]; s.B[d] = w[WB1 + d]; }
+    pt_fill(m, gam, s);
+    const double ia = 1.0 / m.alpha;
+    double u[4], b[4];
+    u[0] = s.W * ia;
+    u[1] = s.W * (s.v[0] - m.beta[0] * ia);
+    u[2] = s.W * s.v[1];
+    u[3] = s.W * s.v[2];
+    b[0] = s.W * s.Bv * ia;
+#pragma unroll
+    for (int d = 0; d < 3; d++) b[1 + d] = (s.B[d] + m.alpha * b[0] * u[1 + d]) / s.W;
+    // lowering with g_tt = -alpha^2 + beta_r beta^r, g_ti = beta_i = gamma_ij beta^j
+    const double br_ = m.g[0] * m.beta[0], bph_ = m.g[2] * m.beta[0];
+    const double gtt = -m.alpha * m.alpha + br_ * m.beta[0];
+    auto low_t = [&](const double x[4]) { return gtt * x[0] + br_ * x[1] + bph_ * x[3]; };
+    auto low_ph = [&](const double x[4]) { return bph_ * x[0] + m.g[2] * x[1] + m.g[5] * x[3]; };
+    const double uph = low_ph(u), bph = low_ph(b), ut = low_t(u), bt = low_t(b);
+    const bool dense = s.rho >= rho_min;
+    out[0] = s.rho * w[WEPS];
+    out[1] = 0.5 * s.b2;
+    out[2] = s.p;
+    out[3] = dense ? s.p + 0.5 * s.b2 : 0.0;
+    out[4] = dense ? s.rhoh * u[1] * uph : 0.0;
+    out[5] = dense ? -b[1] * bph : 0.0;
+    out[6] = horizon ? -s.rho * u[1] : 0.0;
+    out[7] = horizon ? -((s.rhoh + s.b2) * u[1] * uph - b[1] * bph) : 0.0;
+    out[8] = horizon ? -(s.b2 * u[1] * ut - b[1] * bt) : 0.0;
+}
+// the quantities from the run's sums S[NANA + 1] of the terms times their
+// measures (S[NANA] = Phi_B): thermE, magE, Ldot, L_BZ, eta_BZ, phi_BH,
+// P_beta, alpha_SS, Maxwell/Reynolds, Mdot (the reference's guards against
+// vanishing denominators, quantitatives.cpp:129-193)
+SPH_HD void analysis_combine(const double S[NANA + 1], double out[10]) {
+    const double mdot = S[6], amd = mdot < 0.0 ? -mdot : mdot;
+    out[0] = S[0];
+    out[1] = S[1];
+    out[2] = S[7];
+    out[3] = S[8];
+    out[4] = amd > 1e-25 ? S[8] / amd : 0.0;
+    out[5] = amd > 1e-30 ? S[NANA] / sqrt(amd) : 0.0;
+    out[6] = S[1] > 1e-20 ? S[2] / S[1] : 1e30;
+    out[7] = S[3] > 1e-25 ? (S[4] + S[5]) / S[3] : 0.0;
+    out[8] = (S[4] < 0.0 ? -S[4] : S[4]) > 1e-30 ? S[5] / S[4] : 0.0;
+    out[9] = mdot;
+}
+
 }  // namespace sph

@@ -119,6 +119,16 @@ int bhs_c2p(bhs_engine* e, int run);
  * holding the outer horizon (analyser Mdot / Phi_B,
  * src/fluid/data/quantitatives.cpp:99-118). */
 int bhs_run_diagnostics(bhs_engine* e, int run, double* out);
+/* The analyser quantities that read derived primitives
+ * (src/fluid/data/quantitatives.cpp:109-193), in covariant form from u^mu and
+ * b^mu: out[10] = thermal energy int rho eps, magnetic energy int b^2/2 (both
+ * over sqrt(-g) dV), Ldot = -int T^r_phi and L_BZ = -int (b^2 u^r u_t -
+ * b^r b_t) over the horizon row (sqrt(-g) dtheta dphi), eta_BZ = L_BZ/|Mdot|,
+ * phi_BH = Phi_B/sqrt(|Mdot|) (the magnetically-arrested measure; the
+ * reference's MAD is Phi_B/Mdot), P_beta = int p / int b^2/2, alpha_SS =
+ * int (rho h u^r u_phi - b^r b_phi) / int (p + b^2/2) and the Maxwell to
+ * Reynolds stress ratio, both over zones with rho >= rho_min, and Mdot. */
+int bhs_run_analysis(bhs_engine* e, int run, double rho_min, double* out);
 /* bhs_run_diagnostics for callers that cannot allocate (code under graph
  * capture, tight serving loops, harnesses that forbid hipMalloc); no
  * counterpart in the reference.  Same out[8], computed without any device or

@@ -884,6 +884,40 @@ void os_diag(void* r, double* out) {
     out[5] = phib;
 }
 
+// the analyser quantities (bhs_run_analysis): sequential sums over the
+// run's blocks in table order of analysis_terms times sqrt(-g) dV or
+// sqrt(-g) dtheta dphi, and Phi_B from the radial face field at the horizon row
+void os_analysis(void* r, double rho_min, double* out) {
+    Run* R = (Run*)r;
+    const Lay& L = R->L;
+    const double disc = 1.0 - R->a * R->a - R->Q * R->Q;
+    const double rh = 1.0 + std::sqrt(disc > 0.0 ? disc : 0.0);
+    double S[NANA + 1] = {};
+    for (int b = 0; b < R->nb(); b++) {
+        const Level& v = R->lev[R->bl[b]];
+        const double* F0 = R->Bf0.data() + ((size_t)b * 3 + 0) * L.nf;
+        const int ih = (int)std::floor((std::log(rh) - v.lnr0) / v.dlnr);
+        for (int k = 0; k < L.bz; k++)
+            for (int j = 0; j < BY; j++)
+                for (int i = 0; i < BX; i++) {
+                    const int gi = R->bi[b] * BX + i, gj = R->bj[b] * BY + j;
+                    const int c = cidx(i, j, k);
+                    const int p = v.pt(gi, gj);
+                    const Met<double> m = load_met<double>(v.gc.data(), v.npt(), p);
+                    double w[8];
+                    for (int q = 0; q < 8; q++) w[q] = R->W[((size_t)b * 8 + q) * L.ncell + c];
+                    const bool hz = gi == ih;
+                    double t[NANA];
+                    analysis_terms(m, R->gam, w, rho_min, hz, t);
+                    const double sa = m.alpha * m.sg * v.dth * v.dph, sv = sa * v.dr[gi + 2];
+                    for (int q = 0; q < 6; q++) S[q] += t[q] * sv;
+                    for (int q = 6; q < NANA; q++) S[q] += t[q] * sa;
+                    if (hz) S[NANA] += 0.5 * std::fabs(F0[fidx(0, i, j, k)]) * v.dth * v.dph;
+                }
+    }
+    analysis_combine(S, out);
+}
+
 // pointwise entry points for pinning the shared physics
 void os_ptoc(const double* met17, double gam, const double* w, double* u) {
     Met<double> m = load_met<double>(met17, 1, 0);
@@ -921,6 +955,11 @@ void os_hll_point(const double* met17, double gam, const double* wl, const doubl
 void os_source_point(const double* gc, double gam, const double* w, double* out5) {
     Met<double> m = load_met<double>(gc, 1, 0);
     source(gc, 1, 0, m, gam, w, out5);
+}
+void os_analysis_point(const double* met17, double gam, const double* w, double rho_min, int horizon,
+                       double* out) {
+    Met<double> m = load_met<double>(met17, 1, 0);
+    analysis_terms(m, gam, w, rho_min, horizon != 0, out);
 }
 // metric record of (r, theta): out = the NGC cell-centre words
 void os_metric_point(double a, double Q, double r, double th, double* out) {

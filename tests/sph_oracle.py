@@ -34,6 +34,8 @@ def lib():
         L.os_c2p.argtypes = [ctypes.c_void_p]
         L.os_counters.argtypes = [ctypes.c_void_p, _llp]
         L.os_diag.argtypes = [ctypes.c_void_p, _dp]
+        L.os_analysis.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
+        L.os_analysis_point.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_double, ctypes.c_int, _dp]
         L.os_ptoc.argtypes = [_dp, ctypes.c_double, _dp, _dp]
         L.os_c2p_point.restype = ctypes.c_int
         L.os_c2p_point.argtypes = [_dp, ctypes.c_double, _dp, _dp]
@@ -126,3 +128,8 @@ class OracleRun:
         d = np.zeros(6)
         lib().os_diag(self._h, P(d))
         return dict(mass=d[0], energy=d[1], divb_max=d[2], divb_scale=d[3], mdot=d[4], phi_b=d[5])
+
+    def analysis(self, rho_min=0.0):
+        d = np.zeros(10)
+        lib().os_analysis(self._h, float(rho_min), P(d))
+        return dict(thermE=d[0], magE=d[1], Ldot=d[2], L_BZ=d[3], eta_BZ=d[4], phi_BH=d[5], P_beta=d[6], alpha_SS=d[7], maxwell_reynolds=d[8], mdot=d[9])

@@ -273,3 +273,24 @@ def test_diagnostics_against_oracle():
     for k in ("mass", "energy", "mdot", "phi_b"):
         assert abs(dg[k] - do[k]) <= n * 2.2e-16 * abs(do[k]) + 1e-300, k
     assert dg["mdot"] > 0 and dg["phi_b"] >= 0
+
+
+def test_analysis_against_oracle():
+    """The analyser quantities (bhs_run_analysis: thermal and magnetic
+    energy, Ldot, L_BZ, eta_BZ, phi_BH, P_beta, alpha_SS, Maxwell/Reynolds)
+    from the device's block sums against the oracle's sequential sums after
+    5 steps of the 3-level tree, within the summation-order bound."""
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    eng, run, orc, _, _ = make_pair(g, 16, blocks)
+    for _ in range(5):
+        eng.step([run])
+        orc.step(False)
+    rho_min = 1e-3
+    ag, ao = run.analysis(rho_min), orc.analysis(rho_min)
+    print("device", ag, "oracle", ao)
+    for k, vo in ao.items():
+        vg = ag[k]
+        assert np.isfinite(vg), k
+        assert abs(vg - vo) <= 1e-8 * abs(vo) + 1e-300, (k, vg, vo)
+    assert ag["thermE"] > 0 and ag["magE"] > 0 and ag["P_beta"] > 1.0

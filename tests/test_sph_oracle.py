@@ -463,3 +463,34 @@ def test_oracle_unmagnetised_torus_is_stationary():
     rel = np.abs(r1[core] - r0[core]) / r0[core]
     assert rel.max() < 2e-2, rel.max()
     assert np.isfinite(W1).all()
+
+
+def test_analysis_terms_against_stress_energy():
+    """analysis_terms (the analyser quantities' per-zone terms) against u^m,
+    b^m and T^mn built independently: b^2/2, rho h u^r u_phi, -b^r b_phi, the
+    inflows -rho u^r and -T^r_phi, and the electromagnetic part of -T^r_t,
+    with the rho_min cut and the horizon switch."""
+    rng = np.random.default_rng(21)
+    for a in (0.9375, 0.5):
+        for _ in range(60):
+            r = math.exp(rng.uniform(math.log(1.3), math.log(40)))
+            th = rng.uniform(0.1, math.pi - 0.1)
+            rec = metric_rec(a, 0.0, r, th)
+            w = random_prim(rng, rec)
+            u, b, T, g, W = tmunu(rec, w)
+            out = np.zeros(9)
+            so.lib().os_analysis_point(P(rec[:NMET].copy()), GAM, P(w), 0.0, 1, P(out))
+            b2 = b @ g @ b
+            rho, p = w[0], (GAM - 1) * w[0] * w[1]
+            h = 1 + GAM / (GAM - 1) * p / rho
+            ul, bl = g @ u, g @ b
+            Tmix = T @ g                                   # T^m_n
+            ref = [rho * w[1], b2 / 2, p, p + b2 / 2, rho * h * u[1] * ul[3], -b[1] * bl[3],
+                   -rho * u[1], -Tmix[1, 3], -(b2 * u[1] * ul[0] - b[1] * bl[0])]
+            for q in range(9):
+                sc = abs(ref[q]) + 1e-12 * (abs(rho * h * u[1] * ul[3]) + b2 + p + abs(Tmix[1, 3]) + 1e-300)
+                assert abs(out[q] - ref[q]) <= 1e-9 * sc, (q, out[q], ref[q])
+            # the density cut and the horizon switch
+            so.lib().os_analysis_point(P(rec[:NMET].copy()), GAM, P(w), 2 * rho, 0, P(out))
+            assert out[0] == pytest.approx(ref[0]) and out[1] == pytest.approx(ref[1])
+            assert (out[3:] == 0.0).all()
