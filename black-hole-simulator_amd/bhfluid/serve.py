@@ -85,11 +85,11 @@ class Server:
     """Continuous batching over one ``sph.Engine`` of ``budget_blocks``
     block slots (bz = 16)."""
 
-    def __init__(self, budget_blocks: int, mixed: bool = True, persistent: bool = False,
+    def __init__(self, budget_blocks: int, mixed: bool = True, schedule: int = 0,
                  clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
         self.eng = sph.Engine(16, budget_blocks)
         self.eng.set_precision(mixed)
-        self.eng.set_schedule(persistent)
+        self.eng.set_schedule(schedule)
         self.budget = budget_blocks
         self.used = 0
         self.clock = clock

@@ -158,8 +158,11 @@ class Engine:
     def set_precision(self, mixed: bool):
         self._check(self._L.bhs_set_precision(self._h, int(bool(mixed))), "set_precision")
 
-    def set_schedule(self, persistent: bool, wg_per_cu: int = 0):
-        self._check(self._L.bhs_set_schedule(self._h, int(bool(persistent)), int(wg_per_cu)), "set_schedule")
+    def set_schedule(self, schedule: int, wg_per_cu: int = 0):
+        """0 (or False): one kernel per sub-phase; 1 (or True): one
+        persistent queue-driven kernel per stage; 2: sub-phase kernels with
+        the fluxes and the hydro update fused per block (slab)."""
+        self._check(self._L.bhs_set_schedule(self._h, int(schedule), int(wg_per_cu)), "set_schedule")
 
     def set_variants(self, flux: int = 0, update: int = 0, c2p: int = 0):
         self._check(self._L.bhs_set_variants(self._h, int(flux), int(update), int(c2p)), "set_variants")

@@ -2,11 +2,12 @@
 
 The step's result does not depend on how it is launched (every schedule and
 variant is bit-identical), so the tuner only times.  It searches, one
-dimension at a time, the schedule (one kernel per sub-phase, or one
-persistent queue-driven kernel per RK stage with 1 or 2 workgroups per
-compute unit) and, for the sub-phase schedule, the workgroup size and
-occupancy hint of the flux, update and con2prim kernels; each candidate is
-timed over a few graph-replayed steps of the caller's own runs.  The choice
+dimension at a time, the schedule (one kernel per sub-phase; one persistent
+queue-driven kernel per RK stage with 1 or 2 workgroups per compute unit; or
+the sub-phase kernels with fluxes and hydro update fused per block, "slab")
+and, for the sub-phase schedules, the XCD numbering and the workgroup size
+and occupancy hint of the flux, update and con2prim kernels; each candidate
+is timed over a few graph-replayed steps of the caller's own runs.  The choice
 is cached per (device, block shape, block count, precision) in a JSON file
 (``$BHFLUID_TUNE_CACHE``, default ``~/.cache/bhfluid/tune.json``).
 """
@@ -21,7 +22,7 @@ import numpy as np
 
 from . import sph
 
-DEFAULT = dict(persistent=0, wg_per_cu=0, flux=0, update=0, c2p=1, xcd=1)
+DEFAULT = dict(schedule=0, wg_per_cu=0, flux=0, update=0, c2p=1, xcd=1)
 
 
 def _cache_path() -> Path:
@@ -36,7 +37,7 @@ def _key(eng: sph.Engine, runs, mixed: bool) -> str:
 
 
 def apply(eng: sph.Engine, cfg: dict):
-    eng.set_schedule(bool(cfg["persistent"]), int(cfg["wg_per_cu"]))
+    eng.set_schedule(int(cfg.get("schedule", cfg.get("persistent", 0))), int(cfg["wg_per_cu"]))
     eng.set_variants(int(cfg["flux"]), int(cfg["update"]), int(cfg["c2p"]))
     eng.set_xcd_remap(bool(cfg.get("xcd", 1)))
 
@@ -69,20 +70,23 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
     best = dict(DEFAULT)
     best_ms = _time(eng, runs, best, steps)
     timings.append((dict(best), best_ms))
-    for wg in (1, 2):
-        c = dict(best, persistent=1, wg_per_cu=wg)
+    for c in [dict(best, schedule=1, wg_per_cu=wg) for wg in (1, 2)] + [dict(best, schedule=2)]:
         t = _time(eng, runs, c, steps)
         timings.append((c, t))
         if t < best_ms:
             best, best_ms = c, t
-    if not best["persistent"]:
+    if best["schedule"] != 1:
         c = dict(best, xcd=0)
         t = _time(eng, runs, c, steps)
         timings.append((c, t))
         if t < best_ms:
             best, best_ms = c, t
-        for dim in ("flux", "update", "c2p"):
-            for v in range(1, 12 if dim == "flux" else 6):
+        # the slab schedule has no separate update kernel and computes only
+        # the side faces with k_flux (one face per thread)
+        dims = ("flux", "update", "c2p") if best["schedule"] == 0 else ("flux", "c2p")
+        for dim in dims:
+            top = 12 if (dim == "flux" and best["schedule"] == 0) else 6
+            for v in range(1, top):
                 c = dict(best, **{dim: v})
                 t = _time(eng, runs, c, steps)
                 timings.append((c, t))

@@ -115,6 +115,7 @@ struct KPS {
     // side is a leaf of the same level, which computes the same face as its
     // own, the block skips it and the EMFs read the neighbour's word.
     const int* side;      // [slot][6]: same-level leaf across r-, r+, theta-, theta+, phi-, phi+ (-1: none)
+    const int* sgrp;      // [3 directions][6 sides][2]: offset and count of those side faces in flf
     int reuse;            // 1: skip and share side faces (the sub-phase schedule)
 };
 
@@ -470,7 +471,8 @@ __device__ void phase_a_block(const KPS& P, int b, bool want_dt) {
 // the face of direction D (a template parameter: every state index is then
 // static and the states stay in registers)
 template <class Rt, int D>
-__device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int b, int i, int j, int k) {
+__device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, int b, int i, int j, int k,
+                                              Rt out[NFW]) {
     constexpr int d = D;
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
@@ -552,9 +554,13 @@ __device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int 
     Side<Rt> SL, SR;
     face_side(m, gam, PL, d, SL);
     face_side(m, gam, PR, d, SR);
-    Rt out[NFW];
     hll(m, SL, SR, out);
-    Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)d * NFW * P.nf + fidx(d, i, j, k);
+}
+template <class Rt, int D>
+__device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int b, int i, int j, int k) {
+    Rt out[NFW];
+    flux_face_val<Rt, D>(P, Bf, b, i, j, k, out);
+    Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)D * NFW * P.nf + fidx(D, i, j, k);
 #pragma unroll
     for (int q = 0; q < NFW; q++) F[(size_t)q * P.nf] = out[q];
 }
@@ -744,8 +750,10 @@ __device__ void face_update(const KPS& P, int b, int d, int i, int j, int k, dou
     else P.Bf0[x] = 0.5 * (P.Bf0[x] + (P.Bf1[x] + dt * rhs));
 }
 
-template <class Rt>
-__device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
+// flux divergence, sources and the RK2 combination of D, S_j, tau in zone a
+// of block b; fx(d, q, i, j, k) gives flux word q of face (d; i, j, k)
+template <class Rt, class FX>
+__device__ __forceinline__ void cell_hydro(const KPS& P, int b, int a, double dt, int stage, FX fx) {
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
     const RunPar& R = P.run[v.run];
@@ -757,7 +765,6 @@ __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
     double* Ub = P.U + (size_t)b * 8 * P.ncell;
     const double* Wb = P.W + (size_t)b * 8 * P.ncell;
     double* Usb = P.Us + (size_t)b * 5 * P.nint;
-    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
     double w[8];
 #pragma unroll
     for (int q = 0; q < 8; q++) w[q] = Wb[(size_t)q * P.ncell + c];
@@ -766,9 +773,9 @@ __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
     const double dri = gl(v.dr)[gi + 2];
 #pragma unroll
     for (int q = 0; q < 5; q++) {
-        const double dx = ((double)Fat(P, Fb, 0, q, i + 1, j, k) - (double)Fat(P, Fb, 0, q, i, j, k)) / dri;
-        const double dy = ((double)Fat(P, Fb, 1, q, i, j + 1, k) - (double)Fat(P, Fb, 1, q, i, j, k)) / v.dth;
-        const double dz = ((double)Fat(P, Fb, 2, q, i, j, k + 1) - (double)Fat(P, Fb, 2, q, i, j, k)) / v.dph;
+        const double dx = ((double)fx(0, q, i + 1, j, k) - (double)fx(0, q, i, j, k)) / dri;
+        const double dy = ((double)fx(1, q, i, j + 1, k) - (double)fx(1, q, i, j, k)) / v.dth;
+        const double dz = ((double)fx(2, q, i, j, k + 1) - (double)fx(2, q, i, j, k)) / v.dph;
         const double dU = src[q] - (dx + dy + dz);
         double* u = Ub + (size_t)q * P.ncell + c;
         if (stage == 0) {
@@ -778,10 +785,22 @@ __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
             *u = 0.5 * (Usb[(size_t)q * P.nint + a] + (*u + dt * dU));
         }
     }
+}
+// the cell field of zone (i, j, k) from the stage's new face fields
+__device__ __forceinline__ void cell_bfield(const KPS& P, int b, int i, int j, int k, int stage) {
+    double* Ub = P.U + (size_t)b * 8 * P.ncell;
+    const int c = cidx(i, j, k);
     const double* Bn = (stage == 0 ? P.Bf1 : P.Bf0) + (size_t)b * 3 * P.nf;
     Ub[(size_t)UB1 * P.ncell + c] = 0.5 * (Bn[fidx(0, i, j, k)] + Bn[fidx(0, i + 1, j, k)]);
     Ub[(size_t)UB2 * P.ncell + c] = 0.5 * (Bn[P.nf + fidx(1, i, j, k)] + Bn[P.nf + fidx(1, i, j + 1, k)]);
     Ub[(size_t)UB3 * P.ncell + c] = 0.5 * (Bn[2 * P.nf + fidx(2, i, j, k)] + Bn[2 * P.nf + fidx(2, i, j, k + 1)]);
+}
+template <class Rt>
+__device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
+    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    cell_hydro<Rt>(P, b, a, dt, stage,
+                   [&](int d, int q, int i, int j, int k) { return Fat(P, Fb, d, q, i, j, k); });
+    cell_bfield(P, b, a % BX, (a / BX) % BY, a / (BX * BY), stage);
 }
 
 __device__ __forceinline__ double run_dt(const KPS& P, int b) {
@@ -888,6 +907,108 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     const int a = (w % per) * TB + threadIdx.x;
     if (a >= P.nint) return;
     cell_update<Rt>(P, b, a, run_dt(P, b), stage);
+}
+
+// ---------------------------------------------------- slab schedule
+// Fluxes and the hydro update fused per block: one workgroup walks the
+// block's phi layers k = 0 .. bz-1; for layer k it forms the fluxes of the
+// layer's r and theta faces and of the phi faces at k + 1 (those at k are
+// kept from the layer before), keeps words 0-4 of each in LDS and stores the
+// EMF words 5, 6 for k_face, then updates the layer's zones from LDS.  Words
+// 0-4 never reach HBM.  The side faces of the sides the block does not share
+// are formed first (EMF words only); the cell field follows k_face (k_cellb).  Every
+// flux and update is computed by the functions of the sub-phase kernels, so
+// the results are the same bit for bit.
+constexpr int SL_X = (BX + 1) * BY, SL_Y = BX * (BY + 1), SL_Z = BX * BY;
+// nsplit workgroups share a block: part h takes layers [h bz/nsplit,
+// (h+1) bz/nsplit) and forms the phi faces at its first layer itself (those
+// at a shared boundary layer are formed by both parts, with the same bits);
+// the side faces are dealt out over the parts.
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsplit) {
+    __shared__ Rt sx[5][SL_X], sy[5][SL_Y], sz[2][5][SL_Z];
+    const int wg = wg_index(P);
+    const int b = P.list[wg / nsplit], part = wg % nsplit;
+    const int k0 = part * P.bz / nsplit, k1 = (part + 1) * P.bz / nsplit;
+    const double* Bf = buf ? P.Bf1 : P.Bf0;
+    const double dt = run_dt(P, b);
+    Rt* Fb = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    auto emf_words = [&](int D, int i, int j, int k, const Rt out[NFW]) {
+        const size_t x = fidx(D, i, j, k);
+        Fb[((size_t)D * NFW + 5) * P.nf + x] = out[5];
+        Fb[((size_t)D * NFW + 6) * P.nf + x] = out[6];
+    };
+    // the side faces of the sides this block does not share (only their EMF
+    // words are read; a shared side's faces are the neighbour's own)
+    {
+        const SideNb n = side_nb(P, b);
+        for (int g = part; g < 18; g += nsplit) {
+            const int e = g % 6, D = g / 6;
+            if ((e == 0 ? n.s[0] : e == 1 ? n.s[1] : e == 2 ? n.s[2] : e == 3 ? n.s[3] : e == 4 ? n.s[4] : n.s[5]) >= 0)
+                continue;
+            const int off = P.sgrp[2 * g], cnt = P.sgrp[2 * g + 1];
+            for (int t = threadIdx.x; t < cnt; t += TPB) {
+                int i, j, k;
+                unpack(P.flf[off + t], i, j, k);
+                Rt out[NFW];
+                if (D == 0) flux_face_val<Rt, 0>(P, Bf, b, i, j, k, out);
+                else if (D == 1) flux_face_val<Rt, 1>(P, Bf, b, i, j, k, out);
+                else flux_face_val<Rt, 2>(P, Bf, b, i, j, k, out);
+                emf_words(D, i, j, k, out);
+            }
+        }
+    }
+    for (int t = threadIdx.x; t < SL_Z; t += TPB) {
+        const int i = t % BX, j = t / BX;
+        Rt out[NFW];
+        flux_face_val<Rt, 2>(P, Bf, b, i, j, k0, out);
+#pragma unroll
+        for (int q = 0; q < 5; q++) sz[k0 & 1][q][t] = out[q];
+        emf_words(2, i, j, k0, out);
+    }
+    __syncthreads();
+    for (int k = k0; k < k1; k++) {
+        const int cur = k & 1;
+        for (int t = threadIdx.x; t < SL_X + SL_Y + SL_Z; t += TPB) {
+            Rt out[NFW];
+            if (t < SL_X) {
+                const int i = t % (BX + 1), j = t / (BX + 1);
+                flux_face_val<Rt, 0>(P, Bf, b, i, j, k, out);
+#pragma unroll
+                for (int q = 0; q < 5; q++) sx[q][t] = out[q];
+                emf_words(0, i, j, k, out);
+            } else if (t < SL_X + SL_Y) {
+                const int u = t - SL_X, i = u % BX, j = u / BX;
+                flux_face_val<Rt, 1>(P, Bf, b, i, j, k, out);
+#pragma unroll
+                for (int q = 0; q < 5; q++) sy[q][u] = out[q];
+                emf_words(1, i, j, k, out);
+            } else {
+                const int u = t - SL_X - SL_Y, i = u % BX, j = u / BX;
+                flux_face_val<Rt, 2>(P, Bf, b, i, j, k + 1, out);
+#pragma unroll
+                for (int q = 0; q < 5; q++) sz[cur ^ 1][q][u] = out[q];
+                emf_words(2, i, j, k + 1, out);
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < BX * BY; t += TPB)
+            cell_hydro<Rt>(P, b, k * BX * BY + t, dt, stage, [&](int d, int q, int ii, int jj, int kk) -> Rt {
+                if (d == 0) return sx[q][jj * (BX + 1) + ii];
+                if (d == 1) return sy[q][jj * BX + ii];
+                return sz[kk == k ? cur : cur ^ 1][q][jj * BX + ii];
+            });
+        __syncthreads();
+    }
+}
+// the cell field of every active zone from the stage's new faces
+__global__ __launch_bounds__(TPB) void k_cellb(KPS P, int stage) {
+    const int per = (P.nint + TPB - 1) / TPB;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int a = (w % per) * TPB + threadIdx.x;
+    if (a >= P.nint) return;
+    cell_bfield(P, b, a % BX, (a / BX) % BY, a / (BX * BY), stage);
 }
 
 // ---------------------------------------------------- persistent schedule
@@ -1376,6 +1497,7 @@ public:
     DB<int> ghc, ghf, flf, owf;
     int nghc = 0, nghf = 0, nflf = 0, nowf = 0, nflp = 0;
     DB<int> flp;
+    DB<int> sgrp;                 // side-face groups of flf (offset, count) per direction and side
     std::vector<char> own_mask;   // [3][nf]: own faces (the evolved face field)
     DB<int> list;                 // active block slots (sorted by run)
     DB<int> runlist;              // active runs
@@ -1390,7 +1512,7 @@ public:
     std::vector<int> act_runs;    // runs of the current active list
     std::vector<int> act_slots;
     bool mixed = false;
-    int mode = 0;                 // 0 multi-kernel, 1 persistent
+    int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update)
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
     int var_flux = 0, var_update = 0, var_c2p = 1;  // launch variants (VAR_TB, VAR_MW)
     bool xcd_remap = true;        // XCD-contiguous workgroup numbering
@@ -1480,11 +1602,15 @@ public:
                         }
                     }
         }
+        std::vector<int> sg(36);
         for (int d = 0; d < 3; d++)
             for (int e = 0; e < 6; e++) {
+                sg[2 * (6 * d + e)] = (int)ff_.size();
+                sg[2 * (6 * d + e) + 1] = (int)sides[d][e].size();
                 ff_.insert(ff_.end(), sides[d][e].begin(), sides[d][e].end());
                 gf_.insert(gf_.end(), sides[d][e].begin(), sides[d][e].end());
             }
+        sgrp.upload(sg.data(), sg.size(), stream);
         own_mask.assign((size_t)3 * L.nf, 0);
         for (int x : ow_) {
             int i, j, k;
@@ -1532,7 +1658,8 @@ public:
         P.qctr = qctr.p;
         P.xcd = xcd_remap ? 1 : 0;
         P.side = side.p;
-        P.reuse = mode == 0 ? 1 : 0;
+        P.sgrp = sgrp.p;
+        P.reuse = mode != 1 ? 1 : 0;
         return P;
     }
 
@@ -1855,6 +1982,23 @@ public:
             tm("k_ghost_faces", [&] {
                 hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
             });
+            if (mode == 2) {
+                // the side faces (after the own faces in the face list), then
+                // fluxes and hydro update fused per block
+                // the flux variant v picks how many workgroups share a
+                // block's layers: 2^(v mod 5), at most bz
+                const int ns = std::min(1 << (var_flux % 5), L.bz);
+                tm("k_slab", [&] {
+                    hipLaunchKernelGGL(k_slab<Rt>, dim3(nl * ns), dim3(TPB), 0, stream, P, buf, stage, ns);
+                });
+                tm("k_face", [&] {
+                    hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+                });
+                tm("k_cellb", [&] {
+                    hipLaunchKernelGGL(k_cellb, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+                });
+                continue;
+            }
             tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, nflp, stream, P, buf); });
             tm("k_face", [&] {
                 hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
@@ -2184,7 +2328,8 @@ int bhs_set_precision(bhs_engine* h, int mixed) {
 int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
     return guard(h, [&] {
         if (wg_per_cu < 0 || wg_per_cu > 32) throw std::invalid_argument("wg_per_cu out of range");
-        h->e->mode = persistent ? 1 : 0;
+        if (persistent < 0 || persistent > 2) throw std::invalid_argument("schedule must be 0, 1 or 2");
+        h->e->mode = persistent;
         h->e->wg_per_cu = wg_per_cu;
         return BHS_OK;
     });

@@ -142,9 +142,12 @@ int bhs_run_diagnostics_static(bhs_engine* e, int run, double* out);
 
 /* Launch configuration (no reference counterpart): mixed 1 = FP32
  * reconstruction, Riemann fluxes and EMFs (con2prim, sources, update and
- * reductions FP64); persistent 1 = one queue-driven kernel per RK stage with
- * wg_per_cu workgroups per compute unit (0 = as many as fit); graphs on by
- * default. */
+ * reductions FP64).  Schedule ("persistent"): 0 = one kernel per sub-phase;
+ * 1 = one queue-driven kernel per RK stage with wg_per_cu workgroups per
+ * compute unit (0 = as many as fit); 2 = slab: the sub-phase kernels with the
+ * fluxes and the hydro update fused (a block's phi layers walked by 2^(flux
+ * variant mod 5) workgroups, the fluxes kept in LDS).  Graphs on by default.
+ * Every schedule gives the same bits. */
 int bhs_set_precision(bhs_engine* e, int mixed);
 int bhs_set_schedule(bhs_engine* e, int persistent, int wg_per_cu);
 int bhs_use_graph(bhs_engine* e, int enable);
@@ -153,7 +156,8 @@ int bhs_use_graph(bhs_engine* e, int enable);
  * thread per line): workgroup size and minimum wavefronts per SIMD of each
  * are given by bhs_variant_info.  Results do not depend on the variant; the
  * autotuner (bhfluid.tune) times them.  Defaults: flux 0, update 0,
- * con2prim 1. */
+ * con2prim 1.  Under the slab schedule the flux variant v sets the
+ * workgroups per block to 2^(v mod 5) (at most bz). */
 int bhs_set_variants(bhs_engine* e, int flux, int update, int c2p);
 int bhs_variant_info(int variant, int* tpb, int* min_waves);
 /* Number the workgroups of the multi-workgroup kernels so that each of the

@@ -121,12 +121,22 @@ def test_amr_three_levels_fp64():
     _steps(eng, run, orc, 4, False, "AMR3")
 
 
-def test_persistent_schedule_identical():
-    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
-    blocks = sph.amr_blocks(g, 3)
-    a = sph.Engine(16, len(blocks))
-    b = sph.Engine(16, len(blocks))
-    b.set_schedule(True)
+@pytest.mark.parametrize("schedule,mixed,bz", [(1, False, 16), (2, False, 16), (2, True, 16), (2, False, 1)])
+def test_persistent_schedule_identical(schedule, mixed, bz):
+    """The persistent (1) and slab (2: fluxes and hydro update fused per
+    block) schedules give the sub-phase schedule's state bit for bit (3-level
+    tree; bz = 1: the axisymmetric C1 grid)."""
+    if bz == 16:
+        g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+        blocks = sph.amr_blocks(g, 3)
+    else:
+        g = sph.c1_grid()
+        blocks = sph.uniform_blocks(g, 1)
+    a = sph.Engine(bz, len(blocks))
+    b = sph.Engine(bz, len(blocks))
+    a.set_precision(mixed)
+    b.set_precision(mixed)
+    b.set_schedule(schedule)
     ra = a.add_run(sph.PHYSICS, g, blocks)
     rb = b.add_run(sph.PHYSICS, g, blocks)
     ra.init_torus(sph.TORUS, 42)
