@@ -169,7 +169,7 @@ def cpu_baseline(sph, g, blocks, nsteps=2):
     p = sph.PHYSICS
     cfg = dict(bz=16, a=p["a"], Q=p["Q"], gamma=p["gamma"], cfl=p["cfl"], rho_fl=p["rho_floor"],
                eps_fl=p["eps_floor"], sigma_max=p["sigma_max"], w_max=p["gamma_max"], nr=g["nr"], nth=g["nth"],
-               nph=g["nph"], r_in=g["r_in"], r_out=g["r_out"], nlev=g["nlevels"])
+               nph=g["nph"], r_in=g["r_in"], r_out=g["r_out"], nlev=g["nlevels"], riemann=sph.riemann_code(p))
     orc = OracleRun(cfg, blocks)
     orc.set_state(U, W, Bf)
     del U, W, Bf
@@ -214,6 +214,9 @@ def main():
     ap.add_argument("--kernels-only", choices=("fp64", "mixed"),
                     help="for counter collection: C3 initial data, then --steps eager steps of the step's kernels "
                          "in the given precision with the default launch configuration; prints one JSON line")
+    ap.add_argument("--launch", default=None,
+                    help="with --kernels-only: the launch configuration as JSON (as config.launch_fp64 or "
+                         "config.launch_mixed of a bench line); default: the untuned one")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="for counter collection: three plain 1 GiB device copies (reads and writes of known size)")
     args = ap.parse_args()
@@ -232,13 +235,14 @@ def main():
     if args.kernels_only:
         g, blocks = c3_setup(sph)
         eng, run = make_engine(sph, g, blocks, args.kernels_only == "mixed")
-        tune.apply(eng, tune.DEFAULT)
+        launch = dict(tune.DEFAULT, **json.loads(args.launch)) if args.launch else dict(tune.DEFAULT)
+        tune.apply(eng, launch)
         eng.use_graph(False)
         eng.step([run], args.steps, want_dt=False)
         eng.sync()
         print(json.dumps({"mode": "kernels-only", "precision": args.kernels_only, "steps": args.steps,
                           "blocks": int(len(blocks)), "zones": int(len(blocks)) * 4096,
-                          "launch": tune.DEFAULT}))
+                          "launch": launch}))
         return
 
     g, blocks = c3_setup(sph)
@@ -301,7 +305,7 @@ def main():
         "data": "synthetic: Fishbone-Moncrief torus initial data computed on the device (PCG64 seed 42 perturbation)",
         "config": {
             "workload": "3-level AMR FM torus (C3): a=0.9375, Gamma=13/9, r 1.2-40 log x theta x phi, level 0 "
-                        "128x64x64, 16^3 blocks + 2 ghosts refined where rho>0.1 rho_max, CFL 0.4, PLM minmod + HLL "
+                        "128x64x64, 16^3 blocks + 2 ghosts refined where rho>0.1 rho_max, CFL 0.4, PLM minmod + HLLD "
                         "+ CT + RK2, consistent preset",
             "blocks": int(len(blocks)), "blocks_per_level": per_level, "zones": zones,
             "ms_per_step_mean": float(np.mean(ms64)), "ms_per_step_p90": float(np.percentile(ms64, 90)),
@@ -311,7 +315,7 @@ def main():
             "kernels_source": "a separate pass of 4 steps with eager launches bracketed by HIP events",
         },
         "mixed": {"value": value_mx, "ms_per_step": medmx, "speedup_vs_fp64": value_mx / value,
-                  "dtype": "f32 reconstruction/HLL/EMF, f64 con2prim/sources/update/reductions",
+                  "dtype": "f32 reconstruction/HLLD/EMF, f64 con2prim/sources/update/reductions",
                   "dominant_kernel": dommx, "dominant_avg_launch_ms": alonemx["avg_ms"], **acc},
         "roofline": roofline,
         "step_roofline": step_roof,

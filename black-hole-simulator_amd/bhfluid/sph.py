@@ -40,8 +40,21 @@ BX = BY = 16
 G = 2
 
 
+_PHYS_WORDS = ("a", "Q", "gamma", "cfl", "rho_floor", "eps_floor", "sigma_max", "gamma_max")
+# Riemann solvers (include/bhsph.h BHS_HLL, BHS_HLLD)
+RIEMANN = {"hll": 0, "hlld": 1}
+
+
 class Physics(ctypes.Structure):
-    _fields_ = [(n, _d) for n in ("a", "Q", "gamma", "cfl", "rho_floor", "eps_floor", "sigma_max", "gamma_max")]
+    _fields_ = [(n, _d) for n in _PHYS_WORDS] + [("riemann", _i), ("reserved", _i)]
+
+
+def riemann_code(physics: dict) -> int:
+    """The solver code of a physics dict ("hlld" when absent)."""
+    name = physics.get("riemann", "hlld")
+    if name not in RIEMANN:
+        raise ValueError(f"riemann must be one of {sorted(RIEMANN)}, got {name!r}")
+    return RIEMANN[name]
 
 
 class Grid(ctypes.Structure):
@@ -139,7 +152,7 @@ class Engine:
         return int(lib().bhs_block_bytes(bz))
 
     def add_run(self, physics: dict, grid: dict, blocks) -> "Run":
-        ph = Physics(*[float(physics[n]) for n, _ in Physics._fields_])
+        ph = Physics(*[float(physics[n]) for n in _PHYS_WORDS], riemann_code(physics), 0)
         g = Grid(int(grid["nr"]), int(grid["nth"]), int(grid.get("nph", 1)), int(grid.get("nlevels", 1)),
                  float(grid["r_in"]), float(grid["r_out"]))
         b = np.ascontiguousarray(blocks, dtype=np.int32).reshape(-1, 4)
@@ -277,7 +290,7 @@ class Run:
 # BASELINE physics of the torus runs: a = 0.9375, Gamma = 13/9, CFL 0.4, the
 # reference's floor constants (src/fluid/grid.hpp:140-143)
 PHYSICS = dict(a=0.9375, Q=0.0, gamma=13.0 / 9.0, cfl=0.4, rho_floor=1e-7, eps_floor=1e-5, sigma_max=100.0,
-               gamma_max=50.0)
+               gamma_max=50.0, riemann="hlld")
 TORUS = dict(r_in=6.0, r_max=12.0, kappa=1e-3, beta=100.0, a_cut=0.2, perturbation=0.04)
 R_IN, R_OUT = 1.2, 40.0
 

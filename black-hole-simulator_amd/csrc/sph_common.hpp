@@ -109,7 +109,7 @@ struct RunPar {
     double w_max;            // Lorentz-factor ceiling
     int nlev, rl0;           // levels and the RL index of level 0
     int nz;                  // 1: axisymmetric (no phi CFL), else BZ
-    int pad;
+    int riemann;             // 0: HLL, 1: HLLD
 };
 
 // Block record: its (run, level) and block coordinates at that level.

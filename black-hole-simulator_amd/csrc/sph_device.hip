@@ -495,6 +495,12 @@ __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, in
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
         m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
         m.sg = (Rt)tab[M_SG * np + p];
+        if (R.riemann) {
+            // the face frame of the HLLD solver also needs gamma^r phi and,
+            // on theta faces, gamma^rr
+            m.gi[2] = (Rt)tab[M_I13 * np + p];
+            if (d == 1) m.gi[0] = (Rt)tab[M_I11 * np + p];
+        }
     }
     constexpr int o0 = d == 0, o1 = d == 1, o2 = d == 2;
     constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
@@ -551,6 +557,10 @@ __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, in
     PL.B[d] = bn;
     PR.B[d] = bn;
     const Rt gam = (Rt)R.gam;
+    if (R.riemann) {
+        hlld(m, m.gi[2], gam, PL, PR, d, out);
+        return;
+    }
     Side<Rt> SL, SR;
     face_side(m, gam, PL, d, SL);
     face_side(m, gam, PR, d, SR);
@@ -622,6 +632,12 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
         m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
         m.sg = (Rt)tab[M_SG * np + p];
+        if (R.riemann) {
+            // the face frame of the HLLD solver also needs gamma^r phi and,
+            // on theta faces, gamma^rr
+            m.gi[2] = (Rt)tab[M_I13 * np + p];
+            if (d == 1) m.gi[0] = (Rt)tab[M_I11 * np + p];
+        }
     };
     Met<Rt> m;
     if (d == 2) load_m(gl(v.gc), (B.bj * BY + j0 + 2) * (v.nr + 5) + (gi + 2), m);
@@ -660,11 +676,15 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         const Rt bn = m.sg > Rt(0) ? (Rt)Bfd[fx] / m.sg : Rt(0);
         PL.B[d] = bn;
         PR.B[d] = bn;
-        Side<Rt> SL, SR;
-        face_side(m, gam, PL, d, SL);
-        face_side(m, gam, PR, d, SR);
         Rt out[NFW];
-        hll(m, SL, SR, out);
+        if (R.riemann) {
+            hlld(m, m.gi[2], gam, PL, PR, d, out);
+        } else {
+            Side<Rt> SL, SR;
+            face_side(m, gam, PL, d, SL);
+            face_side(m, gam, PR, d, SR);
+            hll(m, SL, SR, out);
+        }
 #pragma unroll
         for (int q = 0; q < NFW; q++) Fd[(size_t)q * P.nf + fx] = out[q];
 #pragma unroll
@@ -1670,6 +1690,7 @@ public:
         if (L.bz == 1 && g.nlevels != 1) throw std::invalid_argument("axisymmetric runs have one level");
         if (g.nr % BX || g.nth % BY || (L.bz > 1 && g.nph % L.bz)) throw std::invalid_argument("grid not a whole number of blocks");
         if (!(g.r_in > 0.0) || !(g.r_out > g.r_in)) throw std::invalid_argument("need 0 < r_in < r_out");
+        if (ph.riemann != BHS_HLL && ph.riemann != BHS_HLLD) throw std::invalid_argument("riemann must be BHS_HLL or BHS_HLLD");
         if ((long long)free_slots.size() < nb) throw std::length_error("not enough free block slots");
         int id = -1;
         for (size_t r = 0; r < runs.size(); r++)
@@ -1682,6 +1703,7 @@ public:
         R.par.a = ph.a; R.par.Q = ph.Q; R.par.gam = ph.gamma; R.par.cfl = ph.cfl;
         R.par.rho_fl = ph.rho_floor; R.par.eps_fl = ph.eps_floor; R.par.sigma_max = ph.sigma_max;
         R.par.w_max = ph.gamma_max; R.par.nlev = g.nlevels; R.par.nz = L.bz;
+        R.par.riemann = ph.riemann;
         // levels and their tables
         std::vector<std::vector<int>> lut(g.nlevels);
         long long z0 = 0;

@@ -20,10 +20,13 @@
 // con2prim is a one-dimensional Newton iteration on Z = rho h W^2 as the
 // reference's (conserved.cpp:80-230) with the correct residual and
 // derivative, started from U alone so that the result is a function of U.
-// The Riemann solver is HLL with the reference's signal-speed structure
-// (alpha (v +- c) - beta, hlld.cpp:240-241) written for a general spatial
-// metric; see the build's design notes for why the reference's HLLD
-// intermediate states are not used here.
+// The Riemann solver is HLLD (hlld below): the reference's five-wave
+// structure (hlld.cpp:230-312: fast waves, Alfven waves, contact, HLL
+// fallback) with the relativistic intermediate states of Mignone, Ugliano &
+// Bodo (2009), solved in the orthonormal frame of the normal observer at the
+// face; HLL (hll), with the signal speeds alpha (v +- c) - beta of
+// hlld.cpp:240-241 written for a general spatial metric, is its fallback and
+// a selectable solver of its own.
 //
 // Everything is a template on the arithmetic type R so that the same code
 // gives the FP64 step and the FP32 reconstruction / Riemann / EMF path of
@@ -215,6 +218,561 @@ SPH_HD void hll(const Met<R>& m, const Side<R>& L, const Side<R>& Rt, R out[NFW]
     for (int q = 0; q < 7; q++) out[q] = (sr * L.f[q] - sl * Rt.f[q] + slsr * (Rt.u[q] - L.u[q])) * id;
 }
 
+// The con2prim energy residual in Z = rho h W^2 (its definition is with
+// c2p below), for any arithmetic type: the FP64 con2prim and the HLLD
+// solver's initial total pressure both use it.
+template <class R>
+struct C2PInT {
+    R D, E, S2, SB, B2;
+};
+using C2PIn = C2PInT<double>;
+template <class R>
+SPH_HD R c2p_v2(const C2PInT<R>& q, R Z) {
+    const R ZB = Z + q.B2;
+    return (Z * Z * q.S2 + q.SB * q.SB * (R(2) * Z + q.B2)) / (Z * Z * ZB * ZB);
+}
+template <class R>
+SPH_HD R c2p_f(const C2PInT<R>& q, R gam, R Z, R& df) {
+    const R ZB = Z + q.B2;
+    const R v2 = c2p_v2(q, Z);
+    const R omv = R(1) - v2;
+    const R sq = sqrt(omv);
+    const R gg = (gam - R(1)) / gam;
+    const R p = gg * (Z * omv - q.D * sq);
+    const R Z3 = Z * Z * Z;
+    const R dv2 = R(-2) * (q.S2 / (ZB * ZB * ZB) + q.SB * q.SB * (R(3) * Z * Z + R(3) * q.B2 * Z + q.B2 * q.B2) /
+                                                       (Z3 * ZB * ZB * ZB));
+    const R dp = gg * (omv - Z * dv2 + q.D * dv2 / (R(2) * sq));
+    df = R(1) - dp + R(0.5) * q.B2 * dv2 + q.SB * q.SB / Z3;
+    return Z + q.B2 - p - R(0.5) * (q.B2 * omv + q.SB * q.SB / (Z * Z)) - q.E;
+}
+
+// ------------------------------------------------------------------- HLLD
+// The five-wave relativistic HLLD solver of Mignone, Ugliano & Bodo (2009,
+// MNRAS 393, 1141; "MUB09") in the orthonormal frame of the normal observer
+// at the face, where special-relativistic formulas hold (Pons et al. 1998;
+// White, Stone & Gammie 2016).  This is the wave structure of the
+// reference's hlld (hlld.cpp:230-312: outer fast waves, Alfven waves around
+// a contact, HLL fallback) with intermediate states that satisfy the
+// relativistic jump conditions.
+//
+// Frame: x along the unit normal n_i = delta_i^n / sqrt(gamma^nn), y and z
+// orthonormal in the face, cyclic after n (n = r: theta, phi; n = theta:
+// phi, r_perp; n = phi: r, theta).  The Kerr-Schild chart's block-diagonal
+// metric gives the triads in closed form:
+//   n = r:     e_x = gamma^{k r} / sqrt(gamma^rr), e_y = e_theta / sqrt(g_thth),
+//              e_z = e_phi / sqrt(g_phph)
+//   n = theta: e_x = e_theta / sqrt(g_thth), e_y = e_phi / sqrt(g_phph),
+//              e_z = gamma^{k r} / sqrt(gamma^rr)
+//   n = phi:   e_x = gamma^{k phi} / sqrt(gamma^phph), e_y = e_r / sqrt(g_rr),
+//              e_z = e_theta / sqrt(g_thth)
+// Frame components: X^(a) = e_a^k X_k.  The coordinate surface x^n = const
+// moves through the frame with normal speed v_s = beta^n / (alpha sqrt(gamma^nn)),
+// and the coordinate flux is F^n = sqrt(gamma) alpha sqrt(gamma^nn) M (F^ - v_s U^)
+// (M maps frame components back: S_j with the lowered triad, B^k with the
+// triad itself), plus sqrt(gamma) beta^k B^n in the B^k flux (k != n), the
+// part of B^k V^n - B^n V^k that the normal observer's frame does not see.
+
+// the frame scalars of a face of direction n
+template <class R>
+struct Frame {
+    R rn, irn;    // sqrt(gamma^nn) and its inverse
+    R s1, is1;    // sqrt of the first transverse metric word and its inverse
+    R s2, is2;    // the second (n = theta: sqrt(gamma^rr), the r_perp axis)
+    R g13, gi13;  // gamma_r phi, gamma^r phi
+    R vs;         // normal speed of the coordinate surface in the frame
+    R c;          // sqrt(gamma) alpha sqrt(gamma^nn)
+};
+template <class R>
+SPH_HD Frame<R> make_frame(const Met<R>& m, R gi13, int n) {
+    Frame<R> f;
+    f.rn = sqrt(n == 0 ? m.gi[0] : n == 1 ? m.gi[3] : m.gi[5]);
+    f.irn = R(1) / f.rn;
+    f.s1 = sqrt(n == 0 ? m.g[3] : n == 1 ? m.g[5] : m.g[0]);
+    f.s2 = sqrt(n == 0 ? m.g[5] : n == 1 ? m.gi[0] : m.g[3]);
+    f.is1 = R(1) / f.s1;
+    f.is2 = R(1) / f.s2;
+    f.g13 = m.g[2];
+    f.gi13 = gi13;
+    f.vs = n == 0 ? m.beta[0] * f.irn / m.alpha : R(0);
+    f.c = m.sg * m.alpha * f.rn;
+    return f;
+}
+// frame components (x, y, z) of a vector given contravariant X and lowered Xl
+template <class R>
+SPH_HD void to_frame(const Frame<R>& f, int n, const R X[3], const R Xl[3], R o[3]) {
+    if (n == 0) {
+        o[0] = X[0] * f.irn; o[1] = Xl[1] * f.is1; o[2] = Xl[2] * f.is2;
+    } else if (n == 1) {
+        o[0] = X[1] * f.irn; o[1] = Xl[2] * f.is1; o[2] = X[0] * f.is2;
+    } else {
+        o[0] = X[2] * f.irn; o[1] = Xl[0] * f.is1; o[2] = Xl[1] * f.is2;
+    }
+}
+
+// One side of a face in the frame: velocity, field (x = the common normal
+// field), the seven conserved words D, m_x, m_y, m_z, E (= tau + D), B_y,
+// B_z, their fluxes, and the fast signal speeds (MUB09 sec. 2).
+template <class R>
+struct FSide {
+    R v[3], B[3], iW2, vB, pt;
+    R u[7], f[7], lm, lp;
+};
+template <class R>
+SPH_HD void frame_side(const Met<R>& m, const Frame<R>& fr, R gam, Pt<R>& s, int n, R bx, FSide<R>& o) {
+    pt_fill(m, gam, s);
+    to_frame(fr, n, s.v, s.vl, o.v);
+    R Bf[3];
+    to_frame(fr, n, s.B, s.Bl, Bf);
+    o.B[0] = bx;
+    o.B[1] = Bf[1];
+    o.B[2] = Bf[2];
+    o.iW2 = s.iW * s.iW;
+    o.vB = s.Bv;
+    o.pt = s.ps;
+    const R E = s.rhoh * s.W * s.W + s.B2 - s.ps;
+    o.u[0] = s.rho * s.W;
+#pragma unroll
+    for (int a = 0; a < 3; a++) o.u[1 + a] = (E + s.ps) * o.v[a] - s.Bv * o.B[a];
+    o.u[4] = E;
+    o.u[5] = o.B[1];
+    o.u[6] = o.B[2];
+    const R vx = o.v[0];
+    o.f[0] = o.u[0] * vx;
+#pragma unroll
+    for (int a = 0; a < 3; a++) o.f[1 + a] = o.u[1 + a] * vx - bx * (o.B[a] * o.iW2 + s.Bv * o.v[a]);
+    o.f[1] += s.ps;
+    o.f[4] = o.u[1];
+    o.f[5] = o.B[1] * vx - bx * o.v[1];
+    o.f[6] = o.B[2] * vx - bx * o.v[2];
+    // fast speeds: the coordinate formula of pt_speeds with gamma^nn = 1,
+    // alpha = 1, beta = 0
+    R cf2 = (gam * s.p + s.b2) / (s.rhoh + s.b2);
+    cf2 = rmin(cf2, R(1));
+    const R v2 = R(1) - o.iW2;
+    const R den = R(1) - v2 * cf2;
+    R disc = (R(1) - v2) * (den - vx * vx * (R(1) - cf2));
+    disc = rmax(disc, R(0));
+    const R sq = sqrt(cf2 * disc);
+    const R a = vx * (R(1) - cf2);
+    const R id = R(1) / den;
+    o.lp = (a + sq) * id;
+    o.lm = (a - sq) * id;
+}
+
+// The state behind a fast wave of speed lam with total pressure p, from
+// Rv = lam U - F(U) of the outer state (MUB09 eqs. 21-31): velocity, field,
+// total enthalpy w = rho h + b^2, eta = sgn sqrt(w) and the vector K whose
+// x component is the Alfven speed (MUB09 eq. 43).
+template <class R>
+struct AState {
+    R v[3], B[3], vB, w, eta, K[3];
+};
+template <class R>
+SPH_HD void fast_state(const R Rv[7], R lam, R bx, R p, R sgn, AState<R>& a) {
+    const R Rmx = Rv[1], Rmy = Rv[2], Rmz = Rv[3], RE = Rv[4], RBy = Rv[5], RBz = Rv[6];
+    const R l2 = R(1) - lam * lam;
+    const R A = Rmx - lam * RE + p * l2;
+    const R G = RBy * RBy + RBz * RBz;
+    const R C = RBy * Rmy + RBz * Rmz;
+    const R Q = -A - G + bx * bx * l2;
+    const R X = bx * (A * lam * bx + C) - (A + G) * (lam * p + RE);
+    const R iX = R(1) / X;
+    a.v[0] = (bx * (A * bx + lam * C) - (A + G) * (p + Rmx)) * iX;
+    const R cc = C + bx * (lam * Rmx - RE);
+    a.v[1] = (Q * Rmy + RBy * cc) * iX;
+    a.v[2] = (Q * Rmz + RBz * cc) * iX;
+    const R ilv = R(1) / (lam - a.v[0]);
+    a.B[0] = bx;
+    a.B[1] = (RBy - bx * a.v[1]) * ilv;
+    a.B[2] = (RBz - bx * a.v[2]) * ilv;
+    a.vB = a.v[0] * bx + a.v[1] * a.B[1] + a.v[2] * a.B[2];
+    a.w = p + (RE - (a.v[0] * Rmx + a.v[1] * Rmy + a.v[2] * Rmz)) * ilv;
+    a.eta = sgn * sqrt(rmax(a.w, R(0)));
+    const R iden = R(1) / (lam * p + RE + a.eta * bx);
+    a.K[0] = (Rmx + p + a.eta * lam * bx) * iden;
+    a.K[1] = (Rmy + a.eta * RBy) * iden;
+    a.K[2] = (Rmz + a.eta * RBz) * iden;
+}
+// conserved words (D, m, E, B_y, B_z) of a state from its velocity v, field
+// B, v.B, total pressure p, D and E
+template <class R>
+SPH_HD void fan_cons(const R v[3], const R B[3], R vB, R p, R D, R E, R u[7]) {
+    u[0] = D;
+#pragma unroll
+    for (int a = 0; a < 3; a++) u[1 + a] = (E + p) * v[a] - vB * B[a];
+    u[4] = E;
+    u[5] = B[1];
+    u[6] = B[2];
+}
+// Next to the contact: the field between the Alfven waves is the HLL
+// average of the induction equation over [K_x^L, K_x^R] (MUB09 eq. 45),
+// B^c_t = N_t / dK with dK = K_x^R - K_x^L, and the velocity on the side of
+// Alfven state a is v = K - B^c (1 - K^2) / (eta - K.B^c) (MUB09 eq. 47).
+// Both are formed here without dividing by dK: with Yh = (1 - K^2) /
+// (dK (eta - K_x B_x) - K_t.N_t), v_x = K_x - dK Yh B_x and v_t = K_t - Yh N_t.
+// The normal velocity jump across the contact is then dK (1 - B_x (Yh_R -
+// Yh_L)); its factor dK vanishes where the two Alfven waves coincide, which
+// is no fan, so the contact condition solved is g = 1 - B_x (Yh_R - Yh_L) = 0.
+template <class R>
+SPH_HD R contact_side(const AState<R>& a, R dK, R bx, const R N[3], R vc[3]) {
+    const R K2 = a.K[0] * a.K[0] + a.K[1] * a.K[1] + a.K[2] * a.K[2];
+    const R Yh = (R(1) - K2) / (dK * (a.eta - a.K[0] * bx) - (a.K[1] * N[1] + a.K[2] * N[2]));
+    vc[0] = a.K[0] - dK * Yh * bx;
+    vc[1] = a.K[1] - Yh * N[1];
+    vc[2] = a.K[2] - Yh * N[2];
+    return Yh;
+}
+
+template <class R>
+struct HlldTol {};
+// Secant tolerance and first step per precision; bdeg: a normal field with
+// B_x^2 < bdeg p is degenerate (Alfven waves merged with the contact).  The
+// threshold is the same in both precisions, so that FP32 and FP64 solve the
+// same fan, and large enough that the five-wave states, whose contact field
+// divides by K_x^R - K_x^L ~ B_x, stay well conditioned in FP32 (on the
+// axisymmetric torus over 200 steps, FP32 against FP64: L1(rho) 1.2e-5 at
+// 1e-6, 4.9e-7 at 1e-4)
+template <>
+struct HlldTol<double> {
+    static constexpr double tol = 1e-11, bdeg = 1e-4, dp = 1e-4;
+};
+template <>
+struct HlldTol<float> {
+    static constexpr float tol = 1e-6f, bdeg = 1e-4f, dp = 1e-3f;
+};
+constexpr int HLLD_ITER = 24;
+
+// residual of the contact condition at total pressure p (g above; with a
+// degenerate normal field the Alfven waves merge with the contact and the
+// residual is the normal velocity jump between the two fast-wave states);
+// also the contact field Bc (valid where K_x^L < K_x^R) and velocities
+template <class R>
+SPH_HD R contact_residual(const R RL[7], const R RR[7], R lL, R lR, R bx, R p, bool deg, AState<R>& aL,
+                          AState<R>& aR, R Bc[3], R vcL[3], R vcR[3]) {
+    const R sb = bx < R(0) ? R(-1) : R(1);
+    fast_state(RL, lL, bx, p, -sb, aL);
+    fast_state(RR, lR, bx, p, sb, aR);
+    if (deg) return aR.v[0] - aL.v[0];
+    const R dK = aR.K[0] - aL.K[0];
+    R N[3];
+    N[0] = R(0);
+#pragma unroll
+    for (int a = 1; a < 3; a++)
+        N[a] = (aR.B[a] * (aR.K[0] - aR.v[0]) + bx * aR.v[a]) - (aL.B[a] * (aL.K[0] - aL.v[0]) + bx * aL.v[a]);
+    const R YL = contact_side(aL, dK, bx, N, vcL);
+    const R YR = contact_side(aR, dK, bx, N, vcR);
+    const R idK = R(1) / dK;
+    Bc[0] = bx;
+    Bc[1] = N[1] * idK;
+    Bc[2] = N[2] * idK;
+    return R(1) - bx * (YR - YL);
+}
+
+// Total pressure p + b^2/2 of the HLL state U = (RR - RL) / (lR - lL) in the
+// frame: a Newton solve of the con2prim residual from the initial Z of c2p;
+// returns -1 where it does not converge (the caller keeps its estimate).
+template <class R>
+SPH_HD R hll_total_pressure(const R RL[7], const R RR[7], R lL, R lR, R bx, R gam) {
+    const R idl = R(1) / (lR - lL);
+    R U[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) U[q] = (RR[q] - RL[q]) * idl;
+    C2PInT<R> q;
+    q.D = U[0];
+    q.E = U[4];
+    q.S2 = U[1] * U[1] + U[2] * U[2] + U[3] * U[3];
+    q.SB = U[1] * bx + U[2] * U[5] + U[3] * U[6];
+    q.B2 = bx * bx + U[5] * U[5] + U[6] * U[6];
+    if (!(q.D > R(0)) || !(q.E > R(0))) return R(-1);
+    R Z = q.D + gam * rmax(q.E - R(0.5) * q.B2 - q.D, R(0));
+    for (int it = 0; it < 30; it++) {
+        R df;
+        const R f = c2p_f(q, gam, Z, df);
+        const R Zn = Z - f / df;
+        if (!(Zn > R(0)) || !isfinite(Zn)) return R(-1);
+        const R dz = Zn - Z;
+        Z = Zn;
+        if (rabs(dz) <= HlldTol<R>::tol * Z) {
+            const R v2 = c2p_v2(q, Z);
+            if (!(v2 < R(1))) return R(-1);
+            const R W2i = R(1) - v2;
+            const R p = (gam - R(1)) / gam * (Z * W2i - q.D * sqrt(W2i));
+            const R vB = q.SB / Z;
+            return rmax(p, R(0)) + R(0.5) * (q.B2 * W2i + vB * vB);
+        }
+    }
+    return R(-1);
+}
+
+// The Riemann fan of a solved face, for checking the intermediate states:
+// speeds lL <= laL <= lc <= laR <= lR and the conserved words (D, m, E,
+// B_y, B_z) of the regions aL, cL, cR, aR, at total pressure p (deg: the
+// Alfven waves merged with the contact, no c regions).
+template <class R>
+struct HlldFan {
+    R lL, laL, lc, laR, lR, p;
+    R UaL[7], UcL[7], UcR[7], UaR[7];
+    int deg, hll;
+    int why;  // 0 solved; fallback: 1 no convergence, 2 fast-wave states, 3 fan ordering, 4 non-finite flux
+};
+
+// Frame HLLD flux through a surface moving with normal speed vs: out[8] =
+// D, m_x, m_y, m_z, E, B_y, B_z, B_x words of F - vs U of the Riemann fan at
+// x/t = vs.  Returns 0 for an HLLD flux, 1 where the solver fell back to HLL
+// (no converged total pressure, or intermediate states outside the fan's
+// wave ordering or non-finite: MUB09 sec. 3.4).  hll_only takes the HLL
+// branch directly; fan, if given, receives the fan (both for checking).
+// the HLLD part of hlld_frame for a surface inside the fan (lL < vs < lR):
+// true with out[0..6] set, false where the solver falls back to HLL.  RL,
+// RR = lambda U - F of the outer states; pmk = the reference's HLL estimate
+// of the total pressure (hlld.cpp:263), pmin half the smaller outer total
+// pressure.
+template <class R>
+SPH_HD bool hlld_solve(const R RL[7], const R RR[7], R pmk, R pmin, R bx, R vs, R gam, R lL, R lR, R out[8],
+                       HlldFan<R>* fan) {
+    const R idl = R(1) / (lR - lL);
+    // initial total pressure of the secant iteration: the total pressure of
+    // the HLL state (MUB09 sec. 3.4; exact for an isolated contact, and on
+    // the admissible branch of the contact condition, which has spurious
+    // roots), else the reference's estimate
+    R p0 = pmk;
+    if (!(p0 > R(0)) || !isfinite(p0)) p0 = pmin;
+    const bool deg = bx * bx < HlldTol<R>::bdeg * p0;
+    if (deg) {
+        // the contact of the two-wave problem on the HLL state (Mignone &
+        // Bodo 2006): with B_x = 0 the normal velocities behind the fast
+        // waves are (p + R_mx) / (lambda p + R_E) and their equality is a
+        // quadratic in p, here through the contact speed's root (the
+        // solution at B_x = 0, the start of the iteration below otherwise);
+        // no positive root (a vacuum between the waves) takes the fallback
+        const R UE = (RR[4] - RL[4]) * idl, Um = (RR[1] - RL[1]) * idl;
+        const R FE = (lL * RR[4] - lR * RL[4]) * idl, Fm = (lL * RR[1] - lR * RL[1]) * idl;
+        const R b = UE + Fm;
+        const R disc = rmax(b * b - R(4) * FE * Um, R(0));
+        const R ls = R(2) * Um / (b + sqrt(disc));
+        p0 = Fm - ls * FE;
+        if (!(p0 > R(0)) || !isfinite(p0)) {
+            if (fan) fan->why = 1;
+            return false;
+        }
+    } else {
+        const R ph = hll_total_pressure(RL, RR, lL, lR, bx, gam);
+        if (ph > R(0) && isfinite(ph)) p0 = ph;
+    }
+    AState<R> aL, aR;
+    R Bc[3], vcL[3], vcR[3];
+    R pa = p0;
+    R fa = contact_residual(RL, RR, lL, lR, bx, pa, deg, aL, aR, Bc, vcL, vcR);
+    R pb = p0, fb = fa;
+    // with no normal field the two-wave contact is the root itself
+    bool ok = fa == R(0) || (deg && bx == R(0));
+    if (!ok) {
+        pb = p0 * (R(1) + HlldTol<R>::dp);
+        fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
+    }
+    for (int it = 0; it < HLLD_ITER && !ok; it++) {
+        if (!isfinite(fa) || !isfinite(fb)) break;
+        if (fb == fa) {
+            // no change of the residual: converged to rounding, unless the
+            // step was still large
+            ok = fb == R(0) || rabs(pb - pa) <= R(10) * HlldTol<R>::tol * pb;
+            break;
+        }
+        R pn = pb - fb * (pb - pa) / (fb - fa);
+        if (!(pn > R(0))) pn = R(0.5) * pb;
+        pa = pb;
+        fa = fb;
+        pb = pn;
+        fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
+        if (rabs(pb - pa) <= HlldTol<R>::tol * pb) ok = true;
+    }
+    // admissibility at the converged pressure (aL, aR, Bc, vcL, vcR are
+    // those of pb): positive total enthalpy, sub-luminal velocities and the
+    // wave ordering of the fan
+    auto sub = [](const R v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < R(1); };
+    bool good = ok && isfinite(fb) && aL.w > R(0) && aR.w > R(0) && sub(aL.v) && sub(aR.v);
+    if (fan) fan->why = !(ok && isfinite(fb)) ? 1 : !good ? 2 : 0;
+    const R p = pb;
+    R lc, laL = lL, laR = lR;
+    if (deg) {
+        lc = R(0.5) * (aL.v[0] + aR.v[0]);
+        good = good && lL <= lc && lc <= lR;
+    } else {
+        laL = aL.K[0];
+        laR = aR.K[0];
+        lc = R(0.5) * (vcL[0] + vcR[0]);
+        // a fan whose Alfven waves coincide is not one
+        good = good && lL <= laL && laL < laR && laL <= lc && lc <= laR && laR <= lR && sub(vcL) && sub(vcR);
+    }
+    if (fan) {
+        if (fan->why == 0 && !good) fan->why = 3;
+        fan->laL = laL; fan->lc = lc; fan->laR = laR; fan->p = p; fan->deg = deg ? 1 : 0;
+        for (int side = 0; side < 2; side++) {
+            const AState<R>& a = side ? aR : aL;
+            const R* Rv = side ? RR : RL;
+            const R lf = side ? lR : lL, la = side ? laR : laL;
+            R* Ua = side ? fan->UaR : fan->UaL;
+            R* Uc = side ? fan->UcR : fan->UcL;
+            const R ia = R(1) / (lf - a.v[0]);
+            fan_cons(a.v, a.B, a.vB, p, Rv[0] * ia, (Rv[4] + p * a.v[0] - a.vB * bx) * ia, Ua);
+            for (int q = 0; q < 7; q++) Uc[q] = Ua[q];
+            if (!deg) {
+                const R* vc = side ? vcR : vcL;
+                const R vBc = vc[0] * Bc[0] + vc[1] * Bc[1] + vc[2] * Bc[2];
+                const R ic = R(1) / (la - vc[0]);
+                fan_cons(vc, Bc, vBc, p, Ua[0] * (la - a.v[0]) * ic, (la * Ua[4] - Ua[1] + p * vc[0] - vBc * bx) * ic,
+                         Uc);
+            }
+        }
+    }
+    if (!good) return false;
+    // the region of the fan that contains x/t = vs: the fast-wave state of
+    // its side (F = lambda U - R), then across the Alfven wave if inside it
+    const bool left = vs <= lc;
+    const AState<R> a = left ? aL : aR;
+    R Rv[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) Rv[q] = left ? RL[q] : RR[q];
+    const R lf = left ? lL : lR;
+    const R la = left ? laL : laR;
+    R Ux[7], Fx[7];
+    {
+        const R ia = R(1) / (lf - a.v[0]);
+        fan_cons(a.v, a.B, a.vB, p, Rv[0] * ia, (Rv[4] + p * a.v[0] - a.vB * bx) * ia, Ux);
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) Fx[q] = lf * Ux[q] - Rv[q];
+    const bool inner = !deg && (left ? vs > laL : vs <= laR);
+    if (inner) {
+        R vc[3];
+#pragma unroll
+        for (int q = 0; q < 3; q++) vc[q] = left ? vcL[q] : vcR[q];
+        const R vBc = vc[0] * Bc[0] + vc[1] * Bc[1] + vc[2] * Bc[2];
+        const R ic = R(1) / (la - vc[0]);
+        const R Dc = Ux[0] * (la - a.v[0]) * ic;
+        const R Ec = (la * Ux[4] - Ux[1] + p * vc[0] - vBc * bx) * ic;
+        R Uc[7];
+        fan_cons(vc, Bc, vBc, p, Dc, Ec, Uc);
+#pragma unroll
+        for (int q = 0; q < 7; q++) {
+            Fx[q] += la * (Uc[q] - Ux[q]);
+            Ux[q] = Uc[q];
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        out[q] = Fx[q] - vs * Ux[q];
+        good = good && isfinite(out[q]);
+    }
+    if (fan) {
+        fan->hll = good ? 0 : 1;
+        if (!good) fan->why = 4;
+    }
+    return good;
+}
+
+// Frame flux through a surface moving with normal speed vs: out[8] = D,
+// m_x, m_y, m_z, E, B_y, B_z, B_x words of F - vs U of the Riemann fan at
+// x/t = vs.  Returns 0 for the upwind or HLLD flux, 1 where the HLLD solver
+// fell back to HLL (no converged total pressure, or intermediate states
+// outside the fan's wave ordering or non-finite: MUB09 sec. 3.4).  hll_only
+// takes the HLL flux directly; fan, if given, receives the fan (both for
+// checking).
+template <class R>
+SPH_HD int hlld_frame(const FSide<R>& L, const FSide<R>& Rr, R bx, R vs, R gam, R out[8], bool hll_only = false,
+                      HlldFan<R>* fan = nullptr) {
+    const R lL = rmin(L.lm, Rr.lm), lR = rmax(L.lp, Rr.lp);
+    out[7] = -vs * bx;
+    if (fan) {
+        fan->lL = lL;
+        fan->lR = lR;
+        fan->hll = 1;
+    }
+    // the surface outside the fan: the upwind state's flux (HLL's too)
+    if (!(vs > lL)) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) out[q] = L.f[q] - vs * L.u[q];
+        if (fan) fan->hll = 0;
+        return hll_only ? 1 : 0;
+    }
+    if (!(vs < lR)) {
+#pragma unroll
+        for (int q = 0; q < 7; q++) out[q] = Rr.f[q] - vs * Rr.u[q];
+        if (fan) fan->hll = 0;
+        return hll_only ? 1 : 0;
+    }
+    // from here on the outer states enter only through R = lambda U - F
+    R RL[7], RR[7];
+#pragma unroll
+    for (int q = 0; q < 7; q++) {
+        RL[q] = lL * L.u[q] - L.f[q];
+        RR[q] = lR * Rr.u[q] - Rr.f[q];
+    }
+    const R idl = R(1) / (lR - lL);
+    if (!hll_only) {
+        const R pmk = (lR * L.pt - lL * Rr.pt + lL * lR * (Rr.u[1] - L.u[1])) * idl;
+        if (hlld_solve(RL, RR, pmk, R(0.5) * rmin(L.pt, Rr.pt), bx, vs, gam, lL, lR, out, fan)) return 0;
+    }
+    // HLL: F_hll - vs U_hll = ((lL - vs) RR - (lR - vs) RL) / (lR - lL)
+#pragma unroll
+    for (int q = 0; q < 7; q++) out[q] = ((lL - vs) * RR[q] - (lR - vs) * RL[q]) * idl;
+    return 1;
+}
+
+// map a frame flux (hlld_frame's out) to the coordinate flux words of hll
+// (D, S_1..3, tau, the two transverse B^k), densitised; bn = the coordinate
+// normal field B^n
+template <class R>
+SPH_HD void frame_to_coord(const Met<R>& m, const Frame<R>& f, int n, const R F[8], R bn, R out[NFW]) {
+    const R c = f.c;
+    out[0] = c * F[0];
+    out[4] = c * (F[4] - F[0]);
+    const R ext = m.sg * m.beta[0] * bn;  // sqrt(gamma) beta^r B^n in the B^r flux (n != r)
+    if (n == 0) {
+        out[1] = c * (F[1] * f.irn + f.g13 * f.is2 * F[3]);
+        out[2] = c * (f.s1 * F[2]);
+        out[3] = c * (f.s2 * F[3]);
+        out[5] = c * (F[5] * f.is1);                          // B^theta
+        out[6] = c * (F[6] * f.is2 + f.gi13 * f.irn * F[7]);  // B^phi
+    } else if (n == 1) {
+        out[1] = c * (f.g13 * f.is1 * F[2] + F[3] * f.is2);
+        out[2] = c * (F[1] * f.irn);
+        out[3] = c * (f.s1 * F[2]);
+        out[5] = c * (F[5] * f.is1 + f.gi13 * f.is2 * F[6]);  // B^phi
+        out[6] = c * (f.s2 * F[6]) + ext;                     // B^r
+    } else {
+        out[1] = c * (f.s1 * F[2]);
+        out[2] = c * (f.s2 * F[3]);
+        out[3] = c * (F[1] * f.irn + f.g13 * f.is1 * F[2]);
+        out[5] = c * (F[5] * f.is1 + f.gi13 * f.irn * F[7]) + ext;  // B^r
+        out[6] = c * (F[6] * f.is2);                                // B^theta
+    }
+}
+
+// HLLD flux at a face (the coordinate words of hll); gi13 = gamma^r phi at
+// the face.  Axis faces (sqrt(gamma) = 0) carry no flux.  Returns 1 where
+// the HLL fallback was taken.
+template <class R>
+SPH_HD int hlld(const Met<R>& m, R gi13, R gam, Pt<R>& PL, Pt<R>& PR, int n, R out[NFW], bool hll_only = false,
+                HlldFan<R>* fan = nullptr) {
+    if (!(m.sg > R(0))) {
+#pragma unroll
+        for (int q = 0; q < NFW; q++) out[q] = R(0);
+        return 0;
+    }
+    const Frame<R> fr = make_frame(m, gi13, n);
+    const R bn = PL.B[n];
+    const R bx = bn * fr.irn;
+    FSide<R> L, Rr;
+    frame_side(m, fr, gam, PL, n, bx, L);
+    frame_side(m, fr, gam, PR, n, bx, Rr);
+    R F[8];
+    const int fb = hlld_frame(L, Rr, bx, fr.vs, gam, F, hll_only, fan);
+    frame_to_coord(m, fr, n, F, bn, out);
+    return fb;
+}
+
 // Limited linear (minmod) reconstruction at the face between cells c-1 and
 // c from the four cells c-2 .. c+1 along the direction (recon.hpp:16-19,
 // reconstruction.cpp:28-38): left state from cell c-1, right from cell c.
@@ -254,28 +812,6 @@ SPH_HD void ptoc(const Met<double>& m, double gam, const double w[8], double U[8
 //      floor state.
 constexpr int C2P_NEWTON = 60;
 constexpr int C2P_BISECT = 200;
-
-struct C2PIn {
-    double D, E, S2, SB, B2;
-};
-SPH_HD double c2p_v2(const C2PIn& q, double Z) {
-    const double ZB = Z + q.B2;
-    return (Z * Z * q.S2 + q.SB * q.SB * (2.0 * Z + q.B2)) / (Z * Z * ZB * ZB);
-}
-SPH_HD double c2p_f(const C2PIn& q, double gam, double Z, double& df) {
-    const double ZB = Z + q.B2;
-    const double v2 = c2p_v2(q, Z);
-    const double omv = 1.0 - v2;
-    const double sq = sqrt(omv);
-    const double gg = (gam - 1.0) / gam;
-    const double p = gg * (Z * omv - q.D * sq);
-    const double Z3 = Z * Z * Z;
-    const double dv2 = -2.0 * (q.S2 / (ZB * ZB * ZB) + q.SB * q.SB * (3.0 * Z * Z + 3.0 * q.B2 * Z + q.B2 * q.B2) /
-                                                          (Z3 * ZB * ZB * ZB));
-    const double dp = gg * (omv - Z * dv2 + q.D * dv2 / (2.0 * sq));
-    df = 1.0 - dp + 0.5 * q.B2 * dv2 + q.SB * q.SB / Z3;
-    return Z + q.B2 - p - 0.5 * (q.B2 * omv + q.SB * q.SB / (Z * Z)) - q.E;
-}
 
 SPH_HD int c2p(const Met<double>& m, double gam, const double U[8], double w[8]) {
     const double is = 1.0 / m.sg;

@@ -41,11 +41,19 @@ typedef struct bhs_engine bhs_engine;
 /* Spacetime, gas and floors of one run: the reference's compile-time
  * parameters (src/fluid/main.cpp:61-81) and floor constants
  * (src/fluid/grid.hpp:140-143): rho_floor * r^-3/2, eps_floor * r^-3/2, plus
- * a magnetisation ceiling b^2/rho <= sigma_max and a Lorentz-factor ceiling. */
+ * a magnetisation ceiling b^2/rho <= sigma_max and a Lorentz-factor ceiling;
+ * the Riemann solver (src/fluid/solver/hlld.cpp). */
 typedef struct {
     double a, Q, gamma, cfl;
     double rho_floor, eps_floor, sigma_max, gamma_max;
+    /* Riemann solver: BHS_HLLD (the reference's five-wave structure,
+     * hlld.cpp:230-312, with relativistic intermediate states in the face's
+     * orthonormal frame) or BHS_HLL (its two-wave fallback). */
+    int riemann;
+    int reserved;
 } bhs_physics;
+#define BHS_HLL 0
+#define BHS_HLLD 1
 
 /* Level-0 grid: nr x nth x nph zones over r in [r_in, r_out] (logarithmic),
  * theta in [0, pi], phi in [0, 2 pi) (nph ignored by an axisymmetric engine);

@@ -58,11 +58,14 @@ struct Level {
 struct Run {
     Lay L;
     double a, Q, gam, cfl, rfl, efl, smax, wmax;
+    int riemann;  // 0 HLL, 1 HLLD
     std::vector<Level> lev;
     std::vector<int> bl, bi, bj, bk;  // block level and coordinates
     std::vector<double> U, W, Us, Bf0, Bf1;
     std::vector<double> F;   // [nb][3][7][nf] in FP64 or (mixed) values rounded through float
     long long c2p_fail = 0, floors = 0;
+    long long riemann_faces = 0, hll_fallbacks = 0;  // HLLD faces solved and those that fell back to HLL
+    long long fallback_why[4] = {0, 0, 0, 0};         // by reason (HlldFan::why 1..4)
     int nb() const { return (int)bl.size(); }
     double* Ub(int b) { return U.data() + (size_t)b * 8 * L.ncell; }
     double* Wb(int b) { return W.data() + (size_t)b * 8 * L.ncell; }
@@ -354,11 +357,25 @@ void fluxes(Run& R, const std::vector<double>& Bf) {
                         PL.B[d] = bn;
                         PR.B[d] = bn;
                         const Rt gam = (Rt)R.gam;
-                        Side<Rt> SL, SR;
-                        face_side(m, gam, PL, d, SL);
-                        face_side(m, gam, PR, d, SR);
                         Rt out[NFW];
-                        hll(m, SL, SR, out);
+                        if (R.riemann) {
+                            Met<Rt> mf = m;
+                            HlldFan<Rt> fan{};
+                            const int fb = hlld(mf, mf.gi[2], gam, PL, PR, d, out, false, &fan);
+#pragma omp atomic
+                            R.hll_fallbacks += fb;
+                            if (fb && fan.why >= 1 && fan.why <= 4) {
+#pragma omp atomic
+                                R.fallback_why[fan.why - 1] += 1;
+                            }
+#pragma omp atomic
+                            R.riemann_faces += 1;
+                        } else {
+                            Side<Rt> SL, SR;
+                            face_side(m, gam, PL, d, SL);
+                            face_side(m, gam, PR, d, SR);
+                            hll(m, SL, SR, out);
+                        }
                         for (int q = 0; q < NFW; q++)
                             R.F[(((size_t)b * 3 + d) * NFW + q) * L.nf + fidx(d, i, j, k)] = (double)out[q];
                     }
@@ -543,7 +560,8 @@ double step(Run& R) {
 
 extern "C" {
 
-// One run.  phys[8] = a, Q, gamma, cfl, rho_fl, eps_fl, sigma_max, w_max;
+// One run.  phys[9] = a, Q, gamma, cfl, rho_fl, eps_fl, sigma_max, w_max,
+// Riemann solver (0 HLL, 1 HLLD);
 // grid: level-0 zones nr0 x nth0 x nph0 over r in [r_in, r_out] (log),
 // theta in [0, pi], phi in [0, 2 pi); blocks[nb][4] = level, bi, bj, bk.
 void* os_run_new(int bz, const double* phys, int nr0, int nth0, int nph0, double r_in, double r_out, int nlev,
@@ -552,6 +570,7 @@ void* os_run_new(int bz, const double* phys, int nr0, int nth0, int nph0, double
     R->L = make_lay(bz);
     R->a = phys[0]; R->Q = phys[1]; R->gam = phys[2]; R->cfl = phys[3];
     R->rfl = phys[4]; R->efl = phys[5]; R->smax = phys[6]; R->wmax = phys[7];
+    R->riemann = (int)phys[8];
     long long z0 = 0;
     for (int l = 0; l < nlev; l++) {
         Level v;
@@ -817,6 +836,9 @@ void os_counters(void* r, long long* out) {
     Run* R = (Run*)r;
     out[0] = R->c2p_fail;
     out[1] = R->floors;
+    out[2] = R->riemann_faces;
+    out[3] = R->hll_fallbacks;
+    for (int q = 0; q < 4; q++) out[4 + q] = R->fallback_why[q];
 }
 
 // out[6] = total rest mass (sum of U.D dr dth dph), Killing energy
@@ -950,6 +972,33 @@ void os_hll_point(const double* met17, double gam, const double* wl, const doubl
     face_side(m, gam, L, n, SL);
     face_side(m, gam, Rr, n, SR);
     hll(m, SL, SR, out7);
+}
+// HLLD flux at a face between primitive states wl, wr (B^n taken from wl);
+// hll_only: the solver's HLL branch; fan (36 words, may be NULL): lL, laL,
+// lc, laR, lR, p, deg, fallback (0: HLLD; else 1 + the reason), then the conserved words (D, m, E, B_y, B_z in
+// the face frame) of the regions aL, cL, cR, aR.  Returns 1 for the HLL
+// fallback.
+int os_hlld_point(const double* met17, double gam, const double* wl, const double* wr, int n, int hll_only,
+                  double* out7, double* fan) {
+    Met<double> m = load_met<double>(met17, 1, 0);
+    Pt<double> L, Rr;
+    L.rho = wl[0]; L.p = (gam - 1.0) * wl[0] * wl[1];
+    Rr.rho = wr[0]; Rr.p = (gam - 1.0) * wr[0] * wr[1];
+    for (int d = 0; d < 3; d++) { L.u[d] = wl[2 + d]; L.B[d] = wl[5 + d]; Rr.u[d] = wr[2 + d]; Rr.B[d] = wr[5 + d]; }
+    Rr.B[n] = L.B[n];
+    HlldFan<double> F{};
+    const int rc = hlld(m, m.gi[2], gam, L, Rr, n, out7, hll_only != 0, fan ? &F : nullptr);
+    if (fan) {
+        const double h[8] = {F.lL, F.laL, F.lc, F.laR, F.lR, F.p, (double)F.deg, (double)(F.hll ? 1 + F.why : 0)};
+        for (int q = 0; q < 8; q++) fan[q] = h[q];
+        for (int q = 0; q < 7; q++) {
+            fan[8 + q] = F.UaL[q];
+            fan[15 + q] = F.UcL[q];
+            fan[22 + q] = F.UcR[q];
+            fan[29 + q] = F.UaR[q];
+        }
+    }
+    return rc;
 }
 // source terms at a cell: gc holds the NGC words of one point
 void os_source_point(const double* gc, double gam, const double* w, double* out5) {

@@ -41,6 +41,8 @@ def lib():
         L.os_c2p_point.argtypes = [_dp, ctypes.c_double, _dp, _dp]
         L.os_flux_point.argtypes = [_dp, ctypes.c_double, _dp, ctypes.c_int, _dp, _dp, _dp]
         L.os_hll_point.argtypes = [_dp, ctypes.c_double, _dp, _dp, ctypes.c_int, _dp]
+        L.os_hlld_point.restype = ctypes.c_int
+        L.os_hlld_point.argtypes = [_dp, ctypes.c_double, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]
         L.os_source_point.argtypes = [_dp, ctypes.c_double, _dp, _dp]
         L.os_metric_point.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         L.os_pcg64.restype = ctypes.c_double
@@ -81,8 +83,9 @@ class OracleRun:
         self.cfg = cfg
         self.L = lay(cfg["bz"])
         self.blocks = np.ascontiguousarray(blocks, dtype=np.int32)
+        # phys[8]: the Riemann solver, 0 HLL, 1 HLLD (the default)
         phys = np.array([cfg["a"], cfg.get("Q", 0.0), cfg["gamma"], cfg["cfl"], cfg["rho_fl"], cfg["eps_fl"],
-                         cfg["sigma_max"], cfg["w_max"]], dtype=np.float64)
+                         cfg["sigma_max"], cfg["w_max"], float(cfg.get("riemann", 1))], dtype=np.float64)
         self.nb = len(self.blocks)
         self._h = lib().os_run_new(cfg["bz"], P(phys), cfg["nr"], cfg["nth"], cfg["nph"], cfg["r_in"], cfg["r_out"],
                                    cfg["nlev"], self.nb, self.blocks.ctypes.data_as(_ip))
@@ -120,7 +123,10 @@ class OracleRun:
         return lib().os_c2p(self._h)
 
     def counters(self):
-        c = np.zeros(2, np.int64)
+        """con2prim failures, floored zones, HLLD faces solved, HLLD faces
+        that fell back to HLL, and those by reason: no convergence,
+        fast-wave states, fan ordering, non-finite flux."""
+        c = np.zeros(8, np.int64)
         lib().os_counters(self._h, c.ctypes.data_as(_llp))
         return c
 

@@ -23,7 +23,8 @@ def _cfg(grid, bz):
     p = sph.PHYSICS
     return dict(bz=bz, a=p["a"], Q=p["Q"], gamma=p["gamma"], cfl=p["cfl"], rho_fl=p["rho_floor"],
                 eps_fl=p["eps_floor"], sigma_max=p["sigma_max"], w_max=p["gamma_max"], nr=grid["nr"],
-                nth=grid["nth"], nph=grid["nph"], r_in=grid["r_in"], r_out=grid["r_out"], nlev=grid["nlevels"])
+                nth=grid["nth"], nph=grid["nph"], r_in=grid["r_in"], r_out=grid["r_out"], nlev=grid["nlevels"],
+                riemann=sph.riemann_code(p))
 
 
 def _tor():
@@ -46,13 +47,14 @@ def rel_linf(a, b):
     return worst
 
 
-def make_pair(grid, bz, blocks, mixed=False, persistent=False, max_blocks=None):
+def make_pair(grid, bz, blocks, mixed=False, persistent=False, max_blocks=None, physics=None):
+    physics = physics or sph.PHYSICS
     eng = sph.Engine(bz, max_blocks or len(blocks))
     eng.set_precision(mixed)
     eng.set_schedule(persistent)
-    run = eng.add_run(sph.PHYSICS, grid, blocks)
+    run = eng.add_run(physics, grid, blocks)
     info_g = run.init_torus(sph.TORUS, 42)
-    orc = OracleRun(_cfg(grid, bz), blocks)
+    orc = OracleRun(dict(_cfg(grid, bz), riemann=sph.riemann_code(physics)), blocks)
     info_o = orc.init_torus(_tor(), 42)
     return eng, run, orc, info_g, info_o
 
@@ -96,6 +98,13 @@ def test_c1_ten_steps_fp64():
     _steps(eng, run, orc, 10, False, "C1")
     d = run.diagnostics()
     assert d["divb_max"] <= 1e-13 * d["divb_scale"]
+
+
+def test_c1_ten_steps_fp64_hll():
+    """The two-wave solver (riemann = "hll"), C1, 10 steps."""
+    g = sph.c1_grid()
+    eng, run, orc, _, _ = make_pair(g, 1, sph.uniform_blocks(g, 1), physics=dict(sph.PHYSICS, riemann="hll"))
+    _steps(eng, run, orc, 10, False, "C1 HLL")
 
 
 def test_c1_ten_steps_mixed():
