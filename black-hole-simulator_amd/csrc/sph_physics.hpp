@@ -233,18 +233,22 @@ SPH_HD R c2p_v2(const C2PInT<R>& q, R Z) {
 }
 template <class R>
 SPH_HD R c2p_f(const C2PInT<R>& q, R gam, R Z, R& df) {
+    // 1/Z and 1/(Z + B^2) from one division
     const R ZB = Z + q.B2;
-    const R v2 = c2p_v2(q, Z);
+    const R r = R(1) / (Z * ZB);
+    const R iZ = ZB * r, iZB = Z * r;
+    const R iZ2 = iZ * iZ, iZB2 = iZB * iZB;
+    const R SB2 = q.SB * q.SB;
+    const R v2 = (q.S2 + SB2 * (R(2) * Z + q.B2) * iZ2) * iZB2;
     const R omv = R(1) - v2;
     const R sq = sqrt(omv);
     const R gg = (gam - R(1)) / gam;
     const R p = gg * (Z * omv - q.D * sq);
-    const R Z3 = Z * Z * Z;
-    const R dv2 = R(-2) * (q.S2 / (ZB * ZB * ZB) + q.SB * q.SB * (R(3) * Z * Z + R(3) * q.B2 * Z + q.B2 * q.B2) /
-                                                       (Z3 * ZB * ZB * ZB));
+    const R iZ3 = iZ2 * iZ, iZB3 = iZB2 * iZB;
+    const R dv2 = R(-2) * (q.S2 * iZB3 + SB2 * (R(3) * Z * Z + R(3) * q.B2 * Z + q.B2 * q.B2) * iZ3 * iZB3);
     const R dp = gg * (omv - Z * dv2 + q.D * dv2 / (R(2) * sq));
-    df = R(1) - dp + R(0.5) * q.B2 * dv2 + q.SB * q.SB / Z3;
-    return Z + q.B2 - p - R(0.5) * (q.B2 * omv + q.SB * q.SB / (Z * Z)) - q.E;
+    df = R(1) - dp + R(0.5) * q.B2 * dv2 + SB2 * iZ3;
+    return Z + q.B2 - p - R(0.5) * (q.B2 * omv + SB2 * iZ2) - q.E;
 }
 
 // ------------------------------------------------------------------- HLLD
@@ -426,7 +430,8 @@ SPH_HD R contact_side(const AState<R>& a, R dK, R bx, const R N[3], R vc[3]) {
 
 template <class R>
 struct HlldTol {};
-// Secant tolerance and first step per precision; bdeg: a normal field with
+// Secant tolerance on the pressure step (tol) and on the residual, which is
+// dimensionless (ftol), and its first step per precision; bdeg: a normal field with
 // B_x^2 < bdeg p is degenerate (Alfven waves merged with the contact).  The
 // threshold is the same in both precisions, so that FP32 and FP64 solve the
 // same fan, and large enough that the five-wave states, whose contact field
@@ -435,11 +440,11 @@ struct HlldTol {};
 // 1e-6, 4.9e-7 at 1e-4)
 template <>
 struct HlldTol<double> {
-    static constexpr double tol = 1e-11, bdeg = 1e-4, dp = 1e-4;
+    static constexpr double tol = 1e-11, ftol = 1e-13, bdeg = 1e-4, dp = 1e-4;
 };
 template <>
 struct HlldTol<float> {
-    static constexpr float tol = 1e-6f, bdeg = 1e-4f, dp = 1e-3f;
+    static constexpr float tol = 1e-6f, ftol = 1e-6f, bdeg = 1e-4f, dp = 1e-3f;
 };
 constexpr int HLLD_ITER = 24;
 
@@ -566,10 +571,11 @@ SPH_HD bool hlld_solve(const R RL[7], const R RR[7], R pmk, R pmin, R bx, R vs, 
     R fa = contact_residual(RL, RR, lL, lR, bx, pa, deg, aL, aR, Bc, vcL, vcR);
     R pb = p0, fb = fa;
     // with no normal field the two-wave contact is the root itself
-    bool ok = fa == R(0) || (deg && bx == R(0));
+    bool ok = rabs(fa) <= HlldTol<R>::ftol || (deg && bx == R(0));
     if (!ok) {
         pb = p0 * (R(1) + HlldTol<R>::dp);
         fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
+        ok = rabs(fb) <= HlldTol<R>::ftol;
     }
     for (int it = 0; it < HLLD_ITER && !ok; it++) {
         if (!isfinite(fa) || !isfinite(fb)) break;
@@ -585,7 +591,7 @@ SPH_HD bool hlld_solve(const R RL[7], const R RR[7], R pmk, R pmin, R bx, R vs, 
         fa = fb;
         pb = pn;
         fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
-        if (rabs(pb - pa) <= HlldTol<R>::tol * pb) ok = true;
+        if (rabs(pb - pa) <= HlldTol<R>::tol * pb || rabs(fb) <= HlldTol<R>::ftol) ok = true;
     }
     // admissibility at the converged pressure (aL, aR, Bc, vcL, vcR are
     // those of pb): positive total enthalpy, sub-luminal velocities and the
