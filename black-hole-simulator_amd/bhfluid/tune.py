@@ -5,7 +5,8 @@ variant is bit-identical), so the tuner only times.  It searches, one
 dimension at a time, the schedule (one kernel per sub-phase; one persistent
 queue-driven kernel per RK stage with 1 or 2 workgroups per compute unit; or
 the sub-phase kernels with fluxes and hydro update fused per block, "slab")
-and, for the sub-phase schedules, the XCD numbering and the workgroup size
+(or the sub-phase kernels with stage 1's con2prim fused into stage 0's
+update) and, for the sub-phase schedules, the XCD numbering and the workgroup size
 and occupancy hint of the flux, update and con2prim kernels; each candidate
 is timed over a few graph-replayed steps of the caller's own runs.  The choice
 is cached per (device, block shape, block count, precision) in a JSON file
@@ -70,7 +71,7 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
     best = dict(DEFAULT)
     best_ms = _time(eng, runs, best, steps)
     timings.append((dict(best), best_ms))
-    for c in [dict(best, schedule=1, wg_per_cu=wg) for wg in (1, 2)] + [dict(best, schedule=2)]:
+    for c in [dict(best, schedule=1, wg_per_cu=wg) for wg in (1, 2)] + [dict(best, schedule=s) for s in (2, 3)]:
         t = _time(eng, runs, c, steps)
         timings.append((c, t))
         if t < best_ms:
@@ -83,9 +84,9 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
             best, best_ms = c, t
         # the slab schedule has no separate update kernel and computes only
         # the side faces with k_flux (one face per thread)
-        dims = ("flux", "update", "c2p") if best["schedule"] == 0 else ("flux", "c2p")
+        dims = ("flux", "c2p") if best["schedule"] == 2 else ("flux", "update", "c2p")
         for dim in dims:
-            top = 12 if (dim == "flux" and best["schedule"] == 0) else 6
+            top = 12 if (dim == "flux" and best["schedule"] != 2) else 6
             for v in range(1, top):
                 c = dict(best, **{dim: v})
                 t = _time(eng, runs, c, steps)

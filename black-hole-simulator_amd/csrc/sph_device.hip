@@ -929,6 +929,25 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     cell_update<Rt>(P, b, a, run_dt(P, b), stage);
 }
 
+// Stage 0's update with stage 1's con2prim of the same zone fused behind it
+// (schedule 3): con2prim is a function of the zone's new U alone, so doing it
+// here instead of in a separate pass gives the same W, U and counters, and
+// saves a pass over U and the launch.
+template <class Rt, int TB, int MW>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_update_c2p(KPS P) {
+    const int per = (P.nint + TB - 1) / TB;
+    const int w = wg_index(P);
+    const int b = P.list[w / per];
+    const int a = (w % per) * TB + threadIdx.x;
+    int nfail = 0, nfl = 0;
+    if (a < P.nint) {
+        cell_update<Rt>(P, b, a, run_dt(P, b), 0);
+        c2p_zone(P, b, a, false, nfail, nfl);
+    }
+    // every thread of the workgroup reaches the commit's barriers
+    wg_commit<TB>(P, P.rl[P.blk[b].rl].run, 1e300, nfail, nfl, false);
+}
+
 // ---------------------------------------------------- slab schedule
 // Fluxes and the hydro update fused per block: one workgroup walks the
 // block's phi layers k = 0 .. bz-1; for layer k it forms the fluxes of the
@@ -1454,6 +1473,15 @@ void launch_update(int v, int nl, int nitems, hipStream_t st, KPS P, int stage) 
 #undef L_
     }
 }
+template <class Rt>
+void launch_update_c2p(int v, int nl, int nitems, hipStream_t st, KPS P) {
+    switch (v) {
+#define L_(i, tb, mw) \
+    case i: hipLaunchKernelGGL((k_update_c2p<Rt, tb, mw>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P); break;
+        SPH_VARIANTS(L_)
+#undef L_
+    }
+}
 inline void launch_c2p(int v, int nl, hipStream_t st, KPS P, int want_dt) {
     switch (v) {
 #define L_(i, tb, mw) \
@@ -1532,7 +1560,8 @@ public:
     std::vector<int> act_runs;    // runs of the current active list
     std::vector<int> act_slots;
     bool mixed = false;
-    int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update)
+    int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update), 3 multi-kernel with
+                                  // stage 1's con2prim fused into stage 0's update
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
     int var_flux = 0, var_update = 0, var_c2p = 1;  // launch variants (VAR_TB, VAR_MW)
     bool xcd_remap = true;        // XCD-contiguous workgroup numbering
@@ -1996,7 +2025,9 @@ public:
                 tm("k_stage", [&] { hipLaunchKernelGGL(k_stage<Rt>, dim3(ng), dim3(TPB), 0, stream, P, stage); });
                 continue;
             }
-            tm("k_c2p", [&] { launch_c2p(var_c2p, nl, stream, P, stage == 0 ? 1 : 0); });
+            // schedule 3: stage 1's con2prim ran inside stage 0's update
+            if (!(mode == 3 && stage == 1))
+                tm("k_c2p", [&] { launch_c2p(var_c2p, nl, stream, P, stage == 0 ? 1 : 0); });
             const int buf = stage;
             tm("k_ghost_cells", [&] {
                 hipLaunchKernelGGL(k_ghost_cells, dim3(nl * ((nghc + TPB - 1) / TPB)), dim3(TPB), 0, stream, P);
@@ -2025,7 +2056,10 @@ public:
             tm("k_face", [&] {
                 hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
             });
-            tm("k_update", [&] { launch_update<Rt>(var_update, nl, L.nint, stream, P, stage); });
+            if (mode == 3 && stage == 0)
+                tm("k_update_c2p", [&] { launch_update_c2p<Rt>(var_update, nl, L.nint, stream, P); });
+            else
+                tm("k_update", [&] { launch_update<Rt>(var_update, nl, L.nint, stream, P, stage); });
         }
         SCHK(hipGetLastError());
     }
@@ -2350,7 +2384,7 @@ int bhs_set_precision(bhs_engine* h, int mixed) {
 int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
     return guard(h, [&] {
         if (wg_per_cu < 0 || wg_per_cu > 32) throw std::invalid_argument("wg_per_cu out of range");
-        if (persistent < 0 || persistent > 2) throw std::invalid_argument("schedule must be 0, 1 or 2");
+        if (persistent < 0 || persistent > 3) throw std::invalid_argument("schedule must be 0, 1, 2 or 3");
         h->e->mode = persistent;
         h->e->wg_per_cu = wg_per_cu;
         return BHS_OK;

@@ -381,22 +381,47 @@ SPH_HD void fast_state(const R Rv[7], R lam, R bx, R p, R sgn, AState<R>& a) {
     const R C = RBy * Rmy + RBz * Rmz;
     const R Q = -A - G + bx * bx * l2;
     const R X = bx * (A * lam * bx + C) - (A + G) * (lam * p + RE);
-    const R iX = R(1) / X;
-    a.v[0] = (bx * (A * bx + lam * C) - (A + G) * (p + Rmx)) * iX;
+    // v = N / X; lambda - v_x = D2 / X
+    const R Nx = bx * (A * bx + lam * C) - (A + G) * (p + Rmx);
     const R cc = C + bx * (lam * Rmx - RE);
-    a.v[1] = (Q * Rmy + RBy * cc) * iX;
-    a.v[2] = (Q * Rmz + RBz * cc) * iX;
-    const R ilv = R(1) / (lam - a.v[0]);
+    const R Ny = Q * Rmy + RBy * cc, Nz = Q * Rmz + RBz * cc;
+    const R D2 = lam * X - Nx;
+    R iX, iD2;
+    if constexpr (sizeof(R) == 8) {
+        // both reciprocals from one division (FP64's range holds X D2)
+        const R t = R(1) / (X * D2);
+        iX = D2 * t;
+        iD2 = X * t;
+    } else {
+        iX = R(1) / X;
+        iD2 = R(1) / D2;
+    }
+    a.v[0] = Nx * iX;
+    a.v[1] = Ny * iX;
+    a.v[2] = Nz * iX;
+    // (R_B - B_x v) / (lambda - v_x) and p + (R_E - v.R_m) / (lambda - v_x)
     a.B[0] = bx;
-    a.B[1] = (RBy - bx * a.v[1]) * ilv;
-    a.B[2] = (RBz - bx * a.v[2]) * ilv;
+    a.B[1] = (RBy * X - bx * Ny) * iD2;
+    a.B[2] = (RBz * X - bx * Nz) * iD2;
     a.vB = a.v[0] * bx + a.v[1] * a.B[1] + a.v[2] * a.B[2];
-    a.w = p + (RE - (a.v[0] * Rmx + a.v[1] * Rmy + a.v[2] * Rmz)) * ilv;
+    a.w = p + (RE * X - (Nx * Rmx + Ny * Rmy + Nz * Rmz)) * iD2;
     a.eta = sgn * sqrt(rmax(a.w, R(0)));
     const R iden = R(1) / (lam * p + RE + a.eta * bx);
     a.K[0] = (Rmx + p + a.eta * lam * bx) * iden;
     a.K[1] = (Rmy + a.eta * RBy) * iden;
     a.K[2] = (Rmz + a.eta * RBz) * iden;
+}
+// the normal velocity behind a fast wave at total pressure p as N_x / X
+// (fast_state's first step)
+template <class R>
+SPH_HD void fast_vx(const R Rv[7], R lam, R bx, R p, R& Nx, R& X) {
+    const R Rmx = Rv[1], Rmy = Rv[2], Rmz = Rv[3], RE = Rv[4], RBy = Rv[5], RBz = Rv[6];
+    const R l2 = R(1) - lam * lam;
+    const R A = Rmx - lam * RE + p * l2;
+    const R G = RBy * RBy + RBz * RBz;
+    const R C = RBy * Rmy + RBz * Rmz;
+    X = bx * (A * lam * bx + C) - (A + G) * (lam * p + RE);
+    Nx = bx * (A * bx + lam * C) - (A + G) * (p + Rmx);
 }
 // conserved words (D, m, E, B_y, B_z) of a state from its velocity v, field
 // B, v.B, total pressure p, D and E
@@ -430,31 +455,34 @@ SPH_HD R contact_side(const AState<R>& a, R dK, R bx, const R N[3], R vc[3]) {
 
 template <class R>
 struct HlldTol {};
-// Secant tolerance on the pressure step (tol) and on the residual, which is
-// dimensionless (ftol), and its first step per precision; bdeg: a normal field with
-// B_x^2 < bdeg p is degenerate (Alfven waves merged with the contact).  The
-// threshold is the same in both precisions, so that FP32 and FP64 solve the
+// Secant tolerance on the pressure step (tol: the secant converges
+// superlinearly, so the pressure is then good to about tol^1.6) and on the
+// residual, which is dimensionless (ftol), and its first step per precision;
+// bdeg: a normal field with B_x^2 < bdeg p is degenerate (Alfven waves
+// merged with the contact).  The threshold is the same in both precisions, so that FP32 and FP64 solve the
 // same fan, and large enough that the five-wave states, whose contact field
 // divides by K_x^R - K_x^L ~ B_x, stay well conditioned in FP32 (on the
 // axisymmetric torus over 200 steps, FP32 against FP64: L1(rho) 1.2e-5 at
 // 1e-6, 4.9e-7 at 1e-4)
 template <>
 struct HlldTol<double> {
-    static constexpr double tol = 1e-11, ftol = 1e-13, bdeg = 1e-4, dp = 1e-4;
+    static constexpr double tol = 1e-8, ftol = 1e-13, bdeg = 1e-4, dp = 1e-4;
 };
 template <>
 struct HlldTol<float> {
-    static constexpr float tol = 1e-6f, ftol = 1e-6f, bdeg = 1e-4f, dp = 1e-3f;
+    static constexpr float tol = 1e-4f, ftol = 1e-6f, bdeg = 1e-4f, dp = 1e-3f;
 };
 constexpr int HLLD_ITER = 24;
 
 // residual of the contact condition at total pressure p (g above; with a
 // degenerate normal field the Alfven waves merge with the contact and the
 // residual is the normal velocity jump between the two fast-wave states);
-// also the contact field Bc (valid where K_x^L < K_x^R) and velocities
+// also the velocities next to the contact and Nc = (dK, N_y, N_z), from
+// which the contact field is B^c_t = N_t / dK (formed once, after the
+// iteration)
 template <class R>
 SPH_HD R contact_residual(const R RL[7], const R RR[7], R lL, R lR, R bx, R p, bool deg, AState<R>& aL,
-                          AState<R>& aR, R Bc[3], R vcL[3], R vcR[3]) {
+                          AState<R>& aR, R Nc[3], R vcL[3], R vcR[3]) {
     const R sb = bx < R(0) ? R(-1) : R(1);
     fast_state(RL, lL, bx, p, -sb, aL);
     fast_state(RR, lR, bx, p, sb, aR);
@@ -467,11 +495,41 @@ SPH_HD R contact_residual(const R RL[7], const R RR[7], R lL, R lR, R bx, R p, b
         N[a] = (aR.B[a] * (aR.K[0] - aR.v[0]) + bx * aR.v[a]) - (aL.B[a] * (aL.K[0] - aL.v[0]) + bx * aL.v[a]);
     const R YL = contact_side(aL, dK, bx, N, vcL);
     const R YR = contact_side(aR, dK, bx, N, vcR);
-    const R idK = R(1) / dK;
-    Bc[0] = bx;
-    Bc[1] = N[1] * idK;
-    Bc[2] = N[2] * idK;
+    Nc[0] = dK;
+    Nc[1] = N[1];
+    Nc[2] = N[2];
     return R(1) - bx * (YR - YL);
+}
+
+// The residual of contact_residual alone, with fewer divisions, for the
+// trial pressures of the iteration: degenerate, (N_x^R X^L - N_x^L X^R) /
+// (X^R X^L); else g with the two sides' Yh over one denominator.
+template <class R>
+SPH_HD R contact_g(const R RL[7], const R RR[7], R lL, R lR, R bx, R p, bool deg) {
+    const R sb = bx < R(0) ? R(-1) : R(1);
+    if (deg) {
+        R NL, XL, NR, XR;
+        fast_vx(RL, lL, bx, p, NL, XL);
+        fast_vx(RR, lR, bx, p, NR, XR);
+        if constexpr (sizeof(R) == 8) return (NR * XL - NL * XR) / (XR * XL);
+        else return NR / XR - NL / XL;
+    }
+    AState<R> aL, aR;
+    fast_state(RL, lL, bx, p, -sb, aL);
+    fast_state(RR, lR, bx, p, sb, aR);
+    const R dK = aR.K[0] - aL.K[0];
+    R N[3];
+#pragma unroll
+    for (int a = 1; a < 3; a++)
+        N[a] = (aR.B[a] * (aR.K[0] - aR.v[0]) + bx * aR.v[a]) - (aL.B[a] * (aL.K[0] - aL.v[0]) + bx * aL.v[a]);
+    auto nd = [&](const AState<R>& a, R& n, R& d) {
+        n = R(1) - (a.K[0] * a.K[0] + a.K[1] * a.K[1] + a.K[2] * a.K[2]);
+        d = dK * (a.eta - a.K[0] * bx) - (a.K[1] * N[1] + a.K[2] * N[2]);
+    };
+    R nL, dL, nR, dR;
+    nd(aL, nL, dL);
+    nd(aR, nR, dR);
+    return R(1) - bx * (nR * dL - nL * dR) / (dR * dL);
 }
 
 // Total pressure p + b^2/2 of the HLL state U = (RR - RL) / (lR - lL) in the
@@ -566,36 +624,47 @@ SPH_HD bool hlld_solve(const R RL[7], const R RR[7], R pmk, R pmin, R bx, R vs, 
         if (ph > R(0) && isfinite(ph)) p0 = ph;
     }
     AState<R> aL, aR;
-    R Bc[3], vcL[3], vcR[3];
+    R Nc[3], vcL[3], vcR[3];
     R pa = p0;
-    R fa = contact_residual(RL, RR, lL, lR, bx, pa, deg, aL, aR, Bc, vcL, vcR);
+    R fa = contact_residual(RL, RR, lL, lR, bx, pa, deg, aL, aR, Nc, vcL, vcR);
     R pb = p0, fb = fa;
     // with no normal field the two-wave contact is the root itself
     bool ok = rabs(fa) <= HlldTol<R>::ftol || (deg && bx == R(0));
     if (!ok) {
+        // the iteration needs only the residual: the states of each trial
+        // pressure are temporaries, and those of the last one are formed
+        // again after it (fewer registers live across the loop)
+        auto g = [&](R p) { return contact_g(RL, RR, lL, lR, bx, p, deg); };
         pb = p0 * (R(1) + HlldTol<R>::dp);
-        fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
+        fb = g(pb);
         ok = rabs(fb) <= HlldTol<R>::ftol;
-    }
-    for (int it = 0; it < HLLD_ITER && !ok; it++) {
-        if (!isfinite(fa) || !isfinite(fb)) break;
-        if (fb == fa) {
-            // no change of the residual: converged to rounding, unless the
-            // step was still large
-            ok = fb == R(0) || rabs(pb - pa) <= R(10) * HlldTol<R>::tol * pb;
-            break;
+        for (int it = 0; it < HLLD_ITER && !ok; it++) {
+            if (!isfinite(fa) || !isfinite(fb)) break;
+            if (fb == fa) {
+                // no change of the residual: converged to rounding, unless
+                // the step was still large
+                ok = fb == R(0) || rabs(pb - pa) <= R(10) * HlldTol<R>::tol * pb;
+                break;
+            }
+            R pn = pb - fb * (pb - pa) / (fb - fa);
+            if (!(pn > R(0))) pn = R(0.5) * pb;
+            pa = pb;
+            fa = fb;
+            pb = pn;
+            fb = g(pb);
+            if (rabs(pb - pa) <= HlldTol<R>::tol * pb || rabs(fb) <= HlldTol<R>::ftol) ok = true;
         }
-        R pn = pb - fb * (pb - pa) / (fb - fa);
-        if (!(pn > R(0))) pn = R(0.5) * pb;
-        pa = pb;
-        fa = fb;
-        pb = pn;
-        fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Bc, vcL, vcR);
-        if (rabs(pb - pa) <= HlldTol<R>::tol * pb || rabs(fb) <= HlldTol<R>::ftol) ok = true;
+        fb = contact_residual(RL, RR, lL, lR, bx, pb, deg, aL, aR, Nc, vcL, vcR);
     }
-    // admissibility at the converged pressure (aL, aR, Bc, vcL, vcR are
+    // admissibility at the converged pressure (aL, aR, Nc, vcL, vcR are
     // those of pb): positive total enthalpy, sub-luminal velocities and the
     // wave ordering of the fan
+    R Bc[3] = {bx, R(0), R(0)};
+    if (!deg) {
+        const R idK = R(1) / Nc[0];
+        Bc[1] = Nc[1] * idK;
+        Bc[2] = Nc[2] * idK;
+    }
     auto sub = [](const R v[3]) { return v[0] * v[0] + v[1] * v[1] + v[2] * v[2] < R(1); };
     bool good = ok && isfinite(fb) && aL.w > R(0) && aR.w > R(0) && sub(aL.v) && sub(aR.v);
     if (fan) fan->why = !(ok && isfinite(fb)) ? 1 : !good ? 2 : 0;

@@ -130,11 +130,13 @@ def test_amr_three_levels_fp64():
     _steps(eng, run, orc, 4, False, "AMR3")
 
 
-@pytest.mark.parametrize("schedule,mixed,bz", [(1, False, 16), (2, False, 16), (2, True, 16), (2, False, 1)])
+@pytest.mark.parametrize("schedule,mixed,bz", [(1, False, 16), (2, False, 16), (2, True, 16), (2, False, 1),
+                                               (3, False, 16), (3, True, 1)])
 def test_persistent_schedule_identical(schedule, mixed, bz):
-    """The persistent (1) and slab (2: fluxes and hydro update fused per
-    block) schedules give the sub-phase schedule's state bit for bit (3-level
-    tree; bz = 1: the axisymmetric C1 grid)."""
+    """The persistent (1), slab (2: fluxes and hydro update fused per block)
+    and fused-con2prim (3: stage 1's con2prim inside stage 0's update)
+    schedules give the sub-phase schedule's state and counters bit for bit
+    (3-level tree; bz = 1: the axisymmetric C1 grid)."""
     if bz == 16:
         g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
         blocks = sph.amr_blocks(g, 3)
@@ -156,6 +158,8 @@ def test_persistent_schedule_identical(schedule, mixed, bz):
         assert np.array_equal(da, db)
         for x, y in zip(ra.state(), rb.state()):
             assert np.array_equal(x, y), f"step {s}"
+    xa, xb = ra.diagnostics(), rb.diagnostics()
+    assert (xa["c2p_fail"], xa["floors"]) == (xb["c2p_fail"], xb["floors"])
 
 
 def test_graph_replays_identical_to_eager():
