@@ -211,9 +211,12 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--kernels-only", choices=("fp64", "mixed"),
+    ap.add_argument("--kernels-only", choices=("fp64", "mixed", "ref"),
                     help="for counter collection: C3 initial data, then --steps eager steps of the step's kernels "
-                         "in the given precision with the default launch configuration; prints one JSON line")
+                         "in the given precision with the default launch configuration; or (ref) the reference-path "
+                         "engine on the reference's own production tree (main.cpp:61-168: 8^3 root patches on "
+                         "[-50, 50]^3, FM torus a = 0.9, Q = 0.1, Gamma = 5/3, one regrid), initial data and "
+                         "regrid on the device, then --steps steps; prints one JSON line")
     ap.add_argument("--launch", default=None,
                     help="with --kernels-only: the launch configuration as JSON (as config.launch_fp64 or "
                          "config.launch_mixed of a bench line); default: the untuned one")
@@ -231,6 +234,21 @@ def main():
         from bhfluid import copy_probe
         ms = copy_probe(1 << 30, 3)
         print(json.dumps({"mode": "copy-calibration", "bytes": 1 << 30, "reps": 3, "ms_per_copy": ms}))
+        return
+    if args.kernels_only == "ref":
+        import bhfluid
+        t = bhfluid.AmrTree((-50.0,) * 3, (50.0,) * 3, 8, 1.0, 0.9, 0.1, 5.0 / 3.0)
+        bhfluid.torus_setup(t, regrid=True, device=True)
+        rk = bhfluid.RK2Integrator(0.4)
+        faults = 0
+        for _ in range(args.steps):
+            try:
+                rk.step(t)
+            except bhfluid.DomainError:
+                faults += 1   # the reference would stop here; the step itself completed
+        bhfluid.device_sync()
+        print(json.dumps({"mode": "kernels-only", "workload": "reference-path main.cpp torus", "steps": args.steps,
+                          "patches": int(t.nblocks), "steps_with_eos_faults": faults}))
         return
     if args.kernels_only:
         g, blocks = c3_setup(sph)

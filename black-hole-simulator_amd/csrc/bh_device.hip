@@ -846,9 +846,9 @@ __global__ __launch_bounds__(TPB) void k_logs(KP P) {
 // linear v and transverse B; reconstruction.cpp:40-97), the normal field
 // taken from the face, ptoc of both states at the face and the HLLD flux.
 // quirk Q1: the Riemann solver is called with dim 0 for every direction.
-// Four wavefronts per SIMD (128 VGPRs, a few spilled) measured 3 % faster
-// than the three the register count would otherwise allow.
-__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_flux(KP P) {
+// Three wavefronts per SIMD (158 VGPRs, nothing spilled: at four the
+// spilled registers went through the memory hierarchy with every face).
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_flux(KP P) {
     const int b = blockIdx.x / WG_FACE;
     const int f = (blockIdx.x % WG_FACE) * TPB + threadIdx.x;
     if (f >= 3 * NFC) return;
@@ -908,11 +908,9 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     double* Fo = P.F + ((size_t)b * 3 + dir) * 8 * NFC + lf;
     Fo[0] = Fl.D; Fo[NFC] = Fl.S[0]; Fo[2 * NFC] = Fl.S[1]; Fo[3 * NFC] = Fl.S[2];
     Fo[4 * NFC] = Fl.tau;
-    // the field fluxes keep their words where the bits agree (field-free regions)
-    const double f5 = Fo[5 * NFC], f6 = Fo[6 * NFC], f7 = Fo[7 * NFC];
-    store_changed(Fo + 5 * NFC, f5, Fl.B[0]);
-    store_changed(Fo + 6 * NFC, f6, Fl.B[1]);
-    store_changed(Fo + 7 * NFC, f7, Fl.B[2]);
+    Fo[5 * NFC] = Fl.B[0];
+    Fo[6 * NFC] = Fl.B[1];
+    Fo[7 * NFC] = Fl.B[2];
     add_faults(P, nf);
 }
 
