@@ -22,9 +22,12 @@ HBM when timing starts):
      the same initial data; L1(rho), mass and energy drift, max |div B|;
   4. per-kernel HIP-event breakdown (eager launches) and the dominant
      kernel's roofline from its launches timed alone;
-  5. CPU baseline: the oracle (C++/OpenMP restatement) stepping the same tree
+  5. the uniform 3D torus (256 x 128 x 128), 200 FP64 steps, the last 20
+     timed: zone-updates/s, HBM GB/s by the 296 B measure, finiteness and
+     max |div B|;
+  6. CPU baseline: the oracle (C++/OpenMP restatement) stepping the same tree
      from the same initial data on this host's cores;
-  6. serving: requests with Poisson arrivals, grids 64^3 .. 192 x 96 x 96
+  7. serving: requests with Poisson arrivals, grids 64^3 .. 192 x 96 x 96
      (uniform or two levels), spin ~ U(0, 0.99), beta in {30, 100, 300},
      100 steps each, continuous batching under an HBM block budget, mixed
      precision; p50 / p99 latency and aggregate zone-updates/s.
@@ -189,6 +192,38 @@ def cpu_baseline(sph, g, blocks, nsteps=2):
                       f"from the same initial data; median {med:.2f} s per step"}
 
 
+def c2_run(sph, tune, nsteps, timed_steps):
+    """The uniform 3D torus (256 x 128 x 128, 1024 blocks, 4.19 M zones),
+    FP64: the tuned launch configuration, then nsteps steps of which the last
+    timed_steps are timed per step (graph replays, HIP events); finiteness,
+    max |div B| and the mass change after the run."""
+    g = sph.c2_grid()
+    blocks = sph.uniform_blocks(g, 16)
+    eng = sph.Engine(16, len(blocks))
+    run = eng.add_run(sph.PHYSICS, g, blocks)
+    run.init_torus(sph.TORUS, 42)
+    cfg = tune.tune(eng, [run], False, use_cache=False)["config"]
+    run.init_torus(sph.TORUS, 42)
+    d0 = run.diagnostics()
+    eng.step([run], nsteps - timed_steps, want_dt=False)
+    ms = eng.step_timed([run], timed_steps)
+    run.c2p()
+    d1 = run.diagnostics()
+    U, W, Bf = run.state()
+    finite = bool(np.isfinite(run.active(U)).all() and np.isfinite(run.active(W)).all())
+    del U, W, Bf
+    zones = len(blocks) * 4096
+    med = float(np.median(ms))
+    value = zones / (med * 1e-3)
+    out = {"workload": "uniform 3D FM torus 256x128x128 (1024 blocks of 16^3), FP64, HLLD", "zones": zones,
+           "steps": nsteps, "timed_steps": timed_steps, "ms_per_step": med, "value": value,
+           "unit": "zone-updates/s", "hbm_GBps_296B": value * BYTES_PER_ZONE_UPDATE / 1e9,
+           "hbm_frac_296B": value * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS, "launch": cfg, "finite": finite,
+           "divb_rel": d1["divb_max"] / d1["divb_scale"], "mass_change": (d1["mass"] - d0["mass"]) / d0["mass"]}
+    del eng, run
+    return out
+
+
 def serving(sph, nreq, rate, budget):
     from bhfluid.serve import Server, make_requests
     reqs = make_requests(nreq, rate)
@@ -211,6 +246,8 @@ def main():
     ap.add_argument("--serve-budget", type=int, default=4000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--c2-steps", type=int, default=200,
+                    help="steps of the uniform 3D torus (256 x 128 x 128) run; 0 skips it")
     ap.add_argument("--kernels-only", choices=("fp64", "mixed", "ref"),
                     help="for counter collection: C3 initial data, then --steps eager steps of the step's kernels "
                          "in the given precision with the default launch configuration; or (ref) the reference-path "
@@ -305,6 +342,7 @@ def main():
                          fp64_frac=tfs / FP64_PEAK_TFS,
                          fp64_source="rocprofv3 SQ_INSTS_VALU_FLOPS_FP64 (profiles/flops_c3_amr_torus.md)")
 
+    c2 = c2_run(sph, tune, args.c2_steps, min(20, args.c2_steps)) if args.c2_steps > 0 else None
     cpu = None if args.no_cpu_baseline else cpu_baseline(sph, g, blocks)
     srv = None if args.no_serve else serving(sph, args.serve_requests, args.serve_rate, args.serve_budget)
 
@@ -338,6 +376,7 @@ def main():
         "roofline": roofline,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,
+        "c2": c2,
         "serving": srv,
     }
     print(json.dumps(line))

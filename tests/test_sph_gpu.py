@@ -180,6 +180,15 @@ def test_graph_replays_identical_to_eager():
             assert np.array_equal(x, y), f"replay {s}"
 
 
+def test_c2_full_size_ten_steps_fp64():
+    """C2 at its full size (256 x 128 x 128, 1024 blocks, 4.19 M zones): 10
+    FP64 steps against the oracle (which takes a few seconds per step on the
+    host's cores)."""
+    g = sph.c2_grid()
+    eng, run, orc, _, _ = make_pair(g, 16, sph.uniform_blocks(g, 16))
+    _steps(eng, run, orc, 10, False, "C2 full")
+
+
 def test_c2_full_size_stays_finite_div_b_round_off():
     """C2 at its full size (256 x 128 x 128, 1024 blocks), 10 steps FP64:
     finite, max |div B| at round-off of the stencil scale, mass and energy
