@@ -810,7 +810,10 @@ __device__ __forceinline__ void cell_hydro(const KPS& P, int b, int a, double dt
 __device__ __forceinline__ void cell_bfield(const KPS& P, int b, int i, int j, int k, int stage) {
     double* Ub = P.U + (size_t)b * 8 * P.ncell;
     const int c = cidx(i, j, k);
-    const double* Bn = (stage == 0 ? P.Bf1 : P.Bf0) + (size_t)b * 3 * P.nf;
+    // the stage's new face fields (selected as integers: a select of the two
+    // pointers was placed in scratch memory)
+    const uintptr_t a0 = (uintptr_t)P.Bf0, a1 = (uintptr_t)P.Bf1;
+    const double* Bn = (const double*)(stage == 0 ? a1 : a0) + (size_t)b * 3 * P.nf;
     Ub[(size_t)UB1 * P.ncell + c] = 0.5 * (Bn[fidx(0, i, j, k)] + Bn[fidx(0, i + 1, j, k)]);
     Ub[(size_t)UB2 * P.ncell + c] = 0.5 * (Bn[P.nf + fidx(1, i, j, k)] + Bn[P.nf + fidx(1, i, j + 1, k)]);
     Ub[(size_t)UB3 * P.ncell + c] = 0.5 * (Bn[2 * P.nf + fidx(2, i, j, k)] + Bn[2 * P.nf + fidx(2, i, j, k + 1)]);
