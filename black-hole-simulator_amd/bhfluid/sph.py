@@ -99,6 +99,7 @@ def lib():
         sig("bhs_set_variants", _i, [_v, _i, _i, _i])
         sig("bhs_variant_info", _i, [_i, _ip, _ip])
         sig("bhs_set_xcd_remap", _i, [_v, _i])
+        sig("bhs_set_face_variant", _i, [_v, _i])
         sig("bhs_run_diagnostics_static", _i, [_v, _i, _dp])
         sig("bhs_profile", _i, [_v, _i, ctypes.c_char_p])
         sig("bhs_profile_report", _i, [_v, ctypes.c_char_p, _i])
@@ -179,6 +180,9 @@ class Engine:
 
     def set_variants(self, flux: int = 0, update: int = 0, c2p: int = 0):
         self._check(self._L.bhs_set_variants(self._h, int(flux), int(update), int(c2p)), "set_variants")
+
+    def set_face_variant(self, face: int = 2):
+        self._check(self._L.bhs_set_face_variant(self._h, int(face)), "set_face_variant")
 
     def set_xcd_remap(self, enable: bool = True):
         self._check(self._L.bhs_set_xcd_remap(self._h, int(bool(enable))), "set_xcd_remap")

@@ -6,8 +6,10 @@ dimension at a time, the schedule (one kernel per sub-phase; one persistent
 queue-driven kernel per RK stage with 1 or 2 workgroups per compute unit; or
 the sub-phase kernels with fluxes and hydro update fused per block, "slab")
 (or the sub-phase kernels with stage 1's con2prim fused into stage 0's
-update) and, for the sub-phase schedules, the XCD numbering and the workgroup size
-and occupancy hint of the flux, update and con2prim kernels; each candidate
+update) and, for the sub-phase schedules, the XCD numbering, the workgroup size
+and occupancy hint of the flux, update and con2prim kernels and the form of
+the face-field update (one face per thread, or each edge's EMF once in LDS
+with 2 to 16 workgroups per block); each candidate
 is timed over a few graph-replayed steps of the caller's own runs.  The choice
 is cached per (device, block shape, block count, precision) in a JSON file
 (``$BHFLUID_TUNE_CACHE``, default ``~/.cache/bhfluid/tune.json``).
@@ -23,7 +25,7 @@ import numpy as np
 
 from . import sph
 
-DEFAULT = dict(schedule=0, wg_per_cu=0, flux=0, update=0, c2p=1, xcd=1)
+DEFAULT = dict(schedule=0, wg_per_cu=0, flux=0, update=0, c2p=1, xcd=1, face=2)
 
 
 def _cache_path() -> Path:
@@ -41,6 +43,7 @@ def apply(eng: sph.Engine, cfg: dict):
     eng.set_schedule(int(cfg.get("schedule", cfg.get("persistent", 0))), int(cfg["wg_per_cu"]))
     eng.set_variants(int(cfg["flux"]), int(cfg["update"]), int(cfg["c2p"]))
     eng.set_xcd_remap(bool(cfg.get("xcd", 1)))
+    eng.set_face_variant(int(cfg.get("face", 2)))
 
 
 def _time(eng, runs, cfg, steps: int) -> float:
@@ -84,10 +87,12 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
             best, best_ms = c, t
         # the slab schedule has no separate update kernel and computes only
         # the side faces with k_flux (one face per thread)
-        dims = ("flux", "c2p") if best["schedule"] == 2 else ("flux", "update", "c2p")
+        dims = ("flux", "c2p", "face") if best["schedule"] == 2 else ("flux", "update", "c2p", "face")
         for dim in dims:
-            top = 12 if (dim == "flux" and best["schedule"] != 2) else 6
-            for v in range(1, top):
+            top = 12 if (dim == "flux" and best["schedule"] != 2) else 5 if dim == "face" else 6
+            for v in range(0, top):
+                if v == best[dim]:
+                    continue
                 c = dict(best, **{dim: v})
                 t = _time(eng, runs, c, steps)
                 timings.append((c, t))

@@ -469,20 +469,24 @@ __device__ void phase_a_block(const KPS& P, int b, bool want_dt) {
 
 // ---------------------------------------------------------------- phase B
 // the face of direction D (a template parameter: every state index is then
-// static and the states stay in registers)
-template <class Rt, int D>
+// static and the states stay in registers).  RS = 1: every run of the launch
+// uses the HLLD solver, and only its code is compiled in (the HLL branch of
+// RS = 0, which follows the run's setting, raises the register count by the
+// values the two branches would share)
+template <class Rt, int D, int RS = 0>
 __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, int b, int i, int j, int k,
                                               Rt out[NFW]) {
     constexpr int d = D;
     const Blk B = P.blk[b];
     const DRL& v = P.rl[B.rl];
     const RunPar& R = P.run[v.run];
+    const bool use_hlld = RS == 1 || R.riemann;
     const int gi = B.bi * BX + i, gj = B.bj * BY + j;
     const int p = (gj + 2) * (v.nr + 5) + (gi + 2);
     const GLB double* tab = gl(d == 0 ? v.gf : d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc);
     // the face metric: of gamma^ij only gamma^nn enters the flux
-    Met<Rt> m;
-    {
+    auto face_met = [&]() {
+        Met<Rt> m;
         const long long np = v.npt;
         m.alpha = (Rt)tab[M_ALPHA * np + p];
         m.beta[0] = (Rt)tab[M_B1 * np + p];
@@ -495,13 +499,15 @@ __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, in
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
         m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
         m.sg = (Rt)tab[M_SG * np + p];
-        if (R.riemann) {
+        if (use_hlld) {
             // the face frame of the HLLD solver also needs gamma^r phi and,
             // on theta faces, gamma^rr
             m.gi[2] = (Rt)tab[M_I13 * np + p];
             if (d == 1) m.gi[0] = (Rt)tab[M_I11 * np + p];
         }
-    }
+        return m;
+    };
+    Met<Rt> m = face_met();
     constexpr int o0 = d == 0, o1 = d == 1, o2 = d == 2;
     constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
     // each stencil cell's word 0, in this block or, for an own face whose
@@ -557,8 +563,22 @@ __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, in
     PL.B[d] = bn;
     PR.B[d] = bn;
     const Rt gam = (Rt)R.gam;
-    if (R.riemann) {
-        hlld(m, m.gi[2], gam, PL, PR, d, out);
+    if (use_hlld) {
+        if constexpr (sizeof(Rt) == 8) {
+            // FP64: the metric is read again for the map back to
+            // coordinates (the asm statement keeps the compiler from reusing
+            // the first loads), so that neither it nor the frame is held in
+            // registers through the solve: 166 VGPRs, three wavefronts per
+            // SIMD (184 and two without)
+            hlld_m(m, m.gi[2], gam, PL, PR, d, out, [&](Rt& g13) {
+                __asm__ volatile("" ::: "memory");
+                const Met<Rt> m2 = face_met();
+                g13 = m2.gi[2];
+                return m2;
+            });
+        } else {
+            hlld(m, m.gi[2], gam, PL, PR, d, out);
+        }
         return;
     }
     Side<Rt> SL, SR;
@@ -566,19 +586,19 @@ __device__ __forceinline__ void flux_face_val(const KPS& P, const double* Bf, in
     face_side(m, gam, PR, d, SR);
     hll(m, SL, SR, out);
 }
-template <class Rt, int D>
+template <class Rt, int D, int RS = 0>
 __device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int b, int i, int j, int k) {
     Rt out[NFW];
-    flux_face_val<Rt, D>(P, Bf, b, i, j, k, out);
+    flux_face_val<Rt, D, RS>(P, Bf, b, i, j, k, out);
     Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)D * NFW * P.nf + fidx(D, i, j, k);
 #pragma unroll
     for (int q = 0; q < NFW; q++) F[(size_t)q * P.nf] = out[q];
 }
-template <class Rt>
+template <class Rt, int RS = 0>
 __device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, int j, int k) {
-    if (d == 0) flux_face_d<Rt, 0>(P, Bf, b, i, j, k);
-    else if (d == 1) flux_face_d<Rt, 1>(P, Bf, b, i, j, k);
-    else flux_face_d<Rt, 2>(P, Bf, b, i, j, k);
+    if (d == 0) flux_face_d<Rt, 0, RS>(P, Bf, b, i, j, k);
+    else if (d == 1) flux_face_d<Rt, 1, RS>(P, Bf, b, i, j, k);
+    else flux_face_d<Rt, 2, RS>(P, Bf, b, i, j, k);
 }
 
 // All faces of direction D (1: theta, 2: phi) along one line of the block at
@@ -895,7 +915,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     }
 }
 
-template <class Rt, int TB, int MW>
+template <class Rt, int TB, int MW, int RS>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_flux(KPS P, int buf) {
     const int per = (P.nflf + TB - 1) / TB;
     const int w = wg_index(P);
@@ -906,7 +926,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     const int x = P.flf[t];
     unpack(x, i, j, k);
     if (face_shared(P, b, x >> 24, i, j, k)) return;
-    flux_face<Rt>(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
+    flux_face<Rt, RS>(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
 }
 
 template <class Rt>
@@ -920,6 +940,105 @@ __global__ __launch_bounds__(TPB) void k_face(KPS P, int stage) {
     const int x = P.owf[t];
     unpack(x, i, j, k);
     face_update<Rt>(P, b, x >> 24, i, j, k, run_dt(P, b), stage);
+}
+
+// The face-field update with each edge's EMF formed once.  k_face forms the
+// four EMFs of a face from its four flux words each, so every EMF is formed
+// by each of the (up to four) faces around its edge, and most of the work is
+// flux addressing.  Here nsplit workgroups share a block's phi layers: part
+// h takes layers [k0, k1), first forms the EMFs of every edge one of its own
+// faces reads into LDS (the same emf_x/y/z calls, stored in Rt as k_face
+// uses them), then updates those faces from LDS with face_update's
+// expressions.  The phi faces at k1 belong to the next part, except the
+// block's last one (k1 = bz), which the last part updates.
+//   Ez(i, j, kk): edges along phi at (r_{i-1/2}, theta_{j-1/2}, phi_{k0+kk}),
+//                 i, j in [0, 16], kk in [0, KL)
+//   Ey(i, j, kk): edges along theta at (r_{i-1/2}, theta_j, phi_{k0+kk-1/2}),
+//                 i in [0, 16], j in [0, 16), kk in [0, KL]
+//   Ex(i, j, kk): edges along r at (r_i, theta_{j-1/2}, phi_{k0+kk-1/2}),
+//                 i in [0, 16), j in [0, 16], kk in [0, KL]
+constexpr int FE_Z = (BX + 1) * (BY + 1), FE_Y = (BX + 1) * BY, FE_X = BX * (BY + 1);
+__host__ __device__ constexpr int face_edges(int kl) { return FE_Z * kl + (FE_Y + FE_X) * (kl + 1); }
+
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_face_edges(KPS P, int stage, int nsplit) {
+    extern __shared__ __align__(16) unsigned char face_lds[];
+    Rt* sE = (Rt*)face_lds;
+    const int w = wg_index(P);
+    const int b = P.list[w / nsplit];
+    const int h = w % nsplit;
+    const int k0 = h * P.bz / nsplit, k1 = (h + 1) * P.bz / nsplit;
+    const int kl = k1 - k0;
+    Rt* Ez = sE;
+    Rt* Ey = Ez + FE_Z * kl;
+    Rt* Ex = Ey + FE_Y * (kl + 1);
+    const Blk B = P.blk[b];
+    const DRL& v = P.rl[B.rl];
+    const int gj0 = B.bj * BY;
+    {
+        const SideNb n = side_nb(P, b);
+        const int ne = face_edges(kl);
+        for (int e = threadIdx.x; e < ne; e += TPB) {
+            int r = e;
+            if (r < FE_Z * kl) {
+                const int kk = r / FE_Z, q = r % FE_Z;
+                Ez[r] = emf_z<Rt>(P, b, n, gj0, v.nth, q % (BX + 1), q / (BX + 1), k0 + kk);
+                continue;
+            }
+            r -= FE_Z * kl;
+            if (r < FE_Y * (kl + 1)) {
+                const int kk = r / FE_Y, q = r % FE_Y;
+                Ey[r] = emf_y<Rt>(P, b, n, q % (BX + 1), q / (BX + 1), k0 + kk);
+                continue;
+            }
+            r -= FE_Y * (kl + 1);
+            const int kk = r / FE_X, q = r % FE_X;
+            Ex[r] = emf_x<Rt>(P, b, n, gj0, v.nth, q % BX, q / BX, k0 + kk);
+        }
+    }
+    __syncthreads();
+    const double dt = run_dt(P, b);
+    const bool last = k1 == P.bz;
+    const int nr = (BX + 1) * BY * kl, nt = BX * (BY + 1) * kl, np = BX * BY * (kl + (last ? 1 : 0));
+    auto ez = [&](int i, int j, int kk) { return (double)Ez[(kk * (BY + 1) + j) * (BX + 1) + i]; };
+    auto ey = [&](int i, int j, int kk) { return (double)Ey[(kk * BY + j) * (BX + 1) + i]; };
+    auto ex = [&](int i, int j, int kk) { return (double)Ex[(kk * (BY + 1) + j) * BX + i]; };
+    for (int f = threadIdx.x; f < nr + nt + np; f += TPB) {
+        int d, i, j, kk;
+        double rhs;
+        if (f < nr) {
+            d = 0;
+            i = f % (BX + 1);
+            j = (f / (BX + 1)) % BY;
+            kk = f / ((BX + 1) * BY);
+            const double a1 = (ez(i, j + 1, kk) - ez(i, j, kk));
+            const double a2 = (ey(i, j, kk + 1) - ey(i, j, kk));
+            rhs = -a1 / v.dth + a2 / v.dph;
+        } else if (f < nr + nt) {
+            const int g = f - nr;
+            d = 1;
+            i = g % BX;
+            j = (g / BX) % (BY + 1);
+            kk = g / (BX * (BY + 1));
+            const double dri = gl(v.dr)[B.bi * BX + i + 2];
+            const double a1 = (ex(i, j, kk + 1) - ex(i, j, kk));
+            const double a2 = (ez(i + 1, j, kk) - ez(i, j, kk));
+            rhs = -a1 / v.dph + a2 / dri;
+        } else {
+            const int g = f - nr - nt;
+            d = 2;
+            i = g % BX;
+            j = (g / BX) % BY;
+            kk = g / (BX * BY);
+            const double dri = gl(v.dr)[B.bi * BX + i + 2];
+            const double a1 = (ey(i + 1, j, kk) - ey(i, j, kk));
+            const double a2 = (ex(i, j + 1, kk) - ex(i, j, kk));
+            rhs = -a1 / dri + a2 / v.dth;
+        }
+        const size_t x = ((size_t)b * 3 + d) * P.nf + fidx(d, i, j, k0 + kk);
+        if (stage == 0) P.Bf1[x] = P.Bf0[x] + dt * rhs;
+        else P.Bf0[x] = 0.5 * (P.Bf0[x] + (P.Bf1[x] + dt * rhs));
+    }
 }
 
 template <class Rt, int TB, int MW>
@@ -1452,13 +1571,23 @@ constexpr int VAR_MW[NVAR] = {1, 3, 4, 1, 3, 4};
 #define SPH_VARIANTS(M) M(0, 256, 1) M(1, 256, 3) M(2, 256, 4) M(3, 128, 1) M(4, 128, 3) M(5, 128, 4)
 
 // flux variants 0 .. 5: one face per thread; 6 .. 11: theta and phi faces
-// by lines (same workgroup shapes)
+// by lines (same workgroup shapes).  hlld_all: every active run uses the
+// HLLD solver (the per-face kernels then carry its code only)
 constexpr int NVAR_FLUX = 2 * NVAR;
 template <class Rt>
-void launch_flux(int v, int nl, int nitems, int nlines, hipStream_t st, KPS P, int buf) {
+void launch_flux(int v, int nl, int nitems, int nlines, hipStream_t st, KPS P, int buf, bool hlld_all) {
+    if (hlld_all && v < NVAR) {
+        switch (v) {
+#define L_(i, tb, mw) \
+    case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw, 1>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+            SPH_VARIANTS(L_)
+#undef L_
+        }
+        return;
+    }
     switch (v) {
 #define L_(i, tb, mw) \
-    case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+    case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw, 0>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
         SPH_VARIANTS(L_)
 #undef L_
 #define L_(i, tb, mw) \
@@ -1484,6 +1613,21 @@ void launch_update_c2p(int v, int nl, int nitems, hipStream_t st, KPS P) {
         SPH_VARIANTS(L_)
 #undef L_
     }
+}
+// face variants: 0 one face per thread (k_face); 1 .. 4 the edge-EMF kernel
+// with 2, 4, 8, 16 workgroups per block (at most bz), LDS for the edges of
+// ceil(bz / parts) layers
+constexpr int NVAR_FACE = 5;
+inline int face_parts(int v, int bz) { return std::min(1 << v, bz); }
+template <class Rt>
+void launch_face(int v, int nl, int nown, int bz, hipStream_t st, KPS P, int stage) {
+    if (v <= 0) {
+        hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nown + TPB - 1) / TPB)), dim3(TPB), 0, st, P, stage);
+        return;
+    }
+    const int ns = face_parts(v, bz);
+    const size_t lds = sizeof(Rt) * (size_t)face_edges((bz + ns - 1) / ns);
+    hipLaunchKernelGGL(k_face_edges<Rt>, dim3(nl * ns), dim3(TPB), lds, st, P, stage, ns);
 }
 inline void launch_c2p(int v, int nl, hipStream_t st, KPS P, int want_dt) {
     switch (v) {
@@ -1561,12 +1705,14 @@ public:
     std::vector<HostRun> runs;
     std::vector<int> free_slots;
     std::vector<int> act_runs;    // runs of the current active list
+    bool act_hlld = false;        // every active run uses the HLLD solver
     std::vector<int> act_slots;
     bool mixed = false;
     int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update), 3 multi-kernel with
                                   // stage 1's con2prim fused into stage 0's update
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
     int var_flux = 0, var_update = 0, var_c2p = 1;  // launch variants (VAR_TB, VAR_MW)
+    int var_face = 2;             // face-update variant (launch_face)
     bool xcd_remap = true;        // XCD-contiguous workgroup numbering
     bool tables_dirty = true;
     std::string err;
@@ -1574,7 +1720,7 @@ public:
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     bool use_graph = true;
-    int graph_key = -1;
+    long long graph_key = -1;
 
     double* scratch_dev = nullptr;  // this device's copy of the static scratch array
 
@@ -1946,6 +2092,8 @@ public:
         SCHK(hipStreamSynchronize(stream));
         act_runs = rr;
         act_slots = sl;
+        act_hlld = true;
+        for (int r : rr) act_hlld = act_hlld && runs[r].par.riemann == BHS_HLLD;
         graph_release();
     }
 
@@ -2048,16 +2196,16 @@ public:
                     hipLaunchKernelGGL(k_slab<Rt>, dim3(nl * ns), dim3(TPB), 0, stream, P, buf, stage, ns);
                 });
                 tm("k_face", [&] {
-                    hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+                    launch_face<Rt>(var_face, nl, nowf, L.bz, stream, P, stage);
                 });
                 tm("k_cellb", [&] {
                     hipLaunchKernelGGL(k_cellb, dim3(nl * ((L.nint + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
                 });
                 continue;
             }
-            tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, nflp, stream, P, buf); });
+            tm("k_flux", [&] { launch_flux<Rt>(var_flux, nl, nflf, nflp, stream, P, buf, act_hlld); });
             tm("k_face", [&] {
-                hipLaunchKernelGGL(k_face<Rt>, dim3(nl * ((nowf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, stage);
+                launch_face<Rt>(var_face, nl, nowf, L.bz, stream, P, stage);
             });
             if (mode == 3 && stage == 0)
                 tm("k_update_c2p", [&] { launch_update_c2p<Rt>(var_update, nl, L.nint, stream, P); });
@@ -2070,9 +2218,10 @@ public:
         if (mixed) enqueue_step<float>();
         else enqueue_step<double>();
     }
-    int key() const {
-        return (mixed ? 1 : 0) | (mode << 1) | (wg_per_cu << 3) | (var_flux << 9) | (var_update << 12) |
-               (var_c2p << 15) | ((xcd_remap ? 1 : 0) << 18);
+    long long key() const {
+        return (long long)(mixed ? 1 : 0) | ((long long)mode << 2) | ((long long)wg_per_cu << 5) |
+               ((long long)var_flux << 13) | ((long long)var_update << 19) | ((long long)var_c2p << 25) |
+               ((long long)(xcd_remap ? 1 : 0) << 31) | ((long long)var_face << 32);
     }
     void step() {
         if (act_slots.empty()) throw std::invalid_argument("no active runs");
@@ -2401,6 +2550,13 @@ int bhs_set_variants(bhs_engine* h, int flux, int update, int c2p) {
         h->e->var_flux = flux;
         h->e->var_update = update;
         h->e->var_c2p = c2p;
+        return BHS_OK;
+    });
+}
+int bhs_set_face_variant(bhs_engine* h, int face) {
+    return guard(h, [&] {
+        if (face < 0 || face >= NVAR_FACE) throw std::invalid_argument("face variant out of range");
+        h->e->var_face = face;
         return BHS_OK;
     });
 }

@@ -827,25 +827,40 @@ SPH_HD void frame_to_coord(const Met<R>& m, const Frame<R>& f, int n, const R F[
 
 // HLLD flux at a face (the coordinate words of hll); gi13 = gamma^r phi at
 // the face.  Axis faces (sqrt(gamma) = 0) carry no flux.  Returns 1 where
-// the HLL fallback was taken.
-template <class R>
-SPH_HD int hlld(const Met<R>& m, R gi13, R gam, Pt<R>& PL, Pt<R>& PR, int n, R out[NFW], bool hll_only = false,
-                HlldFan<R>* fan = nullptr) {
+// the HLL fallback was taken.  remet(g) returns the face metric again (and
+// gamma^r phi in g) for the map back to coordinates: the frame is formed a
+// second time from it after the solve, with the same operations, so that
+// neither the metric nor the frame has to be held through the solve (a
+// caller that has the metric at hand returns it).
+template <class R, class MetFn>
+SPH_HD int hlld_m(const Met<R>& m, R gi13, R gam, Pt<R>& PL, Pt<R>& PR, int n, R out[NFW], MetFn remet,
+                  bool hll_only = false, HlldFan<R>* fan = nullptr) {
     if (!(m.sg > R(0))) {
 #pragma unroll
         for (int q = 0; q < NFW; q++) out[q] = R(0);
         return 0;
     }
-    const Frame<R> fr = make_frame(m, gi13, n);
     const R bn = PL.B[n];
-    const R bx = bn * fr.irn;
-    FSide<R> L, Rr;
-    frame_side(m, fr, gam, PL, n, bx, L);
-    frame_side(m, fr, gam, PR, n, bx, Rr);
     R F[8];
-    const int fb = hlld_frame(L, Rr, bx, fr.vs, gam, F, hll_only, fan);
-    frame_to_coord(m, fr, n, F, bn, out);
+    int fb;
+    {
+        const Frame<R> fr = make_frame(m, gi13, n);
+        const R bx = bn * fr.irn;
+        FSide<R> L, Rr;
+        frame_side(m, fr, gam, PL, n, bx, L);
+        frame_side(m, fr, gam, PR, n, bx, Rr);
+        fb = hlld_frame(L, Rr, bx, fr.vs, gam, F, hll_only, fan);
+    }
+    R g13;
+    const Met<R> m2 = remet(g13);
+    const Frame<R> f2 = make_frame(m2, g13, n);
+    frame_to_coord(m2, f2, n, F, bn, out);
     return fb;
+}
+template <class R>
+SPH_HD int hlld(const Met<R>& m, R gi13, R gam, Pt<R>& PL, Pt<R>& PR, int n, R out[NFW], bool hll_only = false,
+                HlldFan<R>* fan = nullptr) {
+    return hlld_m(m, gi13, gam, PL, PR, n, out, [&](R& g) { g = gi13; return m; }, hll_only, fan);
 }
 
 // Limited linear (minmod) reconstruction at the face between cells c-1 and

@@ -169,6 +169,12 @@ int bhs_use_graph(bhs_engine* e, int enable);
  * workgroups per block to 2^(v mod 5) (at most bz). */
 int bhs_set_variants(bhs_engine* e, int flux, int update, int c2p);
 int bhs_variant_info(int variant, int* tpb, int* min_waves);
+/* Variant of the face-field update (constrained transport, the spherical
+ * form of transport.cpp:11-75): 0 = one own face per thread, its four EMFs
+ * formed from the flux words; 1 .. 4 = each edge's EMF formed once into LDS
+ * by 2, 4, 8 or 16 workgroups per block (at most bz), then the faces updated
+ * from LDS (the default is 2).  Results do not depend on the variant. */
+int bhs_set_face_variant(bhs_engine* e, int face);
 /* Number the workgroups of the multi-workgroup kernels so that each of the
  * eight XCDs (own L2) takes a contiguous range of blocks (1, the default) or
  * in dispatch order (0). */

@@ -233,6 +233,35 @@ def test_launch_variants_identical(mixed):
                 assert np.array_equal(x, y), (flux, update, c2p, xcd)
 
 
+@pytest.mark.parametrize("mixed,bz", [(False, 16), (True, 16), (False, 1)])
+def test_face_variants_identical(mixed, bz):
+    """The face-field update with each edge's EMF formed once in LDS (2, 4,
+    8, 16 workgroups per block; bz = 1: one) gives the bits of the update
+    that forms a face's four EMFs itself (variant 0), on a 3-level tree and
+    on the axisymmetric C1 grid."""
+    if bz == 16:
+        g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+        blocks = sph.amr_blocks(g, 3)
+    else:
+        g = sph.c1_grid()
+        blocks = sph.uniform_blocks(g, 1)
+    ref = None
+    for face in (0, 1, 2, 3, 4):
+        eng = sph.Engine(bz, len(blocks))
+        eng.set_precision(mixed)
+        eng.set_face_variant(face)
+        run = eng.add_run(sph.PHYSICS, g, blocks)
+        run.init_torus(sph.TORUS, 42)
+        dt = eng.step([run], 3)
+        st = run.state()
+        if ref is None:
+            ref = (dt, st)
+        else:
+            assert np.array_equal(dt, ref[0]), face
+            for x, y in zip(st, ref[1]):
+                assert np.array_equal(x, y), face
+
+
 def test_serving_results_same_alone_and_in_the_mix():
     """Continuous batching: requests of different grids, spins and fields
     arrive while others run, are admitted as block slots free up (a budget
