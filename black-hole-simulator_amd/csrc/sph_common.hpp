@@ -44,6 +44,7 @@ struct Lay {
     int ncell;      // stored cells per block
     int nint;       // active zones per block
     int nf;         // padded faces per direction (max of the three index spaces)
+    int nff;        // flux words per direction and word (ffidx below)
 };
 SPH_HD Lay make_lay(int bz) {
     Lay L;
@@ -53,6 +54,11 @@ SPH_HD Lay make_lay(int bz) {
     L.nint = BX * BY * bz;
     const int a = (SX + 1) * SY * L.sz, b = SX * SY * (L.sz + 1);
     L.nf = a > b ? a : b;
+    // own faces plus four side slabs (ffidx): r and theta directions
+    // 17 x 16 x bz + 2 (17 bz + 272), phi 256 (bz + 1) + 64 (bz + 1)
+    const int f01 = (BX + 1) * BY * bz + 2 * (BX + 1) * bz + 2 * (BX + 1) * BY;
+    const int f2 = BX * BY * (bz + 1) + 2 * BY * (bz + 1) + 2 * BX * (bz + 1);
+    L.nff = f01 > f2 ? f01 : f2;
     return L;
 }
 SPH_HD int cidx(int i, int j, int k) { return ((k + G) * SY + (j + G)) * SX + (i + G); }
@@ -62,6 +68,44 @@ SPH_HD int fidx(int d, int i, int j, int k) {
     return d == 0 ? ((k + G) * SY + (j + G)) * (SX + 1) + (i + G)
          : d == 1 ? ((k + G) * (SY + 1) + (j + G)) * SX + (i + G)
                   : ((k + G) * SY + (j + G)) * SX + (i + G);
+}
+
+// Flux words of a block, per direction d: the own faces (coordinate along d
+// in [0, n_d], the two transverse ones inside the block) as a dense box with
+// i fastest, then four slabs of side faces (one transverse coordinate at -1
+// or n_e, the other inside), ordered by transverse direction e and low/high
+// side, each slab dense in its two remaining coordinates (the less
+// significant fastest, as the face lists order them).  Corner faces (both
+// transverse coordinates outside) are never formed and have no word.  Every
+// word of a slot is written by the flux kernel (own faces always, a side
+// slab whenever the block does not leave that side to its neighbour), so
+// full cache lines reach memory.
+SPH_HD int ffown(int d, int i, int j, int k, int bz) {
+    (void)bz;
+    return d == 0 ? (k * BY + j) * (BX + 1) + i : d == 1 ? (k * (BY + 1) + j) * BX + i : (k * BY + j) * BX + i;
+}
+SPH_HD int ffidx(int d, int i, int j, int k, int bz) {
+    const int nx = BX + (d == 0), ny = BY + (d == 1), nz = bz + (d == 2);
+    const int own = nx * ny * nz;
+    // the transverse coordinate outside, if any
+    if (d != 0 && (i < 0 || i >= BX)) {
+        // e = 0: j (ny) fastest, then k (nz)
+        const int h = i < 0 ? 0 : 1;
+        return own + h * ny * nz + k * ny + j;
+    }
+    const int s0 = d != 0 ? 2 * ny * nz : 0;  // the e = 0 slabs
+    if (d != 1 && (j < 0 || j >= BY)) {
+        // e = 1: i (nx) fastest, then k (nz)
+        const int h = j < 0 ? 0 : 1;
+        return own + s0 + h * nx * nz + k * nx + i;
+    }
+    const int s1 = s0 + (d != 1 ? 2 * nx * nz : 0);
+    if (d != 2 && (k < 0 || k >= bz)) {
+        // e = 2: i (nx) fastest, then j (ny)
+        const int h = k < 0 ? 0 : 1;
+        return own + s1 + h * nx * ny + j * nx + i;
+    }
+    return (k * ny + j) * nx + i;
 }
 
 // Geometry tables, per (run, level), over the points (ii, jj) of the level's

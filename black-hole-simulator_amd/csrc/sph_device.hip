@@ -83,7 +83,7 @@ __device__ __forceinline__ const GLB T* gl(const T* p) {
 }
 
 struct KPS {
-    int bz, sz, ncell, nint, nf;
+    int bz, sz, ncell, nint, nf, nff;
     const Blk* blk;       // [slot]
     const DRL* rl;        // [rl]
     const RunPar* run;    // [run]
@@ -590,9 +590,9 @@ template <class Rt, int D, int RS = 0>
 __device__ __forceinline__ void flux_face_d(const KPS& P, const double* Bf, int b, int i, int j, int k) {
     Rt out[NFW];
     flux_face_val<Rt, D, RS>(P, Bf, b, i, j, k, out);
-    Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)D * NFW * P.nf + fidx(D, i, j, k);
+    Rt* F = (Rt*)P.F + (size_t)b * 3 * NFW * P.nff + (size_t)D * NFW * P.nff + ffidx(D, i, j, k, P.bz);
 #pragma unroll
-    for (int q = 0; q < NFW; q++) F[(size_t)q * P.nf] = out[q];
+    for (int q = 0; q < NFW; q++) F[(size_t)q * P.nff] = out[q];
 }
 template <class Rt, int RS = 0>
 __device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, int j, int k) {
@@ -671,7 +671,7 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
 #pragma unroll
         for (int q = 0; q < NV; q++) sp[q] = minmod(prev[q] - a[q], cur[q] - prev[q]);
     }
-    Rt* Fd = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf + (size_t)d * NFW * P.nf;
+    Rt* Fd = (Rt*)P.F + (size_t)b * 3 * NFW * P.nff + (size_t)d * NFW * P.nff;
     const double* Bfd = Bf + ((size_t)b * 3 + d) * P.nf;
     for (int f = 0; f <= n; f++) {
         Rt nx[NV], sc[NV];
@@ -706,7 +706,7 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
             hll(m, SL, SR, out);
         }
 #pragma unroll
-        for (int q = 0; q < NFW; q++) Fd[(size_t)q * P.nf + fx] = out[q];
+        for (int q = 0; q < NFW; q++) Fd[(size_t)q * P.nff + ffidx(d, i, fj, fk, P.bz)] = out[q];
 #pragma unroll
         for (int q = 0; q < NV; q++) {
             prev[q] = cur[q];
@@ -718,7 +718,7 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
 
 template <class Rt>
 __device__ __forceinline__ Rt Fat(const KPS& P, const Rt* Fb, int d, int q, int i, int j, int k) {
-    return Fb[((size_t)d * NFW + q) * P.nf + fidx(d, i, j, k)];
+    return Fb[((size_t)d * NFW + q) * P.nff + ffown(d, i, j, k, P.bz)];
 }
 // flux word of face (d; i, j, k) of block b, from the same-level neighbour
 // where b left that side face to it (at most one transverse coordinate is
@@ -738,7 +738,7 @@ __device__ __forceinline__ Rt Fsh(const KPS& P, int b, const SideNb& n, int d, i
         if (k < 0 && n.s[4] >= 0) { s = n.s[4]; k += P.bz; }
         else if (k >= P.bz && n.s[5] >= 0) { s = n.s[5]; k -= P.bz; }
     }
-    return ((const Rt*)P.F)[(((size_t)s * 3 + d) * NFW + q) * P.nf + fidx(d, i, j, k)];
+    return ((const Rt*)P.F)[(((size_t)s * 3 + d) * NFW + q) * P.nff + ffidx(d, i, j, k, P.bz)];
 }
 
 template <class Rt>
@@ -840,7 +840,7 @@ __device__ __forceinline__ void cell_bfield(const KPS& P, int b, int i, int j, i
 }
 template <class Rt>
 __device__ void cell_update(const KPS& P, int b, int a, double dt, int stage) {
-    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    const Rt* Fb = (const Rt*)P.F + (size_t)b * 3 * NFW * P.nff;
     cell_hydro<Rt>(P, b, a, dt, stage,
                    [&](int d, int q, int i, int j, int k) { return Fat(P, Fb, d, q, i, j, k); });
     cell_bfield(P, b, a % BX, (a / BX) % BY, a / (BX * BY), stage);
@@ -1085,7 +1085,7 @@ constexpr int SL_X = (BX + 1) * BY, SL_Y = BX * (BY + 1), SL_Z = BX * BY;
 // (h+1) bz/nsplit) and forms the phi faces at its first layer itself (those
 // at a shared boundary layer are formed by both parts, with the same bits);
 // the side faces are dealt out over the parts.
-template <class Rt>
+template <class Rt, int RS>
 __global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsplit) {
     __shared__ Rt sx[5][SL_X], sy[5][SL_Y], sz[2][5][SL_Z];
     const int wg = wg_index(P);
@@ -1093,11 +1093,11 @@ __global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsp
     const int k0 = part * P.bz / nsplit, k1 = (part + 1) * P.bz / nsplit;
     const double* Bf = buf ? P.Bf1 : P.Bf0;
     const double dt = run_dt(P, b);
-    Rt* Fb = (Rt*)P.F + (size_t)b * 3 * NFW * P.nf;
+    Rt* Fb = (Rt*)P.F + (size_t)b * 3 * NFW * P.nff;
     auto emf_words = [&](int D, int i, int j, int k, const Rt out[NFW]) {
-        const size_t x = fidx(D, i, j, k);
-        Fb[((size_t)D * NFW + 5) * P.nf + x] = out[5];
-        Fb[((size_t)D * NFW + 6) * P.nf + x] = out[6];
+        const size_t x = ffidx(D, i, j, k, P.bz);
+        Fb[((size_t)D * NFW + 5) * P.nff + x] = out[5];
+        Fb[((size_t)D * NFW + 6) * P.nff + x] = out[6];
     };
     // the side faces of the sides this block does not share (only their EMF
     // words are read; a shared side's faces are the neighbour's own)
@@ -1112,9 +1112,9 @@ __global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsp
                 int i, j, k;
                 unpack(P.flf[off + t], i, j, k);
                 Rt out[NFW];
-                if (D == 0) flux_face_val<Rt, 0>(P, Bf, b, i, j, k, out);
-                else if (D == 1) flux_face_val<Rt, 1>(P, Bf, b, i, j, k, out);
-                else flux_face_val<Rt, 2>(P, Bf, b, i, j, k, out);
+                if (D == 0) flux_face_val<Rt, 0, RS>(P, Bf, b, i, j, k, out);
+                else if (D == 1) flux_face_val<Rt, 1, RS>(P, Bf, b, i, j, k, out);
+                else flux_face_val<Rt, 2, RS>(P, Bf, b, i, j, k, out);
                 emf_words(D, i, j, k, out);
             }
         }
@@ -1122,7 +1122,7 @@ __global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsp
     for (int t = threadIdx.x; t < SL_Z; t += TPB) {
         const int i = t % BX, j = t / BX;
         Rt out[NFW];
-        flux_face_val<Rt, 2>(P, Bf, b, i, j, k0, out);
+        flux_face_val<Rt, 2, RS>(P, Bf, b, i, j, k0, out);
 #pragma unroll
         for (int q = 0; q < 5; q++) sz[k0 & 1][q][t] = out[q];
         emf_words(2, i, j, k0, out);
@@ -1134,19 +1134,19 @@ __global__ __launch_bounds__(TPB) void k_slab(KPS P, int buf, int stage, int nsp
             Rt out[NFW];
             if (t < SL_X) {
                 const int i = t % (BX + 1), j = t / (BX + 1);
-                flux_face_val<Rt, 0>(P, Bf, b, i, j, k, out);
+                flux_face_val<Rt, 0, RS>(P, Bf, b, i, j, k, out);
 #pragma unroll
                 for (int q = 0; q < 5; q++) sx[q][t] = out[q];
                 emf_words(0, i, j, k, out);
             } else if (t < SL_X + SL_Y) {
                 const int u = t - SL_X, i = u % BX, j = u / BX;
-                flux_face_val<Rt, 1>(P, Bf, b, i, j, k, out);
+                flux_face_val<Rt, 1, RS>(P, Bf, b, i, j, k, out);
 #pragma unroll
                 for (int q = 0; q < 5; q++) sy[q][u] = out[q];
                 emf_words(1, i, j, k, out);
             } else {
                 const int u = t - SL_X - SL_Y, i = u % BX, j = u / BX;
-                flux_face_val<Rt, 2>(P, Bf, b, i, j, k + 1, out);
+                flux_face_val<Rt, 2, RS>(P, Bf, b, i, j, k + 1, out);
 #pragma unroll
                 for (int q = 0; q < 5; q++) sz[cur ^ 1][q][u] = out[q];
                 emf_words(2, i, j, k + 1, out);
@@ -1735,7 +1735,7 @@ public:
         Us.alloc((size_t)cap * 5 * L.nint);
         Bf0.alloc((size_t)cap * 3 * L.nf);
         Bf1.alloc((size_t)cap * 3 * L.nf);
-        F.alloc((size_t)cap * 3 * NFW * L.nf);  // FP64 words; the FP32 path uses the first half of each block's
+        F.alloc((size_t)cap * 3 * NFW * L.nff);  // FP64 words; the FP32 path uses the first half of each block's
         SCHK(hipMemsetAsync(Bf0.p, 0, Bf0.n * sizeof(double), stream));
         SCHK(hipMemsetAsync(Bf1.p, 0, Bf1.n * sizeof(double), stream));
         SCHK(hipMemsetAsync(F.p, 0, F.n * sizeof(double), stream));
@@ -1846,7 +1846,7 @@ public:
 
     KPS kp(const int* lst, int nlst) {
         KPS P;
-        P.bz = L.bz; P.sz = L.sz; P.ncell = L.ncell; P.nint = L.nint; P.nf = L.nf;
+        P.bz = L.bz; P.sz = L.sz; P.ncell = L.ncell; P.nint = L.nint; P.nf = L.nf; P.nff = L.nff;
         P.blk = blk.p; P.rl = rl.p; P.run = runp.p; P.ctr = ctr.p;
         P.U = U.p; P.W = W.p; P.Us = Us.p; P.Bf0 = Bf0.p; P.Bf1 = Bf1.p; P.F = F.p;
         P.list = lst; P.nlist = nlst;
@@ -2193,7 +2193,10 @@ public:
                 // block's layers: 2^(v mod 5), at most bz
                 const int ns = std::min(1 << (var_flux % 5), L.bz);
                 tm("k_slab", [&] {
-                    hipLaunchKernelGGL(k_slab<Rt>, dim3(nl * ns), dim3(TPB), 0, stream, P, buf, stage, ns);
+                    if (act_hlld)
+                        hipLaunchKernelGGL((k_slab<Rt, 1>), dim3(nl * ns), dim3(TPB), 0, stream, P, buf, stage, ns);
+                    else
+                        hipLaunchKernelGGL((k_slab<Rt, 0>), dim3(nl * ns), dim3(TPB), 0, stream, P, buf, stage, ns);
                 });
                 tm("k_face", [&] {
                     launch_face<Rt>(var_face, nl, nowf, L.bz, stream, P, stage);
@@ -2485,7 +2488,7 @@ const char* bhs_last_error(const bhs_engine* h) { return h ? h->err.c_str() : g_
 
 long long bhs_block_bytes(int bz) {
     const Lay L = make_lay(bz);
-    return (long long)sizeof(double) * (2LL * 8 * L.ncell + 5LL * L.nint + 2LL * 3 * L.nf + 3LL * NFW * L.nf);
+    return (long long)sizeof(double) * (2LL * 8 * L.ncell + 5LL * L.nint + 2LL * 3 * L.nf + 3LL * NFW * L.nff);
 }
 
 int bhs_run_add(bhs_engine* h, const bhs_physics* ph, const bhs_grid* g, int nblocks, const int* blocks, int* run) {
