@@ -98,6 +98,17 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
                 timings.append((c, t))
                 if t < best_ms:
                     best, best_ms = c, t
+            if dim == "update" and best["schedule"] in (0, 3):
+                # schedules 0 and 3 differ in the update kernel (3 runs
+                # stage 1's con2prim inside stage 0's update), so each is
+                # judged with its own best update variant
+                other = 3 - best["schedule"]
+                for v in range(0, 6):
+                    c = dict(best, schedule=other, update=v)
+                    t = _time(eng, runs, c, steps)
+                    timings.append((c, t))
+                    if t < best_ms:
+                        best, best_ms = c, t
     apply(eng, best)
     out = dict(config=best, cached=False, timings=[dict(cfg=c, ms=t) for c, t in timings])
     if use_cache:

@@ -100,6 +100,10 @@ struct KPS {
     int nghc;
     const int* ghf;       // ghost faces: d<<24 | packed cell
     int nghf;
+    // the ghost cells, then the side faces (bit 30 set), that a block whose
+    // shared sides form mask m fills: ghp[ghp_off[m] .. ghp_off[m + 1])
+    const int* ghp;
+    const int* ghp_off;
     const int* flf;       // flux faces
     int nflf;
     const int* owf;       // own faces
@@ -134,11 +138,31 @@ __device__ __forceinline__ int wg_index(const KPS& P) {
 }
 
 // the side (0..5) a flux face of direction d lies outside the block on, or -1
-__device__ __forceinline__ int face_side_of(const KPS& P, int d, int i, int j, int k) {
+__host__ __device__ __forceinline__ int face_side_bz(int d, int i, int j, int k, int bz) {
     if (d != 0) { if (i < 0) return 0; if (i >= BX) return 1; }
     if (d != 1) { if (j < 0) return 2; if (j >= BY) return 3; }
-    if (d != 2) { if (k < 0) return 4; if (k >= P.bz) return 5; }
+    if (d != 2) { if (k < 0) return 4; if (k >= bz) return 5; }
     return -1;
+}
+__device__ __forceinline__ int face_side_of(const KPS& P, int d, int i, int j, int k) {
+    return face_side_bz(d, i, j, k, P.bz);
+}
+// whether any flux stencil reads ghost cell (i, j, k) of a block whose
+// shared sides are the set bits of mask (bit e: side e): corner cells
+// (three coordinates outside) never; a slab cell only where its side is not
+// shared; an edge cell (two outside) only where one of its two sides is not
+// shared (the side faces computed there read it)
+__host__ __device__ __forceinline__ bool ghost_read_mask(int mask, int i, int j, int k, int bz) {
+    const int si = i < 0 ? 0 : (i >= BX ? 1 : -1);
+    const int sj = j < 0 ? 2 : (j >= BY ? 3 : -1);
+    const int sk = k < 0 ? 4 : (k >= bz ? 5 : -1);
+    const int nout = (si >= 0) + (sj >= 0) + (sk >= 0);
+    if (nout == 3) return false;
+    bool all = true;
+    if (si >= 0) all = all && ((mask >> si) & 1);
+    if (sj >= 0) all = all && ((mask >> sj) & 1);
+    if (sk >= 0) all = all && ((mask >> sk) & 1);
+    return !all;
 }
 // true where block b leaves face (d; i, j, k) to its same-level neighbour
 __device__ __forceinline__ bool face_shared(const KPS& P, int b, int d, int i, int j, int k) {
@@ -158,25 +182,15 @@ __device__ __forceinline__ SideNb side_nb(const KPS& P, int b) {
     for (int e = 0; e < 6; e++) n.s[e] = P.reuse ? P.side[(size_t)b * 6 + e] : -1;
     return n;
 }
-// whether any flux stencil reads ghost cell (i, j, k) of a block with
-// sides n: corner cells (three coordinates outside) never; a slab cell
-// only where its side is not shared; an edge cell (two outside) only where
-// one of its two sides is not shared (the side faces computed there read it)
+// the shared sides of a block with sides n as a 6-bit mask
+__device__ __forceinline__ int side_mask(const SideNb& n) {
+    int m = 0;
+#pragma unroll
+    for (int e = 0; e < 6; e++) m |= (n.s[e] >= 0 ? 1 : 0) << e;
+    return m;
+}
 __device__ __forceinline__ bool ghost_read(const KPS& P, const SideNb& n, int i, int j, int k) {
-    const int si = i < 0 ? 0 : (i >= BX ? 1 : -1);
-    const int sj = j < 0 ? 2 : (j >= BY ? 3 : -1);
-    const int sk = k < 0 ? 4 : (k >= P.bz ? 5 : -1);
-    const int nout = (si >= 0) + (sj >= 0) + (sk >= 0);
-    if (nout == 3) return false;
-    auto shared = [&](int e) {
-        return e == 0 ? n.s[0] >= 0 : e == 1 ? n.s[1] >= 0 : e == 2 ? n.s[2] >= 0
-             : e == 3 ? n.s[3] >= 0 : e == 4 ? n.s[4] >= 0 : n.s[5] >= 0;
-    };
-    bool all = true;
-    if (si >= 0) all = all && shared(si);
-    if (sj >= 0) all = all && shared(sj);
-    if (sk >= 0) all = all && shared(sk);
-    return !all;
+    return ghost_read_mask(side_mask(n), i, j, k, P.bz);
 }
 
 __device__ __forceinline__ double dtbits_to(unsigned long long b) { return __longlong_as_double((long long)b); }
@@ -866,30 +880,27 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     phase_a_block<TB>(P, P.list[blockIdx.x], want_dt != 0);
 }
 
-__global__ __launch_bounds__(TPB) void k_ghost_cells(KPS P) {
-    const int per = (P.nghc + TPB - 1) / TPB;
+// The ghost cells a stencil reads (ghost_read) and the side faces of the
+// sides the block does not share, filled by ghost_cell and ghost_face in one
+// launch from the list of the block's sharing pattern, so that no thread is
+// spent on a cell or face the block skips (on the 3-level torus 13 % of the
+// ghost cells are read): parts workgroups per block.  The ghost cells are
+// written from interiors and the side faces from own faces, so the two
+// kinds of work are independent.
+__global__ __launch_bounds__(TPB) void k_ghost(KPS P, int buf, int parts) {
     const int w = wg_index(P);
-    const int b = P.list[w / per];
-    const int t = (w % per) * TPB + threadIdx.x;
-    if (t >= P.nghc) return;
-    int i, j, k;
-    unpack(P.ghc[t], i, j, k);
-    if (!ghost_read(P, side_nb(P, b), i, j, k)) return;
-    ghost_cell(P, b, i, j, k);
-}
-
-__global__ __launch_bounds__(TPB) void k_ghost_faces(KPS P, int buf) {
-    const int per = (P.nghf + TPB - 1) / TPB;
-    const int w = wg_index(P);
-    const int b = P.list[w / per];
-    const int t = (w % per) * TPB + threadIdx.x;
-    if (t >= P.nghf) return;
-    int i, j, k;
-    const int x = P.ghf[t];
-    unpack(x, i, j, k);
-    // a shared side face's field is read only by the flux the neighbour computes
-    if (face_shared(P, b, x >> 24, i, j, k)) return;
-    ghost_face(P, buf ? P.Bf1 : P.Bf0, b, x >> 24, i, j, k);
+    const int b = P.list[w / parts];
+    const int h = w % parts;
+    const int m = side_mask(side_nb(P, b));
+    const int o1 = P.ghp_off[m + 1];
+    double* Bf = buf ? P.Bf1 : P.Bf0;
+    for (int t = P.ghp_off[m] + h * TPB + threadIdx.x; t < o1; t += parts * TPB) {
+        const int x = P.ghp[t];
+        int i, j, k;
+        unpack(x & ((1 << 24) - 1), i, j, k);
+        if (x & (1 << 30)) ghost_face(P, Bf, b, (x >> 24) & 3, i, j, k);
+        else ghost_cell(P, b, i, j, k);
+    }
 }
 
 // fluxes with the theta and phi faces by lines (flux_pencil) and the r faces
@@ -1690,6 +1701,7 @@ public:
     DB<RunPar> runp;
     DB<RunCtr> ctr;
     DB<int> ghc, ghf, flf, owf;
+    DB<int> ghp, ghp_off;         // ghost cells and side faces to fill, per sharing pattern
     int nghc = 0, nghf = 0, nflf = 0, nowf = 0, nflp = 0;
     DB<int> flp;
     DB<int> sgrp;                 // side-face groups of flf (offset, count) per direction and side
@@ -1809,6 +1821,26 @@ public:
                 gf_.insert(gf_.end(), sides[d][e].begin(), sides[d][e].end());
             }
         sgrp.upload(sg.data(), sg.size(), stream);
+        // per sharing pattern m (bit e: side e shared with a same-level
+        // leaf): the ghost cells some stencil reads, then the side faces of
+        // the unshared sides (bit 30)
+        {
+            std::vector<int> gp, go(65, 0);
+            for (int m = 0; m < 64; m++) {
+                go[m] = (int)gp.size();
+                for (int x : gc_) {
+                    const int i = (x & 0xff) - 2, j = ((x >> 8) & 0xff) - 2, k = ((x >> 16) & 0xff) - 2;
+                    if (ghost_read_mask(m, i, j, k, L.bz)) gp.push_back(x);
+                }
+                for (int d = 0; d < 3; d++)
+                    for (int e = 0; e < 6; e++)
+                        if (!((m >> e) & 1))
+                            for (int x : sides[d][e]) gp.push_back(x | (1 << 30));
+            }
+            go[64] = (int)gp.size();
+            ghp.upload(gp.data(), gp.size(), stream);
+            ghp_off.upload(go.data(), go.size(), stream);
+        }
         own_mask.assign((size_t)3 * L.nf, 0);
         for (int x : ow_) {
             int i, j, k;
@@ -1851,6 +1883,7 @@ public:
         P.U = U.p; P.W = W.p; P.Us = Us.p; P.Bf0 = Bf0.p; P.Bf1 = Bf1.p; P.F = F.p;
         P.list = lst; P.nlist = nlst;
         P.ghc = ghc.p; P.nghc = nghc; P.ghf = ghf.p; P.nghf = nghf;
+        P.ghp = ghp.p; P.ghp_off = ghp_off.p;
         P.flf = flf.p; P.nflf = nflf; P.owf = owf.p; P.nowf = nowf;
         P.flp = flp.p; P.nflp = nflp;
         P.qctr = qctr.p;
@@ -2180,11 +2213,9 @@ public:
             if (!(mode == 3 && stage == 1))
                 tm("k_c2p", [&] { launch_c2p(var_c2p, nl, stream, P, stage == 0 ? 1 : 0); });
             const int buf = stage;
-            tm("k_ghost_cells", [&] {
-                hipLaunchKernelGGL(k_ghost_cells, dim3(nl * ((nghc + TPB - 1) / TPB)), dim3(TPB), 0, stream, P);
-            });
-            tm("k_ghost_faces", [&] {
-                hipLaunchKernelGGL(k_ghost_faces, dim3(nl * ((nghf + TPB - 1) / TPB)), dim3(TPB), 0, stream, P, buf);
+            tm("k_ghost", [&] {
+                const int parts = L.bz > 1 ? 4 : 1;
+                hipLaunchKernelGGL(k_ghost, dim3(nl * parts), dim3(TPB), 0, stream, P, buf, parts);
             });
             if (mode == 2) {
                 // the side faces (after the own faces in the face list), then
