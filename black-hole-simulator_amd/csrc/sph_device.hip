@@ -620,7 +620,7 @@ __device__ void flux_face(const KPS& P, const double* Bf, int b, int d, int i, i
 // reconstruction is carried along the line -- per variable the two previous
 // cell values and the previous limited slope -- so that each cell is loaded
 // once; the arithmetic is that of flux_face_d face by face.
-template <class Rt, int D>
+template <class Rt, int D, int RS = 0>
 __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk) {
     constexpr int d = D;
     constexpr int t1 = d == 2 ? 0 : d + 1, t2 = d == 0 ? 2 : d - 1;
@@ -666,15 +666,17 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         constexpr int nn = d == 0 ? 0 : d == 1 ? 3 : 5;
         m.gi[nn] = (Rt)tab[(M_I11 + nn) * np + p];
         m.sg = (Rt)tab[M_SG * np + p];
-        if (R.riemann) {
+        if (RS == 1 || R.riemann) {
             // the face frame of the HLLD solver also needs gamma^r phi and,
             // on theta faces, gamma^rr
             m.gi[2] = (Rt)tab[M_I13 * np + p];
             if (d == 1) m.gi[0] = (Rt)tab[M_I11 * np + p];
         }
     };
-    Met<Rt> m;
-    if (d == 2) load_m(gl(v.gc), (B.bj * BY + j0 + 2) * (v.nr + 5) + (gi + 2), m);
+    // the face metric's table and point of face f (phi faces: the cell
+    // centres, the same point for the whole line)
+    const GLB double* mtab = gl(d == 1 ? v.gf + (size_t)NMET * v.npt : v.gc);
+    auto mpt = [&](int f) { return (B.bj * BY + (d == 1 ? f : j0) + 2) * (v.nr + 5) + (gi + 2); };
     // entering face f: prev = cell f-1, cur = cell f, sp = limited slope of cell f-1
     Rt prev[NV], cur[NV], sp[NV];
     {
@@ -705,14 +707,27 @@ __device__ void flux_pencil(const KPS& P, const double* Bf, int b, int i, int jk
         PL.B[t1] = L[5]; PR.B[t1] = Rr[5];
         PL.B[t2] = L[6]; PR.B[t2] = Rr[6];
         const int fj = d == 1 ? f : j0, fk = d == 1 ? k0 : f;
-        if (d == 1) load_m(gl(v.gf + (size_t)NMET * v.npt), (B.bj * BY + fj + 2) * (v.nr + 5) + (gi + 2), m);
+        Met<Rt> m;
+        load_m(mtab, mpt(f), m);
         const int fx = fidx(d, i, fj, fk);
         const Rt bn = m.sg > Rt(0) ? (Rt)Bfd[fx] / m.sg : Rt(0);
         PL.B[d] = bn;
         PR.B[d] = bn;
         Rt out[NFW];
-        if (R.riemann) {
-            hlld(m, m.gi[2], gam, PL, PR, d, out);
+        if (RS == 1 || R.riemann) {
+            if constexpr (sizeof(Rt) == 8) {
+                // FP64: the metric read again after the solve, as in
+                // flux_face_val
+                hlld_m(m, m.gi[2], gam, PL, PR, d, out, [&](Rt& g13) {
+                    __asm__ volatile("" ::: "memory");
+                    Met<Rt> m2;
+                    load_m(mtab, mpt(f), m2);
+                    g13 = m2.gi[2];
+                    return m2;
+                });
+            } else {
+                hlld(m, m.gi[2], gam, PL, PR, d, out);
+            }
         } else {
             Side<Rt> SL, SR;
             face_side(m, gam, PL, d, SL);
@@ -905,7 +920,7 @@ __global__ __launch_bounds__(TPB) void k_ghost(KPS P, int buf, int parts) {
 
 // fluxes with the theta and phi faces by lines (flux_pencil) and the r faces
 // one per thread; P.flp lists the lines first (their threads run longest)
-template <class Rt, int TB, int MW>
+template <class Rt, int TB, int MW, int RS>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_flux_lines(KPS P, int buf) {
     const int per = (P.nflp + TB - 1) / TB;
     const int w = wg_index(P);
@@ -919,10 +934,10 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
     const double* Bf = buf ? P.Bf1 : P.Bf0;
     if (face_shared(P, b, d, i, j, k)) return;  // a line lies on one side
     if (x & (1 << 26)) {
-        if (d == 1) flux_pencil<Rt, 1>(P, Bf, b, i, k);
-        else flux_pencil<Rt, 2>(P, Bf, b, i, j);
+        if (d == 1) flux_pencil<Rt, 1, RS>(P, Bf, b, i, k);
+        else flux_pencil<Rt, 2, RS>(P, Bf, b, i, j);
     } else {
-        flux_face_d<Rt, 0>(P, Bf, b, i, j, k);
+        flux_face_d<Rt, 0, RS>(P, Bf, b, i, j, k);
     }
 }
 
@@ -1587,10 +1602,14 @@ constexpr int VAR_MW[NVAR] = {1, 3, 4, 1, 3, 4};
 constexpr int NVAR_FLUX = 2 * NVAR;
 template <class Rt>
 void launch_flux(int v, int nl, int nitems, int nlines, hipStream_t st, KPS P, int buf, bool hlld_all) {
-    if (hlld_all && v < NVAR) {
+    if (hlld_all) {
         switch (v) {
 #define L_(i, tb, mw) \
     case i: hipLaunchKernelGGL((k_flux<Rt, tb, mw, 1>), dim3(nl * ((nitems + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+            SPH_VARIANTS(L_)
+#undef L_
+#define L_(i, tb, mw) \
+    case NVAR + i: hipLaunchKernelGGL((k_flux_lines<Rt, tb, mw, 1>), dim3(nl * ((nlines + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
             SPH_VARIANTS(L_)
 #undef L_
         }
@@ -1602,7 +1621,7 @@ void launch_flux(int v, int nl, int nitems, int nlines, hipStream_t st, KPS P, i
         SPH_VARIANTS(L_)
 #undef L_
 #define L_(i, tb, mw) \
-    case NVAR + i: hipLaunchKernelGGL((k_flux_lines<Rt, tb, mw>), dim3(nl * ((nlines + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
+    case NVAR + i: hipLaunchKernelGGL((k_flux_lines<Rt, tb, mw, 0>), dim3(nl * ((nlines + tb - 1) / tb)), dim3(tb), 0, st, P, buf); break;
         SPH_VARIANTS(L_)
 #undef L_
     }
