@@ -293,6 +293,31 @@ def test_serving_results_same_alone_and_in_the_mix():
             assert np.array_equal(x, y), f"request {r.rid} differs alone and in the mix"
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_runs_with_different_solvers_share_launches(mixed):
+    """An HLL run and an HLLD run stepped together (the flux kernels that
+    follow each run's solver) give the bits of each run alone (an HLLD run
+    alone takes the kernels compiled with the HLLD solver only)."""
+    g = dict(sph.c2_grid(), nr=64, nth=32, nph=32)
+    blocks = sph.uniform_blocks(g, 16)
+    phys = [dict(sph.PHYSICS, riemann="hll"), dict(sph.PHYSICS, riemann="hlld", a=0.5)]
+    mix = sph.Engine(16, 2 * len(blocks))
+    mix.set_precision(mixed)
+    runs = [mix.add_run(p, g, blocks) for p in phys]
+    for r in runs:
+        r.init_torus(sph.TORUS, 42)
+    dts = mix.step(runs, 3)
+    for n, p in enumerate(phys):
+        e = sph.Engine(16, len(blocks))
+        e.set_precision(mixed)
+        r = e.add_run(p, g, blocks)
+        r.init_torus(sph.TORUS, 42)
+        d = e.step([r], 3)
+        assert np.array_equal(d[:, 0], dts[:, n]), p["riemann"]
+        for x, y in zip(r.state(), runs[n].state()):
+            assert np.array_equal(x, y), p["riemann"]
+
+
 def test_autotuner_choice_is_applied_and_exact(tmp_path, monkeypatch):
     from bhfluid import tune
     monkeypatch.setenv("BHFLUID_TUNE_CACHE", str(tmp_path / "tune.json"))
