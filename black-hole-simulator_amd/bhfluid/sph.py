@@ -174,8 +174,11 @@ class Engine:
 
     def set_schedule(self, schedule: int, wg_per_cu: int = 0):
         """0 (or False): one kernel per sub-phase; 1 (or True): one
-        persistent queue-driven kernel per stage; 2: sub-phase kernels with
-        the fluxes and the hydro update fused per block (slab)."""
+        persistent queue-driven kernel per stage, a block's phase B in one
+        workgroup; 2: sub-phase kernels with the fluxes and the hydro update
+        fused per block (slab); 3: schedule 0 with stage 1's con2prim inside
+        stage 0's update; 4: one launch per stage whose phases are work
+        queues over all blocks, separated by grid barriers."""
         self._check(self._L.bhs_set_schedule(self._h, int(schedule), int(wg_per_cu)), "set_schedule")
 
     def set_variants(self, flux: int = 0, update: int = 0, c2p: int = 0):

@@ -3,10 +3,11 @@
 The step's result does not depend on how it is launched (every schedule and
 variant is bit-identical), so the tuner only times.  It searches, one
 dimension at a time, the schedule (one kernel per sub-phase; one persistent
-queue-driven kernel per RK stage with 1 or 2 workgroups per compute unit; or
-the sub-phase kernels with fluxes and hydro update fused per block, "slab")
-(or the sub-phase kernels with stage 1's con2prim fused into stage 0's
-update) and, for the sub-phase schedules, the XCD numbering, the workgroup size
+queue-driven kernel per RK stage with 1 or 2 workgroups per compute unit; the
+sub-phase kernels with fluxes and hydro update fused per block, "slab"; the
+sub-phase kernels with stage 1's con2prim fused into stage 0's update; or one
+launch per stage with work-queue phases between grid barriers, with its
+occupancy hint) and, for the sub-phase schedules, the XCD numbering, the workgroup size
 and occupancy hint of the flux, update and con2prim kernels and the form of
 the face-field update (one face per thread, or each edge's EMF once in LDS
 with 2 to 16 workgroups per block); each candidate
@@ -74,12 +75,20 @@ def tune(eng: sph.Engine, runs: List[sph.Run], mixed: bool, steps: int = 4, use_
     best = dict(DEFAULT)
     best_ms = _time(eng, runs, best, steps)
     timings.append((dict(best), best_ms))
-    for c in [dict(best, schedule=1, wg_per_cu=wg) for wg in (1, 2)] + [dict(best, schedule=s) for s in (2, 3)]:
+    for c in [dict(best, schedule=1, wg_per_cu=wg) for wg in (1, 2)] + [dict(best, schedule=s) for s in (2, 3, 4)]:
         t = _time(eng, runs, c, steps)
         timings.append((c, t))
         if t < best_ms:
             best, best_ms = c, t
-    if best["schedule"] != 1:
+    if best["schedule"] == 4:
+        # the queue schedule takes its occupancy hint from the flux variant
+        for v in (1, 3, 4):
+            c = dict(best, flux=v)
+            t = _time(eng, runs, c, steps)
+            timings.append((c, t))
+            if t < best_ms:
+                best, best_ms = c, t
+    elif best["schedule"] != 1:
         c = dict(best, xcd=0)
         t = _time(eng, runs, c, steps)
         timings.append((c, t))

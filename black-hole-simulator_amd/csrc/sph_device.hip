@@ -885,7 +885,7 @@ constexpr int TPB = 256;
 __global__ void k_begin(RunCtr* ctr, const int* runs, int nruns, unsigned int* qctr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nruns) ctr[runs[t]].dtbits = 0x7ff0000000000000ull;  // +inf
-    if (t < 4) qctr[t] = 0;
+    if (t < 16) qctr[t] = 0;
 }
 
 // Launch variants the autotuner chooses from: workgroup size TB and a
@@ -902,10 +902,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) voi
 // ghost cells are read): parts workgroups per block.  The ghost cells are
 // written from interiors and the side faces from own faces, so the two
 // kinds of work are independent.
-__global__ __launch_bounds__(TPB) void k_ghost(KPS P, int buf, int parts) {
-    const int w = wg_index(P);
-    const int b = P.list[w / parts];
-    const int h = w % parts;
+__device__ void ghost_part(const KPS& P, int b, int h, int parts, int buf) {
     const int m = side_mask(side_nb(P, b));
     const int o1 = P.ghp_off[m + 1];
     double* Bf = buf ? P.Bf1 : P.Bf0;
@@ -916,6 +913,10 @@ __global__ __launch_bounds__(TPB) void k_ghost(KPS P, int buf, int parts) {
         if (x & (1 << 30)) ghost_face(P, Bf, b, (x >> 24) & 3, i, j, k);
         else ghost_cell(P, b, i, j, k);
     }
+}
+__global__ __launch_bounds__(TPB) void k_ghost(KPS P, int buf, int parts) {
+    const int w = wg_index(P);
+    ghost_part(P, P.list[w / parts], w % parts, parts, buf);
 }
 
 // fluxes with the theta and phi faces by lines (flux_pencil) and the r faces
@@ -987,12 +988,7 @@ constexpr int FE_Z = (BX + 1) * (BY + 1), FE_Y = (BX + 1) * BY, FE_X = BX * (BY 
 __host__ __device__ constexpr int face_edges(int kl) { return FE_Z * kl + (FE_Y + FE_X) * (kl + 1); }
 
 template <class Rt>
-__global__ __launch_bounds__(TPB) void k_face_edges(KPS P, int stage, int nsplit) {
-    extern __shared__ __align__(16) unsigned char face_lds[];
-    Rt* sE = (Rt*)face_lds;
-    const int w = wg_index(P);
-    const int b = P.list[w / nsplit];
-    const int h = w % nsplit;
+__device__ void face_edges_part(const KPS& P, int b, int h, int nsplit, int stage, Rt* sE) {
     const int k0 = h * P.bz / nsplit, k1 = (h + 1) * P.bz / nsplit;
     const int kl = k1 - k0;
     Rt* Ez = sE;
@@ -1065,6 +1061,12 @@ __global__ __launch_bounds__(TPB) void k_face_edges(KPS P, int stage, int nsplit
         if (stage == 0) P.Bf1[x] = P.Bf0[x] + dt * rhs;
         else P.Bf0[x] = 0.5 * (P.Bf0[x] + (P.Bf1[x] + dt * rhs));
     }
+}
+template <class Rt>
+__global__ __launch_bounds__(TPB) void k_face_edges(KPS P, int stage, int nsplit) {
+    extern __shared__ __align__(16) unsigned char face_lds[];
+    const int w = wg_index(P);
+    face_edges_part<Rt>(P, P.list[w / nsplit], w % nsplit, nsplit, stage, (Rt*)face_lds);
 }
 
 template <class Rt, int TB, int MW>
@@ -1276,6 +1278,80 @@ __global__ __launch_bounds__(TPB) void k_stage(KPS P, int stage) {
         __syncthreads();
         for (int a = threadIdx.x; a < P.nint; a += TPB) cell_update<Rt>(P, b, a, dt, stage);
         __syncthreads();
+    }
+}
+
+
+// ---------------------------------------------------- queue schedule
+// One launch per RK stage (schedule 4).  Every phase of the stage is a queue
+// of work items over all active blocks of all levels and runs -- (block,
+// part) items for con2prim, the ghost fill and the face update, (block,
+// chunk of faces or zones) items for the fluxes and the zone update --
+// that resident workgroups pull with one atomic per item; one grid barrier
+// separates consecutive phases (each phase reads what the one before wrote
+// in other blocks).  Stage 0: con2prim (with the run's CFL dt) | ghost |
+// fluxes | face update | zone update with stage 1's con2prim; stage 1:
+// ghost | fluxes | face update | zone update.  The work of every item is
+// that of the sub-phase kernels, by the same functions, so the results are
+// those of schedule 3 bit for bit.  Queue counters qctr[4..12] and the
+// barrier counters are cleared by k_begin before each stage; the grid is
+// sized from the kernel's occupancy so that every workgroup is resident,
+// and a barrier wait that runs out sets qctr[3] and every workgroup exits.
+constexpr int QPARTS = 4;  // (block, part) items per block of the con2prim and ghost phases
+template <class Rt, int RS, int MW>
+__global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_queue(KPS P, int stage, int pe) {
+    extern __shared__ __align__(16) unsigned char q_lds[];
+    __shared__ int s_item;
+    auto next = [&](int c) {
+        __syncthreads();
+        if (threadIdx.x == 0) s_item = (int)atomicAdd(&P.qctr[c], 1u);
+        __syncthreads();
+        return s_item;
+    };
+    const int nl = P.nlist;
+    const unsigned n = gridDim.x;
+    const double* Bf = stage ? P.Bf1 : P.Bf0;
+    if (stage == 0) {
+        for (int q = next(4); q < nl * QPARTS; q = next(4)) {
+            const int b = P.list[q / QPARTS], h = q % QPARTS;
+            double mn = 1e300;
+            int nfail = 0, nfl = 0;
+            for (int a = h * TPB + threadIdx.x; a < P.nint; a += QPARTS * TPB)
+                mn = fmin(mn, c2p_zone(P, b, a, true, nfail, nfl));
+            wg_commit<TPB>(P, P.rl[P.blk[b].rl].run, mn, nfail, nfl, true);
+        }
+        grid_barrier(&P.qctr[5], &P.qctr[3], n);
+        if (P.qctr[3]) return;
+    }
+    for (int q = next(6); q < nl * QPARTS; q = next(6)) ghost_part(P, P.list[q / QPARTS], q % QPARTS, QPARTS, stage);
+    grid_barrier(&P.qctr[7], &P.qctr[3], n);
+    if (P.qctr[3]) return;
+    const int nfc = (P.nflf + TPB - 1) / TPB;
+    for (int q = next(8); q < nl * nfc; q = next(8)) {
+        const int b = P.list[q / nfc];
+        const int t = (q % nfc) * TPB + threadIdx.x;
+        if (t < P.nflf) {
+            int i, j, k;
+            const int x = P.flf[t];
+            unpack(x, i, j, k);
+            if (!face_shared(P, b, x >> 24, i, j, k)) flux_face<Rt, RS>(P, Bf, b, x >> 24, i, j, k);
+        }
+    }
+    grid_barrier(&P.qctr[9], &P.qctr[3], n);
+    if (P.qctr[3]) return;
+    for (int q = next(10); q < nl * pe; q = next(10)) face_edges_part<Rt>(P, P.list[q / pe], q % pe, pe, stage, (Rt*)q_lds);
+    grid_barrier(&P.qctr[11], &P.qctr[3], n);
+    if (P.qctr[3]) return;
+    const int nuc = (P.nint + TPB - 1) / TPB;
+    for (int q = next(12); q < nl * nuc; q = next(12)) {
+        const int b = P.list[q / nuc];
+        const int a = (q % nuc) * TPB + threadIdx.x;
+        int nfail = 0, nfl = 0;
+        if (a < P.nint) {
+            cell_update<Rt>(P, b, a, run_dt(P, b), stage);
+            if (stage == 0) c2p_zone(P, b, a, false, nfail, nfl);
+        }
+        if (stage == 0) wg_commit<TPB>(P, P.rl[P.blk[b].rl].run, 1e300, nfail, nfl, false);
     }
 }
 
@@ -1777,8 +1853,8 @@ public:
         side.alloc((size_t)cap * 6);
         hside.assign((size_t)cap * 6, -1);
         for (long long s = cap - 1; s >= 0; s--) free_slots.push_back((int)s);
-        qctr.alloc(4);
-        SCHK(hipMemsetAsync(qctr.p, 0, 4 * sizeof(unsigned), stream));
+        qctr.alloc(16);
+        SCHK(hipMemsetAsync(qctr.p, 0, 16 * sizeof(unsigned), stream));
         icmax.alloc(4);
         icscale.alloc(1);
         build_item_lists();
@@ -1909,7 +1985,7 @@ public:
         P.xcd = xcd_remap ? 1 : 0;
         P.side = side.p;
         P.sgrp = sgrp.p;
-        P.reuse = mode != 1 ? 1 : 0;
+        P.reuse = mode != 1 ? 1 : 0;  // the queue schedule (4) separates its phases by grid barriers
         return P;
     }
 
@@ -2150,6 +2226,24 @@ public:
     }
 
     // ---- step
+    // grid of the queue schedule: every workgroup resident (its occupancy
+    // with the face phase's LDS), at most one per queue item of the
+    // smallest phase
+    template <class Rt>
+    int queue_grid(size_t lds, bool w3) {
+        int occ = 0;
+        if (act_hlld && w3) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_queue<Rt, 1, 3>, TPB, lds));
+        else if (act_hlld) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_queue<Rt, 1, 1>, TPB, lds));
+        else if (w3) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_queue<Rt, 0, 3>, TPB, lds));
+        else SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_queue<Rt, 0, 1>, TPB, lds));
+        int dev = 0, ncu = 0;
+        SCHK(hipGetDevice(&dev));
+        SCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        int per = occ;
+        if (wg_per_cu > 0) per = std::min(per, wg_per_cu);
+        per = std::max(per, 1);
+        return std::max(1, std::min(ncu * per, (int)act_slots.size() * 2));
+    }
     int persistent_grid() {
         int occ = 0;
         if (mixed) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<float>, TPB, 0));
@@ -2228,6 +2322,29 @@ public:
                 tm("k_stage", [&] { hipLaunchKernelGGL(k_stage<Rt>, dim3(ng), dim3(TPB), 0, stream, P, stage); });
                 continue;
             }
+            if (mode == 4) {
+                if (stage == 1)
+                    tm("k_begin", [&] {
+                        hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, stream, ctr.p, runlist.p, 0, qctr.p);
+                    });
+                const int pe = face_parts(var_face > 0 ? var_face : 2, L.bz);
+                const size_t lds = sizeof(Rt) * (size_t)face_edges((L.bz + pe - 1) / pe);
+                // the flux variant's occupancy hint: 1 (registers as the
+                // compiler needs) or 3 waves per SIMD
+                const bool w3 = VAR_MW[var_flux % NVAR] != 1;
+                const int ng = queue_grid<Rt>(lds, w3);
+                tm("k_queue", [&] {
+                    if (act_hlld && w3)
+                        hipLaunchKernelGGL((k_queue<Rt, 1, 3>), dim3(ng), dim3(TPB), lds, stream, P, stage, pe);
+                    else if (act_hlld)
+                        hipLaunchKernelGGL((k_queue<Rt, 1, 1>), dim3(ng), dim3(TPB), lds, stream, P, stage, pe);
+                    else if (w3)
+                        hipLaunchKernelGGL((k_queue<Rt, 0, 3>), dim3(ng), dim3(TPB), lds, stream, P, stage, pe);
+                    else
+                        hipLaunchKernelGGL((k_queue<Rt, 0, 1>), dim3(ng), dim3(TPB), lds, stream, P, stage, pe);
+                });
+                continue;
+            }
             // schedule 3: stage 1's con2prim ran inside stage 0's update
             if (!(mode == 3 && stage == 1))
                 tm("k_c2p", [&] { launch_c2p(var_c2p, nl, stream, P, stage == 0 ? 1 : 0); });
@@ -2298,7 +2415,7 @@ public:
         if (!pend.empty()) prof_collect();
     }
     void check_barrier() {
-        if (mode != 1) return;
+        if (mode != 1 && mode != 4) return;
         unsigned f = 0;
         SCHK(hipMemcpyAsync(&f, qctr.p + 3, sizeof f, hipMemcpyDeviceToHost, stream));
         SCHK(hipStreamSynchronize(stream));
@@ -2589,7 +2706,7 @@ int bhs_set_precision(bhs_engine* h, int mixed) {
 int bhs_set_schedule(bhs_engine* h, int persistent, int wg_per_cu) {
     return guard(h, [&] {
         if (wg_per_cu < 0 || wg_per_cu > 32) throw std::invalid_argument("wg_per_cu out of range");
-        if (persistent < 0 || persistent > 3) throw std::invalid_argument("schedule must be 0, 1, 2 or 3");
+        if (persistent < 0 || persistent > 4) throw std::invalid_argument("schedule must be 0, 1, 2, 3 or 4");
         h->e->mode = persistent;
         h->e->wg_per_cu = wg_per_cu;
         return BHS_OK;

@@ -155,8 +155,13 @@ int bhs_run_diagnostics_static(bhs_engine* e, int run, double* out);
  * compute unit (0 = as many as fit); 2 = slab: the sub-phase kernels with the
  * fluxes and the hydro update fused (a block's phi layers walked by 2^(flux
  * variant mod 5) workgroups, the fluxes kept in LDS); 3 = schedule 0 with
- * stage 1's con2prim done by stage 0's update kernel, zone by zone.  Graphs
- * on by default.  Every schedule gives the same bits. */
+ * stage 1's con2prim done by stage 0's update kernel, zone by zone; 4 = one
+ * launch per RK stage (the rkstg of integrator.cpp:191-260) whose phases --
+ * con2prim, ghost fill, fluxes, face update, zone update -- are work queues
+ * over every block of every level, pulled by resident workgroups and
+ * separated by grid barriers (wg_per_cu as for 1; the flux variant's
+ * occupancy hint applies).  Graphs on by default.  Every schedule gives the
+ * same bits. */
 int bhs_set_precision(bhs_engine* e, int mixed);
 int bhs_set_schedule(bhs_engine* e, int persistent, int wg_per_cu);
 int bhs_use_graph(bhs_engine* e, int enable);

@@ -131,10 +131,12 @@ def test_amr_three_levels_fp64():
 
 
 @pytest.mark.parametrize("schedule,mixed,bz", [(1, False, 16), (2, False, 16), (2, True, 16), (2, False, 1),
-                                               (3, False, 16), (3, True, 1)])
+                                               (3, False, 16), (3, True, 1), (4, False, 16), (4, True, 16),
+                                               (4, False, 1)])
 def test_persistent_schedule_identical(schedule, mixed, bz):
-    """The persistent (1), slab (2: fluxes and hydro update fused per block)
-    and fused-con2prim (3: stage 1's con2prim inside stage 0's update)
+    """The persistent (1), slab (2: fluxes and hydro update fused per block),
+    fused-con2prim (3: stage 1's con2prim inside stage 0's update) and queue
+    (4: one launch per stage, work-queue phases between grid barriers)
     schedules give the sub-phase schedule's state and counters bit for bit
     (3-level tree; bz = 1: the axisymmetric C1 grid)."""
     if bz == 16:
