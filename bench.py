@@ -225,6 +225,9 @@ def c2_run(sph, tune, nsteps, timed_steps):
 
 
 def serving(sph, nreq, rate, budget):
+    """The request stream at ``rate`` per second (Poisson arrivals, real
+    time), then the same requests all queued at t = 0 (a closed batch: the
+    engine's throughput when admission, not arrivals, limits the batch)."""
     from bhfluid.serve import Server, make_requests
     reqs = make_requests(nreq, rate)
     srv = Server(budget, mixed=True)
@@ -232,6 +235,16 @@ def serving(sph, nreq, rate, budget):
     s.update(rate_per_s=rate, budget_blocks=budget, steps_per_request=reqs[0].nsteps,
              finite=all(np.isfinite(r.result["mass"]) for r in reqs),
              precision="mixed (FP32 fluxes/EMFs, FP64 state)")
+    del srv
+    closed = make_requests(nreq, rate)
+    for r in closed:
+        r.arrival = 0.0
+    srv = Server(budget, mixed=True)
+    c = srv.run(closed, realtime=False)
+    c.update(finite=all(np.isfinite(r.result["mass"]) for r in closed),
+             latency_from="t = 0 (every request queued at the start)")
+    s["closed_batch"] = c
+    del srv
     return s
 
 

@@ -474,10 +474,10 @@ __device__ void wg_commit(const KPS& P, int run, double dtc, int nfail, int nfl,
 }
 
 template <int TPB>
-__device__ void phase_a_block(const KPS& P, int b, bool want_dt) {
+__device__ void phase_a_block(const KPS& P, int b, bool want_dt, int h = 0, int parts = 1) {
     double mn = 1e300;
     int nfail = 0, nfl = 0;
-    for (int a = threadIdx.x; a < P.nint; a += TPB) mn = fmin(mn, c2p_zone(P, b, a, want_dt, nfail, nfl));
+    for (int a = h * TPB + threadIdx.x; a < P.nint; a += parts * TPB) mn = fmin(mn, c2p_zone(P, b, a, want_dt, nfail, nfl));
     wg_commit<TPB>(P, P.rl[P.blk[b].rl].run, mn, nfail, nfl, want_dt);
 }
 
@@ -891,8 +891,11 @@ __global__ void k_begin(RunCtr* ctr, const int* runs, int nruns, unsigned int* q
 // Launch variants the autotuner chooses from: workgroup size TB and a
 // minimum of MW wavefronts per SIMD (MW > 1 caps the registers per thread).
 template <int TB, int MW>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_c2p(KPS P, int want_dt) {
-    phase_a_block<TB>(P, P.list[blockIdx.x], want_dt != 0);
+// parts workgroups per block, each a strided quarter (or all) of its zones:
+// the run's dt is a minimum and the counters are sums, so how the zones
+// are dealt out does not change them
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_c2p(KPS P, int want_dt, int parts) {
+    phase_a_block<TB>(P, P.list[blockIdx.x / parts], want_dt != 0, blockIdx.x % parts, parts);
 }
 
 // The ghost cells a stencil reads (ghost_read) and the side faces of the
@@ -1736,9 +1739,13 @@ void launch_face(int v, int nl, int nown, int bz, hipStream_t st, KPS P, int sta
     hipLaunchKernelGGL(k_face_edges<Rt>, dim3(nl * ns), dim3(TPB), lds, st, P, stage, ns);
 }
 inline void launch_c2p(int v, int nl, hipStream_t st, KPS P, int want_dt) {
+    // four workgroups per 16^3 block: 7 904 wavefronts of 16 zones each on
+    // the 3-level torus left the last of 2.6 rounds of resident waves half
+    // empty
+    const int parts = P.bz > 1 ? 4 : 1;
     switch (v) {
 #define L_(i, tb, mw) \
-    case i: hipLaunchKernelGGL((k_c2p<tb, mw>), dim3(nl), dim3(tb), 0, st, P, want_dt); break;
+    case i: hipLaunchKernelGGL((k_c2p<tb, mw>), dim3(nl * parts), dim3(tb), 0, st, P, want_dt, parts); break;
         SPH_VARIANTS(L_)
 #undef L_
     }
