@@ -895,7 +895,8 @@ template <int TB, int MW>
 // the run's dt is a minimum and the counters are sums, so how the zones
 // are dealt out does not change them
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_c2p(KPS P, int want_dt, int parts) {
-    phase_a_block<TB>(P, P.list[blockIdx.x / parts], want_dt != 0, blockIdx.x % parts, parts);
+    const int w = wg_index(P);  // a block's parts on one XCD: they share its metric rows in L2
+    phase_a_block<TB>(P, P.list[w / parts], want_dt != 0, w % parts, parts);
 }
 
 // The ghost cells a stencil reads (ghost_read) and the side faces of the
