@@ -100,6 +100,7 @@ def lib():
         sig("bhs_variant_info", _i, [_i, _ip, _ip])
         sig("bhs_set_xcd_remap", _i, [_v, _i])
         sig("bhs_set_face_variant", _i, [_v, _i])
+        sig("bhs_set_barrier_spin_limit", _i, [_v, ctypes.c_uint])
         sig("bhs_run_diagnostics_static", _i, [_v, _i, _dp])
         sig("bhs_profile", _i, [_v, _i, ctypes.c_char_p])
         sig("bhs_profile_report", _i, [_v, ctypes.c_char_p, _i])
@@ -186,6 +187,11 @@ class Engine:
 
     def set_face_variant(self, face: int = 2):
         self._check(self._L.bhs_set_face_variant(self._h, int(face)), "set_face_variant")
+
+    def set_barrier_spin_limit(self, polls: int = 0):
+        """Polls of a grid-barrier wait (schedules 1 and 4); 0 restores the
+        default.  A small value makes the barriers time out (error path)."""
+        self._check(self._L.bhs_set_barrier_spin_limit(self._h, int(polls)), "set_barrier_spin_limit")
 
     def set_xcd_remap(self, enable: bool = True):
         self._check(self._L.bhs_set_xcd_remap(self._h, int(bool(enable))), "set_xcd_remap")

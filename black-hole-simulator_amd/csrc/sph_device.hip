@@ -111,7 +111,9 @@ struct KPS {
     const int* flp;       // theta / phi face lines (bit 26 set), then r faces
     int nflp;
     // persistent schedule
-    unsigned int* qctr;   // [0] stage-A queue, [1] barrier arrivals, [2] stage-B queue, [3] timeout flag
+    unsigned int* qctr;   // [0] stage-A queue, [1] barrier arrivals, [2] stage-B queue, [4..12] queue schedule,
+                          // [15] barrier timeout flag
+    unsigned int spin;    // polls of a grid-barrier wait before it gives up
     int xcd;              // 1: renumber workgroups so that each XCD takes a contiguous range
     // Side faces: a block computes fluxes on the faces one layer outside it
     // (for the EMFs of its boundary edges).  The flux of a face depends only
@@ -885,7 +887,7 @@ constexpr int TPB = 256;
 __global__ void k_begin(RunCtr* ctr, const int* runs, int nruns, unsigned int* qctr) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t < nruns) ctr[runs[t]].dtbits = 0x7ff0000000000000ull;  // +inf
-    if (t < 16) qctr[t] = 0;
+    if (t < 15) qctr[t] = 0;  // not qctr[15], the barrier timeout flag
 }
 
 // Launch variants the autotuner chooses from: workgroup size TB and a
@@ -1209,11 +1211,15 @@ __global__ __launch_bounds__(TPB) void k_cellb(KPS P, int stage) {
 // meet at one grid barrier (every workgroup of the launch is resident: the
 // grid is sized from the occupancy of this kernel), then pull blocks again
 // for phase B and run its four sub-phases with workgroup barriers between
-// them.  The barrier wait is bounded; a launch that is not co-resident sets
-// the timeout flag and every workgroup still exits.
+// them.  The barrier wait is bounded (P.spin polls); a launch that is not
+// co-resident sets the timeout flag qctr[QFLAG] and every workgroup still
+// exits.  The flag is not cleared by k_begin (so a timeout in stage 0 is
+// still seen after stage 1), only by the host once it has read it.
 constexpr unsigned SPIN_LIMIT = 1u << 26;
+constexpr int QFLAG = 15;
 
-__device__ __forceinline__ void grid_barrier(unsigned int* arrive, unsigned int* flag, unsigned int n) {
+__device__ __forceinline__ void grid_barrier(unsigned int* arrive, unsigned int* flag, unsigned int n,
+                                             unsigned int limit) {
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -1221,7 +1227,7 @@ __device__ __forceinline__ void grid_barrier(unsigned int* arrive, unsigned int*
         unsigned spins = 0;
         while (__hip_atomic_load(arrive, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < n) {
             __builtin_amdgcn_s_sleep(2);
-            if (++spins > SPIN_LIMIT) {
+            if (++spins > limit) {
                 atomicExch(flag, 1u);
                 break;
             }
@@ -1243,8 +1249,8 @@ __global__ __launch_bounds__(TPB) void k_stage(KPS P, int stage) {
         if (q >= P.nlist) break;
         phase_a_block<TPB>(P, P.list[q], want_dt);
     }
-    grid_barrier(&P.qctr[1], &P.qctr[3], gridDim.x);
-    if (P.qctr[3]) return;
+    grid_barrier(&P.qctr[1], &P.qctr[QFLAG], gridDim.x, P.spin);
+    if (P.qctr[QFLAG]) return;
     double* Bc = stage == 0 ? P.Bf0 : P.Bf1;
     for (;;) {
         if (threadIdx.x == 0) s_b = (int)atomicAdd(&P.qctr[2], 1u);
@@ -1300,7 +1306,7 @@ __global__ __launch_bounds__(TPB) void k_stage(KPS P, int stage) {
 // those of schedule 3 bit for bit.  Queue counters qctr[4..12] and the
 // barrier counters are cleared by k_begin before each stage; the grid is
 // sized from the kernel's occupancy so that every workgroup is resident,
-// and a barrier wait that runs out sets qctr[3] and every workgroup exits.
+// and a barrier wait that runs out sets qctr[QFLAG] and every workgroup exits.
 constexpr int QPARTS = 4;  // (block, part) items per block of the con2prim and ghost phases
 template <class Rt, int RS, int MW>
 __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(MW, 8))) void k_queue(KPS P, int stage, int pe) {
@@ -1324,12 +1330,12 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(MW, 8))) vo
                 mn = fmin(mn, c2p_zone(P, b, a, true, nfail, nfl));
             wg_commit<TPB>(P, P.rl[P.blk[b].rl].run, mn, nfail, nfl, true);
         }
-        grid_barrier(&P.qctr[5], &P.qctr[3], n);
-        if (P.qctr[3]) return;
+        grid_barrier(&P.qctr[5], &P.qctr[QFLAG], n, P.spin);
+        if (P.qctr[QFLAG]) return;
     }
     for (int q = next(6); q < nl * QPARTS; q = next(6)) ghost_part(P, P.list[q / QPARTS], q % QPARTS, QPARTS, stage);
-    grid_barrier(&P.qctr[7], &P.qctr[3], n);
-    if (P.qctr[3]) return;
+    grid_barrier(&P.qctr[7], &P.qctr[QFLAG], n, P.spin);
+    if (P.qctr[QFLAG]) return;
     const int nfc = (P.nflf + TPB - 1) / TPB;
     for (int q = next(8); q < nl * nfc; q = next(8)) {
         const int b = P.list[q / nfc];
@@ -1341,11 +1347,11 @@ __global__ __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(MW, 8))) vo
             if (!face_shared(P, b, x >> 24, i, j, k)) flux_face<Rt, RS>(P, Bf, b, x >> 24, i, j, k);
         }
     }
-    grid_barrier(&P.qctr[9], &P.qctr[3], n);
-    if (P.qctr[3]) return;
+    grid_barrier(&P.qctr[9], &P.qctr[QFLAG], n, P.spin);
+    if (P.qctr[QFLAG]) return;
     for (int q = next(10); q < nl * pe; q = next(10)) face_edges_part<Rt>(P, P.list[q / pe], q % pe, pe, stage, (Rt*)q_lds);
-    grid_barrier(&P.qctr[11], &P.qctr[3], n);
-    if (P.qctr[3]) return;
+    grid_barrier(&P.qctr[11], &P.qctr[QFLAG], n, P.spin);
+    if (P.qctr[QFLAG]) return;
     const int nuc = (P.nint + TPB - 1) / TPB;
     for (int q = next(12); q < nl * nuc; q = next(12)) {
         const int b = P.list[q / nuc];
@@ -1826,6 +1832,7 @@ public:
     int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update), 3 multi-kernel with
                                   // stage 1's con2prim fused into stage 0's update
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
+    unsigned spin_limit = SPIN_LIMIT;  // polls of a grid-barrier wait (schedules 1 and 4)
     int var_flux = 0, var_update = 0, var_c2p = 1;  // launch variants (VAR_TB, VAR_MW)
     int var_face = 2;             // face-update variant (launch_face)
     bool xcd_remap = true;        // XCD-contiguous workgroup numbering
@@ -1982,6 +1989,7 @@ public:
     KPS kp(const int* lst, int nlst) {
         KPS P;
         P.bz = L.bz; P.sz = L.sz; P.ncell = L.ncell; P.nint = L.nint; P.nf = L.nf; P.nff = L.nff;
+        P.spin = spin_limit;
         P.blk = blk.p; P.rl = rl.p; P.run = runp.p; P.ctr = ctr.p;
         P.U = U.p; P.W = W.p; P.Us = Us.p; P.Bf0 = Bf0.p; P.Bf1 = Bf1.p; P.F = F.p;
         P.list = lst; P.nlist = nlst;
@@ -2399,7 +2407,8 @@ public:
     long long key() const {
         return (long long)(mixed ? 1 : 0) | ((long long)mode << 2) | ((long long)wg_per_cu << 5) |
                ((long long)var_flux << 13) | ((long long)var_update << 19) | ((long long)var_c2p << 25) |
-               ((long long)(xcd_remap ? 1 : 0) << 31) | ((long long)var_face << 32);
+               ((long long)(xcd_remap ? 1 : 0) << 31) | ((long long)var_face << 32) |
+               ((long long)(spin_limit != SPIN_LIMIT ? 1 : 0) << 36);
     }
     void step() {
         if (act_slots.empty()) throw std::invalid_argument("no active runs");
@@ -2425,9 +2434,15 @@ public:
     void check_barrier() {
         if (mode != 1 && mode != 4) return;
         unsigned f = 0;
-        SCHK(hipMemcpyAsync(&f, qctr.p + 3, sizeof f, hipMemcpyDeviceToHost, stream));
+        SCHK(hipMemcpyAsync(&f, qctr.p + QFLAG, sizeof f, hipMemcpyDeviceToHost, stream));
         SCHK(hipStreamSynchronize(stream));
-        if (f) throw std::runtime_error("persistent stage: grid barrier timed out (workgroups not co-resident)");
+        if (f) {
+            // cleared here (never inside a captured step), so the next step
+            // starts without it
+            SCHK(hipMemsetAsync(qctr.p + QFLAG, 0, sizeof(unsigned), stream));
+            SCHK(hipStreamSynchronize(stream));
+            throw std::runtime_error("persistent stage: grid barrier timed out (workgroups not co-resident)");
+        }
     }
     void read_ctr(std::vector<RunCtr>& out) {
         out.resize(runs.size());
@@ -2735,6 +2750,13 @@ int bhs_set_face_variant(bhs_engine* h, int face) {
     return guard(h, [&] {
         if (face < 0 || face >= NVAR_FACE) throw std::invalid_argument("face variant out of range");
         h->e->var_face = face;
+        return BHS_OK;
+    });
+}
+int bhs_set_barrier_spin_limit(bhs_engine* h, unsigned int polls) {
+    return guard(h, [&] {
+        h->e->spin_limit = polls ? polls : SPIN_LIMIT;
+        h->e->graph_release();
         return BHS_OK;
     });
 }

@@ -184,6 +184,12 @@ int bhs_set_face_variant(bhs_engine* e, int face);
  * eight XCDs (own L2) takes a contiguous range of blocks (1, the default) or
  * in dispatch order (0). */
 int bhs_set_xcd_remap(bhs_engine* e, int enable);
+/* Polls of a grid-barrier wait in schedules 1 and 4 before the barrier gives
+ * up (0: the default, 2^26).  A launch whose workgroups are not all resident
+ * then ends with every workgroup exiting and the next bhs_steps returning
+ * BHS_EDEVICE with "grid barrier timed out"; a small value forces that path
+ * (for tests of the error handling).  No reference counterpart. */
+int bhs_set_barrier_spin_limit(bhs_engine* e, unsigned int polls);
 int bhs_sync(bhs_engine* e);
 
 #ifdef __cplusplus

@@ -164,6 +164,33 @@ def test_persistent_schedule_identical(schedule, mixed, bz):
     assert (xa["c2p_fail"], xa["floors"]) == (xb["c2p_fail"], xb["floors"])
 
 
+@pytest.mark.parametrize("schedule", [1, 4])
+def test_grid_barrier_timeout_is_reported(schedule):
+    """The persistent schedules' grid barriers wait a bounded number of polls.
+    With the bound forced down to one poll a barrier gives up: every
+    workgroup still exits, the step reports the timeout (it is not lost when
+    the next stage's launch clears the queue counters), and with the default
+    bound restored the engine steps as an engine that never timed out."""
+    from bhfluid import BhError
+    g = dict(nr=48, nth=32, nph=32, nlevels=3, r_in=sph.R_IN, r_out=sph.R_OUT)
+    blocks = sph.amr_blocks(g, 3)
+    eng = sph.Engine(16, len(blocks))
+    eng.set_schedule(schedule)
+    run = eng.add_run(sph.PHYSICS, g, blocks)
+    run.init_torus(sph.TORUS, 42)
+    eng.set_barrier_spin_limit(1)
+    with pytest.raises(BhError, match="grid barrier timed out"):
+        eng.step([run], 1)
+    eng.set_barrier_spin_limit(0)
+    run.init_torus(sph.TORUS, 42)
+    ref = sph.Engine(16, len(blocks))
+    rr = ref.add_run(sph.PHYSICS, g, blocks)
+    rr.init_torus(sph.TORUS, 42)
+    assert np.array_equal(eng.step([run], 2), ref.step([rr], 2))
+    for x, y in zip(run.state(), rr.state()):
+        assert np.array_equal(x, y)
+
+
 def test_graph_replays_identical_to_eager():
     g = dict(sph.c2_grid(), nr=64, nth=32, nph=32)
     blocks = sph.uniform_blocks(g, 16)
