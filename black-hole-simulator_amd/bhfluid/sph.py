@@ -300,7 +300,7 @@ class Run:
 
 # ----------------------------------------------------------------- presets
 
-# BASELINE physics of the torus runs: a = 0.9375, Gamma = 13/9, CFL 0.4, the
+# Physics of the torus runs: a = 0.9375, Gamma = 13/9, CFL 0.4, the
 # reference's floor constants (src/fluid/grid.hpp:140-143)
 PHYSICS = dict(a=0.9375, Q=0.0, gamma=13.0 / 9.0, cfl=0.4, rho_floor=1e-7, eps_floor=1e-5, sigma_max=100.0,
                gamma_max=50.0, riemann="hlld")
@@ -411,17 +411,17 @@ def amr_blocks(grid: dict, nlevels: int, physics=PHYSICS, torus=TORUS, thresh: f
 
 
 def c1_grid():
-    """2D axisymmetric, 128 (r, log 1.2-40) x 128 (theta)."""
+    """The 2D axisymmetric torus grid (C1): 128 (r, log 1.2-40) x 128 (theta)."""
     return dict(nr=128, nth=128, nph=1, nlevels=1, r_in=R_IN, r_out=R_OUT)
 
 
 def c2_grid():
-    """3D 256 (r) x 128 (theta) x 128 (phi), uniform."""
+    """The uniform 3D torus grid (C2): 256 (r) x 128 (theta) x 128 (phi)."""
     return dict(nr=256, nth=128, nph=128, nlevels=1, r_in=R_IN, r_out=R_OUT)
 
 
 def c3_grid():
-    """3 levels of 16^3 blocks refined where rho > 0.1
+    """The 3-level AMR torus grid (C3): 16^3 blocks refined where rho > 0.1
     rho_max; the level-0 grid is 128 x 64 x 64 zones (1976 blocks: 56, 384
     and 1536 per level, 8.09 M zones)."""
     return dict(nr=128, nth=64, nph=64, nlevels=3, r_in=R_IN, r_out=R_OUT)

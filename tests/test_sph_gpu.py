@@ -218,6 +218,19 @@ def test_c2_full_size_ten_steps_fp64():
     _steps(eng, run, orc, 10, False, "C2 full")
 
 
+@pytest.mark.parametrize("mixed", [False, True])
+def test_c3_full_size_three_steps(mixed):
+    """The benchmark workload itself: the 3-level AMR torus (1976 blocks of
+    16^3, 8.09 M zones; 56, 384 and 1536 blocks per level), 3 steps in the
+    launch configuration the engine uses by default, against the oracle
+    (FP64, and the mixed-precision step against the oracle's mixed mode)."""
+    g = sph.c3_grid()
+    blocks = sph.amr_blocks(g, 3)
+    assert len(blocks) == 1976
+    eng, run, orc, _, _ = make_pair(g, 16, blocks, mixed=mixed)
+    _steps(eng, run, orc, 3, mixed, "C3 mixed" if mixed else "C3")
+
+
 def test_c2_full_size_stays_finite_div_b_round_off():
     """C2 at its full size (256 x 128 x 128, 1024 blocks), 10 steps FP64:
     finite, max |div B| at round-off of the stencil scale, mass and energy
