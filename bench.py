@@ -17,9 +17,10 @@ HBM when timing starts):
   1. autotuning of the launch configuration (bhfluid.tune), per precision;
   2. timed: W warm-up steps, then K steps replayed from the captured HIP
      graph with an event between consecutive steps; value = zones / median
-     step time (FP64; the mixed-precision figure beside it);
-  3. accuracy: the same steps_acc steps in FP64 and in mixed precision from
-     the same initial data; L1(rho), mass and energy drift, max |div B|;
+     step time (FP64; the mixed-precision figures beside it);
+  3. accuracy: the same steps_acc steps in FP64, in mixed precision and in
+     mixed precision with hardware FP32 reciprocal and square root ("fast")
+     from the same initial data; L1(rho), mass and energy drift, max |div B|;
   4. per-kernel HIP-event breakdown (eager launches) and the dominant
      kernel's roofline from its launches timed alone;
   5. the uniform 3D torus (256 x 128 x 128), 200 FP64 steps, the last 20
@@ -112,11 +113,14 @@ def timed(sph, tune, g, blocks, mixed, steps, warmup, tune_log):
     return eng, run, ms, t["config"]
 
 
-def accuracy(sph, g, blocks, nsteps, cfg64, cfgmx):
+def accuracy(sph, g, blocks, nsteps, cfg64, cfgmx, cfgfast):
+    """FP64, mixed and mixed-fast runs of nsteps from the same initial data;
+    the mixed runs' L1(rho), drifts and div B against the FP64 run (the
+    mixed-fast figures under "fast")."""
     from bhfluid import tune
     out = {}
     states = {}
-    for name, mixed, cfg in (("fp64", False, cfg64), ("mixed", True, cfgmx)):
+    for name, mixed, cfg in (("fp64", False, cfg64), ("mixed", True, cfgmx), ("fast", 2, cfgfast)):
         eng, run = make_engine(sph, g, blocks, mixed)
         tune.apply(eng, cfg)
         d0 = run.diagnostics()
@@ -142,6 +146,12 @@ def accuracy(sph, g, blocks, nsteps, cfg64, cfgmx):
                divb_rel_mixed=m["d1"]["divb_max"] / m["d1"]["divb_scale"],
                t_fp64=f["t"], t_mixed=m["t"], finite=f["finite"] and m["finite"],
                c2p_failures_fp64=f["d1"]["c2p_fail"], c2p_failures_mixed=m["d1"]["c2p_fail"])
+    q = states["fast"]
+    out["fast"] = dict(L1_rho=float((np.abs(q["rho"] - b) * w).sum() / (np.abs(b) * w).sum()),
+                       mass_drift=(q["d1"]["mass"] - f["d1"]["mass"]) / f["d1"]["mass"],
+                       energy_drift=(q["d1"]["energy"] - f["d1"]["energy"]) / f["d1"]["energy"],
+                       divb_rel=q["d1"]["divb_max"] / q["d1"]["divb_scale"], t=q["t"], finite=q["finite"],
+                       c2p_failures=q["d1"]["c2p_fail"])
     return out
 
 
@@ -261,7 +271,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--c2-steps", type=int, default=200,
                     help="steps of the uniform 3D torus (256 x 128 x 128) run; 0 skips it")
-    ap.add_argument("--kernels-only", choices=("fp64", "mixed", "ref"),
+    ap.add_argument("--kernels-only", choices=("fp64", "mixed", "fast", "ref"),
                     help="for counter collection: C3 initial data, then --steps eager steps of the step's kernels "
                          "in the given precision with the default launch configuration; or (ref) the reference-path "
                          "engine on the reference's own production tree (main.cpp:61-168: 8^3 root patches on "
@@ -302,7 +312,7 @@ def main():
         return
     if args.kernels_only:
         g, blocks = c3_setup(sph)
-        eng, run = make_engine(sph, g, blocks, args.kernels_only == "mixed")
+        eng, run = make_engine(sph, g, blocks, {"fp64": 0, "mixed": 1, "fast": 2}[args.kernels_only])
         launch = dict(tune.DEFAULT, **json.loads(args.launch)) if args.launch else dict(tune.DEFAULT)
         tune.apply(eng, launch)
         eng.use_graph(False)
@@ -322,11 +332,16 @@ def main():
     del eng, run
     eng, run, msmx, cfgmx = timed(sph, tune, g, blocks, True, args.steps, args.warmup, tune_log["mixed"])
     del eng, run
+    eng, run, msfa, cfgfa = timed(sph, tune, g, blocks, 2, args.steps, args.warmup, [])
+    del eng, run
     med64, medmx = float(np.median(ms64)), float(np.median(msmx))
     value = zones / (med64 * 1e-3)
     value_mx = zones / (medmx * 1e-3)
 
-    acc = accuracy(sph, g, blocks, args.accuracy_steps, cfg64, cfgmx)
+    medfa = float(np.median(msfa))
+    value_fa = zones / (medfa * 1e-3)
+    acc = accuracy(sph, g, blocks, args.accuracy_steps, cfg64, cfgmx, cfgfa)
+    acc_fast = acc.pop("fast")
 
     rep64, dom64, alone64 = kernel_breakdown(sph, g, blocks, False, cfg64, 4)
     repmx, dommx, alonemx = kernel_breakdown(sph, g, blocks, True, cfgmx, 4)
@@ -386,6 +401,10 @@ def main():
         "mixed": {"value": value_mx, "ms_per_step": medmx, "speedup_vs_fp64": value_mx / value,
                   "dtype": "f32 reconstruction/HLLD/EMF, f64 con2prim/sources/update/reductions",
                   "dominant_kernel": dommx, "dominant_avg_launch_ms": alonemx["avg_ms"], **acc},
+        "mixed_fast": {"value": value_fa, "ms_per_step": medfa, "speedup_vs_fp64": value_fa / value,
+                       "dtype": "as mixed, with f32 division and sqrt from the hardware reciprocal and "
+                                "square-root instructions (a few ulp; not bit-identical to a CPU)",
+                       "launch": cfgfa, "steps": args.accuracy_steps, **acc_fast},
         "roofline": roofline,
         "step_roofline": step_roof,
         "cpu_baseline": cpu,

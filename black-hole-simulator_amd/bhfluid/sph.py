@@ -170,8 +170,13 @@ class Engine:
         self.runs.pop(run.id, None)
         run.id = -1
 
-    def set_precision(self, mixed: bool):
-        self._check(self._L.bhs_set_precision(self._h, int(bool(mixed))), "set_precision")
+    def set_precision(self, mixed):
+        """False/0: FP64; True/1: mixed (FP32 reconstruction, Riemann fluxes
+        and EMFs, rounded as the CPU rounds them); 2: mixed with the FP32
+        divisions and square roots of the hardware reciprocal and square-root
+        instructions (faster; judged by its accuracy against FP64)."""
+        mode = 2 if mixed == 2 else int(bool(mixed))
+        self._check(self._L.bhs_set_precision(self._h, mode), "set_precision")
 
     def set_schedule(self, schedule: int, wg_per_cu: int = 0):
         """0 (or False): one kernel per sub-phase; 1 (or True): one

@@ -37,7 +37,8 @@ def _cache_path() -> Path:
 def _key(eng: sph.Engine, runs, mixed: bool) -> str:
     from . import device_info
     nb = sum(r.nb for r in runs)
-    return f"{device_info().split(';')[0]}|bz{eng.bz}|nb{nb}|{'mixed' if mixed else 'fp64'}"
+    prec = "mixed_fast" if mixed == 2 else "mixed" if mixed else "fp64"
+    return f"{device_info().split(';')[0]}|bz{eng.bz}|nb{nb}|{prec}"
 
 
 def apply(eng: sph.Engine, cfg: dict):

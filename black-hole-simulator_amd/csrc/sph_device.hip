@@ -43,6 +43,7 @@
 #include "sph_geom.hpp"
 #include "sph_physics.hpp"
 #include "sph_torus.hpp"
+#include "sph_ffloat.hpp"
 
 namespace sph {
 
@@ -1828,7 +1829,8 @@ public:
     std::vector<int> act_runs;    // runs of the current active list
     bool act_hlld = false;        // every active run uses the HLLD solver
     std::vector<int> act_slots;
-    bool mixed = false;
+    bool mixed = false;           // FP32 fluxes and EMFs
+    bool fastf = false;           // with mixed: FP32 with hardware reciprocal and square root (ffloat)
     int mode = 0;                 // 0 multi-kernel, 1 persistent, 2 slab (fused flux + update), 3 multi-kernel with
                                   // stage 1's con2prim fused into stage 0's update
     int wg_per_cu = 0;            // persistent grid: 0 = occupancy
@@ -2262,7 +2264,8 @@ public:
     }
     int persistent_grid() {
         int occ = 0;
-        if (mixed) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<float>, TPB, 0));
+        if (mixed && fastf) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<ffloat>, TPB, 0));
+        else if (mixed) SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<float>, TPB, 0));
         else SCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_stage<double>, TPB, 0));
         int dev = 0, ncu = 0;
         SCHK(hipGetDevice(&dev));
@@ -2401,11 +2404,13 @@ public:
         SCHK(hipGetLastError());
     }
     void enqueue() {
-        if (mixed) enqueue_step<float>();
+        if (mixed && fastf) enqueue_step<ffloat>();
+        else if (mixed) enqueue_step<float>();
         else enqueue_step<double>();
     }
     long long key() const {
-        return (long long)(mixed ? 1 : 0) | ((long long)mode << 2) | ((long long)wg_per_cu << 5) |
+        return (long long)(mixed ? 1 : 0) | ((long long)(fastf ? 1 : 0) << 1) | ((long long)mode << 2) |
+               ((long long)wg_per_cu << 5) |
                ((long long)var_flux << 13) | ((long long)var_update << 19) | ((long long)var_c2p << 25) |
                ((long long)(xcd_remap ? 1 : 0) << 31) | ((long long)var_face << 32) |
                ((long long)(spin_limit != SPIN_LIMIT ? 1 : 0) << 36);
@@ -2722,7 +2727,9 @@ int bhs_run_set_state(bhs_engine* h, int run, const double* U, const double* W, 
 }
 int bhs_set_precision(bhs_engine* h, int mixed) {
     return guard(h, [&] {
+        if (mixed < 0 || mixed > 2) throw std::invalid_argument("precision must be 0 (FP64), 1 (mixed) or 2 (mixed, fast FP32)");
         h->e->mixed = mixed != 0;
+        h->e->fastf = mixed == 2;
         return BHS_OK;
     });
 }

@@ -150,7 +150,10 @@ int bhs_run_diagnostics_static(bhs_engine* e, int run, double* out);
 
 /* Launch configuration (no reference counterpart): mixed 1 = FP32
  * reconstruction, Riemann fluxes and EMFs (con2prim, sources, update and
- * reductions FP64).  Schedule ("persistent"): 0 = one kernel per sub-phase;
+ * reductions FP64), rounded as the CPU rounds them; 2 = the same with FP32
+ * division and square root from the hardware reciprocal and square-root
+ * instructions (within a few ulp; faster, not bit-identical to a CPU).
+ * Schedule ("persistent"): 0 = one kernel per sub-phase;
  * 1 = one queue-driven kernel per RK stage with wg_per_cu workgroups per
  * compute unit (0 = as many as fit); 2 = slab: the sub-phase kernels with the
  * fluxes and the hydro update fused (a block's phi layers walked by 2^(flux
