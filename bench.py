@@ -240,16 +240,16 @@ def serving(sph, nreq, rate, budget):
     engine's throughput when admission, not arrivals, limits the batch)."""
     from bhfluid.serve import Server, make_requests
     reqs = make_requests(nreq, rate)
-    srv = Server(budget, mixed=True)
+    srv = Server(budget, mixed=2)
     s = srv.run(reqs)
     s.update(rate_per_s=rate, budget_blocks=budget, steps_per_request=reqs[0].nsteps,
              finite=all(np.isfinite(r.result["mass"]) for r in reqs),
-             precision="mixed (FP32 fluxes/EMFs, FP64 state)")
+             precision="mixed-fast (FP32 fluxes/EMFs with hardware reciprocal and sqrt, FP64 state)")
     del srv
     closed = make_requests(nreq, rate)
     for r in closed:
         r.arrival = 0.0
-    srv = Server(budget, mixed=True)
+    srv = Server(budget, mixed=2)
     c = srv.run(closed, realtime=False)
     c.update(finite=all(np.isfinite(r.result["mass"]) for r in closed),
              latency_from="t = 0 (every request queued at the start)")

@@ -83,9 +83,10 @@ def request_blocks(req: Request) -> np.ndarray:
 
 class Server:
     """Continuous batching over one ``sph.Engine`` of ``budget_blocks``
-    block slots (bz = 16)."""
+    block slots (bz = 16); ``mixed`` is the engine's precision (False FP64,
+    True mixed, 2 mixed with the hardware FP32 reciprocal and square root)."""
 
-    def __init__(self, budget_blocks: int, mixed: bool = True, schedule: int = 0,
+    def __init__(self, budget_blocks: int, mixed=True, schedule: int = 0,
                  clock: Callable[[], float] = time.perf_counter, keep_state: bool = False):
         self.eng = sph.Engine(16, budget_blocks)
         self.eng.set_precision(mixed)

@@ -338,7 +338,8 @@ def test_face_variants_identical(mixed, bz):
                 assert np.array_equal(x, y), face
 
 
-def test_serving_results_same_alone_and_in_the_mix():
+@pytest.mark.parametrize("mixed", [True, 2])
+def test_serving_results_same_alone_and_in_the_mix(mixed):
     """Continuous batching: requests of different grids, spins and fields
     arrive while others run, are admitted as block slots free up (a budget
     smaller than their sum), advance in shared launches, and leave when done.
@@ -354,14 +355,14 @@ def test_serving_results_same_alone_and_in_the_mix():
     from bhfluid.serve import request_blocks
     nbs = [len(request_blocks(r)) for r in reqs]
     budget = max(nbs) + min(nbs)          # never all at once: arrivals wait for slots
-    srv = Server(budget, mixed=True, keep_state=True)
+    srv = Server(budget, mixed=mixed, keep_state=True)
     summary = srv.run(reqs, realtime=False)
     print("served", summary, "blocks", nbs)
     assert summary["max_batch_runs"] >= 2
     batch_sizes = {x[1] for x in srv.log}
     assert len(batch_sizes) >= 2            # the mix changed while running
     for r in reqs:
-        alone = Server(nbs[r.rid], mixed=True, keep_state=True)
+        alone = Server(nbs[r.rid], mixed=mixed, keep_state=True)
         solo = Request(r.rid, 0.0, r.grid, r.physics, r.torus, r.seed, r.nsteps)
         alone.run([solo], realtime=False)
         assert solo.result["t"] == r.result["t"]
