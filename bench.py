@@ -88,6 +88,22 @@ def measured_traffic(kernel, precision):
     return None
 
 
+def measured_step_traffic_c2():
+    """HBM GB per FP64 step of the uniform 3D torus from the committed
+    counter summary (profiles/kernels_traffic_c2_uniform_torus.md), or None."""
+    f = PROFILES / "kernels_traffic_c2_uniform_torus.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        cols = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cols) >= 5 and cols[0] == "step" and cols[1] == "fp64":
+            try:
+                return float(cols[4])
+            except ValueError:
+                return None
+    return None
+
+
 def measured_flops(precision):
     f = PROFILES / "flops_c3_amr_torus.md"
     if not f.exists():
@@ -230,6 +246,12 @@ def c2_run(sph, tune, nsteps, timed_steps):
            "unit": "zone-updates/s", "hbm_GBps_296B": value * BYTES_PER_ZONE_UPDATE / 1e9,
            "hbm_frac_296B": value * BYTES_PER_ZONE_UPDATE / 1e9 / HBM_PEAK_GBS, "launch": cfg, "finite": finite,
            "divb_rel": d1["divb_max"] / d1["divb_scale"], "mass_change": (d1["mass"] - d0["mass"]) / d0["mass"]}
+    gb = measured_step_traffic_c2()
+    if gb:
+        out.update(hbm_GB_per_step_rocprof=gb, hbm_GBps_rocprof=gb / (med * 1e-3),
+                   hbm_frac_rocprof=gb / (med * 1e-3) / HBM_PEAK_GBS,
+                   hbm_source="rocprofv3 FETCH_SIZE + WRITE_SIZE of every step kernel, copy-calibrated "
+                              "(profiles/kernels_traffic_c2_uniform_torus.md)")
     del eng, run
     return out
 
@@ -277,6 +299,9 @@ def main():
                          "engine on the reference's own production tree (main.cpp:61-168: 8^3 root patches on "
                          "[-50, 50]^3, FM torus a = 0.9, Q = 0.1, Gamma = 5/3, one regrid), initial data and "
                          "regrid on the device, then --steps steps; prints one JSON line")
+    ap.add_argument("--workload", choices=("c3", "c2"), default="c3",
+                    help="with --kernels-only fp64/mixed/fast: the 3-level AMR torus (c3) or the uniform 3D torus "
+                         "256 x 128 x 128 (c2)")
     ap.add_argument("--launch", default=None,
                     help="with --kernels-only: the launch configuration as JSON (as config.launch_fp64 or "
                          "config.launch_mixed of a bench line); default: the untuned one")
@@ -311,14 +336,19 @@ def main():
                           "patches": int(t.nblocks), "steps_with_eos_faults": faults}))
         return
     if args.kernels_only:
-        g, blocks = c3_setup(sph)
+        if args.workload == "c2":
+            g = sph.c2_grid()
+            blocks = sph.uniform_blocks(g, 16)
+        else:
+            g, blocks = c3_setup(sph)
         eng, run = make_engine(sph, g, blocks, {"fp64": 0, "mixed": 1, "fast": 2}[args.kernels_only])
         launch = dict(tune.DEFAULT, **json.loads(args.launch)) if args.launch else dict(tune.DEFAULT)
         tune.apply(eng, launch)
         eng.use_graph(False)
         eng.step([run], args.steps, want_dt=False)
         eng.sync()
-        print(json.dumps({"mode": "kernels-only", "precision": args.kernels_only, "steps": args.steps,
+        print(json.dumps({"mode": "kernels-only", "workload": args.workload, "precision": args.kernels_only,
+                          "steps": args.steps,
                           "blocks": int(len(blocks)), "zones": int(len(blocks)) * 4096,
                           "launch": launch}))
         return
