@@ -121,7 +121,7 @@ def measured_flops(precision):
 
 def timed(sph, tune, g, blocks, mixed, steps, warmup, tune_log):
     eng, run = make_engine(sph, g, blocks, mixed)
-    t = tune.tune(eng, [run], mixed, use_cache=False, log=tune_log)
+    t = tune.tune(eng, [run], mixed, steps=8, use_cache=False, log=tune_log)
     # fresh initial data for the timed pass
     run.init_torus(sph.TORUS, 42)
     eng.step([run], warmup, want_dt=False)
@@ -228,7 +228,7 @@ def c2_run(sph, tune, nsteps, timed_steps):
     eng = sph.Engine(16, len(blocks))
     run = eng.add_run(sph.PHYSICS, g, blocks)
     run.init_torus(sph.TORUS, 42)
-    cfg = tune.tune(eng, [run], False, use_cache=False)["config"]
+    cfg = tune.tune(eng, [run], False, steps=8, use_cache=False)["config"]
     run.init_torus(sph.TORUS, 42)
     d0 = run.diagnostics()
     eng.step([run], nsteps - timed_steps, want_dt=False)
