@@ -94,6 +94,19 @@ def ref_available() -> bool:
     return REF_SO.exists()
 
 
+def ref_one_thread(L, request):
+    """Run the compiled reference on one OpenMP thread until the calling test
+    ends.  omp_set_num_threads sets the count of the one OpenMP runtime that
+    the compiled reference, the oracle and the product share in this process,
+    so the previous count is put back at the end of the test; left at 1, every
+    later oracle step in the process would run on one thread."""
+    get = L.omp_get_max_threads
+    get.restype, get.argtypes = ctypes.c_int, []
+    n = get()
+    L.ref_set_threads(1)
+    request.addfinalizer(lambda: L.ref_set_threads(n))
+
+
 ADAPTER_SO = REF_SO.parent / "libbhadapter.so"
 _adapter = None
 

@@ -342,7 +342,7 @@ def test_step_with_other_limiters(lim):
 
 
 @pytest.mark.parametrize("name,nsteps", [("R4", 3), ("R3", 1)])
-def test_mode_b_adapter_steps_the_reference_tree(name, nsteps):
+def test_mode_b_adapter_steps_the_reference_tree(name, nsteps, request):
     """The reference-side drop-in (oracle/gpu_rk2_adapter.hpp, compiled
     against the reference's headers): a compiled-reference tree -- its own
     initial data, and for R3 its own regrid to two levels -- stepped by
@@ -351,13 +351,13 @@ def test_mode_b_adapter_steps_the_reference_tree(name, nsteps):
     the same state with the GPU's arithmetic, bit for bit, dt included; where
     the reference's equation of state throws (R3, step 1), the adapter throws
     std::domain_error after copying the state back."""
-    from oracle_libs import RefTree, adapter_lib, ref_available
+    from oracle_libs import RefTree, adapter_lib, ref_available, ref_one_thread
     if not ref_available():
         pytest.skip("oracle/_ref not built")
     import ctypes
     lo, hi, nqlt, regrid, a, Q = getattr(configs, name)
     rt = RefTree(lo, hi, nqlt, 1.0, a, Q, configs.GAMMA)
-    rt.L.ref_set_threads(1)
+    ref_one_thread(rt.L, request)
     rt.torus_init(regrid=regrid)
     s0 = rt.state()
     ot = OracleTree.from_state(s0, (1.0, a, Q, configs.GAMMA), libm=1)

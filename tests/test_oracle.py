@@ -20,7 +20,8 @@ import pytest
 
 import configs
 from ic_cases import ic_math_inputs
-from oracle_libs import (OracleTree, RefTree, State, bits_equal, first_diff, oracle_lib, ref_available)
+from oracle_libs import (OracleTree, RefTree, State, bits_equal, first_diff, oracle_lib, ref_available,
+                         ref_one_thread)
 
 GOLD = Path(__file__).resolve().parent / "golden"
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -168,14 +169,14 @@ needs_ref = pytest.mark.skipif(not ref_available(), reason="oracle/_ref not buil
 
 @needs_ref
 @pytest.mark.parametrize("name", ["R2", "R3"])
-def test_set_blocks_takes_the_reference_quilt(name):
+def test_set_blocks_takes_the_reference_quilt(name, request):
     """bh_tree_set_blocks on the compiled reference's own quilt after its
     regrid: the product's tree has the reference's block table and patch
     edges bit for bit, and takes its state unchanged."""
     import bhfluid
     lo, hi, nqlt, regrid, a, q = getattr(configs, name)
     rt = RefTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
-    rt.L.ref_set_threads(1)
+    ref_one_thread(rt.L, request)
     rt.torus_init(regrid=True)
     s = rt.state()
     t = bhfluid.AmrTree(lo, hi, nqlt, 1.0, a, q, configs.GAMMA)
